@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""MLUPS and HBM-roofline benchmark of the fused D3Q19 collide-and-stream step.
+
+Workload: the triply periodic 3-D Taylor-Green vortex at 256^3 (U0 = 0.05,
+Re = 1600 on L = N / 2 pi, tau = 3 nu + 1/2), initialised on the GPU
+(initField mode 2, Kernels.metal:414-438) and stepped by the HIP kernels in
+two variants: TRT Lambda = 3/16 with FP32 storage (the headline `value`) and
+SRT with FP16 shifted storage. A "step" is one LBM time step of all 256^3
+cells; the DDFs are resident in HBM before timing starts. MLUPS counts every
+cell, as the reference's bench does (Sources/strouhal/Cases.swift:141).
+
+The roofline is HBM: algorithmic bytes per cell per step are 19 DDF reads +
+19 DDF writes = 152 B (FP32) / 76 B (FP16). The step kernel's average launch
+duration comes from HIP events recorded on the library's stream around the
+timed steps (the same stream the kernels run on).
+
+cpu_baseline: the FP32 oracle (a scalar C++ restatement of the same update,
+oracle/strouhal_oracle.cpp) on the host cores for a bounded sample of the
+same workload.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+REFERENCE_M5_FP32_MLUPS = 785.0   # README.md:24-28 of the reference (Apple M5, 256^3 FP32)
+
+
+def tgv_params(n: int, u0: float = 0.05, re: float = 1600.0):
+    L = n / (2.0 * math.pi)
+    nu = u0 * L / re
+    return 3.0 * nu + 0.5
+
+
+def measure(precision: str, lam, n: int, steps: int, warmup: int):
+    from strouhal_amd import Simulation, trt_omegas
+    tau = tgv_params(n)
+    wp, wm = trt_omegas(tau, lam)
+    sim = Simulation(None, precision, nx=n, ny=n, nz=n, omega=wp, omega_minus=wm)
+    sim.init_field(2, 0.05)
+    sim.run(warmup)
+    g0 = sim.gpu_seconds
+    t0 = time.perf_counter()
+    sim.run(steps)
+    wall = time.perf_counter() - t0
+    gpu = sim.gpu_seconds - g0
+    ke = sim.kinetic_energy()
+    cells = n ** 3
+    sim.close()
+    return {
+        "precision": precision,
+        "collision": "SRT" if lam is None else f"TRT(Lambda={lam:g})",
+        "tau": tau,
+        "mlups": cells * steps / wall / 1e6,
+        "mlups_gpu_events": cells * steps / gpu / 1e6,
+        "ms_per_step": wall / steps * 1e3,
+        "kernel_ms": gpu / steps * 1e3,
+        "finite": bool(math.isfinite(ke)),
+        "bytes_per_cell": 152 if precision == "fp32" else 76,
+    }
+
+
+def cpu_baseline(n: int, sample_steps: int):
+    """FP32 oracle on the host cores: `sample_steps` steps of the same 256^3 TGV."""
+    from oracle.oracle import OracleSim
+    from strouhal_amd import trt_omegas
+    threads = min(16, os.cpu_count() or 1)
+    tau = tgv_params(n)
+    wp, wm = trt_omegas(tau, 3.0 / 16.0)
+    sim = OracleSim(n, n, n, wp, wm, arith="f32", storage="f32", threads=threads)
+    sim.init_field(2, 0.05)
+    sim.run(2)   # warm (page in)
+    t0 = time.perf_counter()
+    sim.run(sample_steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n ** 3 * sample_steps / dt / 1e6,
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP32 oracle, {n}^3 TGV TRT(3/16), {sample_steps} steps after 2 warm-up steps, "
+                  f"{threads} OpenMP threads, {dt:.1f} s",
+    }
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "step_kernel_traffic_tgv256.json")
+    if not os.path.exists(p):
+        return None, None
+    with open(p) as fh:
+        d = json.load(fh)
+    return d.get("fp32_bytes_per_launch"), d
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cpu-sample-steps", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU z-slab sharding is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    steps = args.steps + (args.steps & 1)      # even: natural slot order at the end
+    warmup = args.warmup + (args.warmup & 1)
+    n = args.n
+
+    fp32 = measure("fp32", 3.0 / 16.0, n, steps, warmup)
+    fp16 = measure("fp16s", None, n, steps, warmup)
+
+    cells = n ** 3
+    achieved = fp32["bytes_per_cell"] * cells / (fp32["kernel_ms"] * 1e-3) / 1e9
+    traffic, _ = load_traffic()
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+    }
+    fp16_achieved = fp16["bytes_per_cell"] * cells / (fp16["kernel_ms"] * 1e-3) / 1e9
+    cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_sample_steps)
+    line = {
+        "metric": "MLUPS + % HBM roofline, D3Q19 256³ FP32/FP16; adjoint/forward step-time ratio",
+        "value": round(fp32["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(fp32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(fp32["mlups"] / REFERENCE_M5_FP32_MLUPS, 2),
+        "dtype": "f32",
+        "data": "synthetic (analytic 3-D Taylor-Green initial field)",
+        "config": {
+            "workload": f"3-D Taylor-Green vortex {n}^3 periodic, U0=0.05, Re=1600, "
+                        f"TRT Lambda=3/16, FP32 storage (headline); SRT FP16 shifted storage",
+            "lattice": f"D3Q19 {n}x{n}x{n}",
+            "cells": cells,
+            "tau": round(fp32["tau"], 6),
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in fp32.items()},
+        "fp16s": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in fp16.items()},
+        "fp16s_roofline": {"achieved": round(fp16_achieved, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(fp16_achieved / HBM_PEAK_GBS, 4)},
+        "vs_baseline_ref": "reference README.md:24-28, Apple M5 256^3 FP32 ~785 MLUPS",
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * strouhal_hip.h — C ABI of the MI355X (gfx950) D3Q19 lattice-Boltzmann
+ * solver: the drop-in replacement for the Metal backend behind StrouhalCore's
+ * `GPU` + `Simulation` (Sources/StrouhalCore/Sim.swift:60-459).
+ *
+ * Conventions
+ *  - Every entry point returns an int status (STRH_OK = 0). Nothing aborts.
+ *    On failure strh_last_error() returns a thread-local message; this replaces
+ *    Swift's `throws StrouhalError` and the `precondition` traps
+ *    (Sim.swift:127-136, 335, 360, 414).
+ *  - Handles own their device memory; inputs are copied at call time; output
+ *    buffers belong to the caller (the reference returns views of shared
+ *    memory, Sim.swift:351-352 — HBM is discrete here, so the ABI copies).
+ *  - One lattice is not thread-safe; several contexts may coexist in one
+ *    process (the reference app runs a second `GPU()`, StrouhalApp.swift:846).
+ *    Each context owns one HIP stream at default priority.
+ *  - Cell index n = (z*ny + y)*nx + x; DDF slot order i*N + n
+ *    (Kernels.metal:71, 243-246). D3Q19 direction order as Kernels.metal:39-47.
+ */
+#ifndef STROUHAL_HIP_H
+#define STROUHAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define STRH_API __attribute__((visibility("default")))
+#else
+#define STRH_API
+#endif
+
+typedef struct strh_ctx strh_ctx;
+typedef struct strh_lattice strh_lattice;
+
+/* Status codes. */
+enum {
+    STRH_OK = 0,
+    STRH_ERR_INVALID = 1,     /* bad argument (null handle, size, range, tau) */
+    STRH_ERR_NO_DEVICE = 2,   /* Sim.swift:127-135 StrouhalError.noDevice */
+    STRH_ERR_DEVICE = 3,      /* HIP runtime / kernel failure (Sim.swift:328-330) */
+    STRH_ERR_ALLOC = 4,       /* device or host allocation failed (Sim.swift:221) */
+    STRH_ERR_ODD_STEPS = 5,   /* probe needs an even step count (Sim.swift:335, 360) */
+    STRH_ERR_NO_FORCES = 6,   /* probe_force on a lattice created without forces */
+    STRH_ERR_NOT_FINITE = 7   /* a reduction produced a non-finite value */
+};
+
+/* Storage precision. Sim.swift:5-8 has fp32 and fp16s; bf16 is new. */
+enum { STRH_FP32 = 0, STRH_FP16S = 1, STRH_BF16 = 2 };
+
+/* Cell flags, Sim.swift:46-52 / Kernels.metal:65-69 (raw values). */
+enum { STRH_FLUID = 0, STRH_SOLID = 1, STRH_LID = 2, STRH_INFLOW = 3, STRH_OUTFLOW = 4 };
+
+/* Lattice construction parameters: the arguments of Simulation.init
+ * (Sim.swift:195-200). omega_minus < 0 means "same as omega" (SRT), the
+ * `omegaMinus: Float? = nil` default. */
+typedef struct strh_lattice_desc {
+    int32_t precision;        /* STRH_FP32 / STRH_FP16S / STRH_BF16 */
+    int32_t nx, ny, nz;
+    float omega;
+    float omega_minus;
+    float lid[3];
+    float uin;
+    int64_t ramp_steps;
+    float force[3];
+    float c_smago;            /* Smagorinsky Cs^2; 0 = LES off */
+    int32_t wants_forces;
+} strh_lattice_desc;
+
+/* ---- context: replaces GPU.init (Sim.swift:67-78) ----------------------- */
+STRH_API int strh_ctx_create(int device_ordinal, strh_ctx** out);
+STRH_API int strh_ctx_destroy(strh_ctx* ctx);
+STRH_API const char* strh_last_error(void);
+STRH_API int strh_device_name(strh_ctx* ctx, char* out, size_t cap);
+
+/* ---- lattice: replaces Simulation.init (Sim.swift:195-256) --------------
+ * `flags` holds nx*ny*nz raw Cell values in cell-index order; it replaces the
+ * Swift `(Int, Int, Int) -> Cell` closure. The neighbour masks of
+ * Sim.swift:235-255 are built by the library. The DDFs start at the shifted
+ * rest state 0 (Sim.swift:225). */
+STRH_API int strh_lattice_create(strh_ctx* ctx, const strh_lattice_desc* desc,
+                                 const uint8_t* flags, strh_lattice** out);
+STRH_API int strh_lattice_destroy(strh_lattice* lat);
+STRH_API int strh_lattice_cells(strh_lattice* lat, int64_t* cells);
+/* Bytes of the canonical DDF image (19 * N * storage bytes). */
+STRH_API int strh_ddf_bytes(strh_lattice* lat, size_t* bytes);
+
+/* ---- mutable public vars, Sim.swift:164-172 ------------------------------ */
+STRH_API int strh_set_omega(strh_lattice* lat, float omega, float omega_minus);
+STRH_API int strh_set_lid_velocity(strh_lattice* lat, float ux, float uy, float uz);
+STRH_API int strh_set_uin(strh_lattice* lat, float uin_target);
+STRH_API int strh_set_ramp_steps(strh_lattice* lat, int64_t ramp_steps);
+STRH_API int strh_set_force(strh_lattice* lat, float fx, float fy, float fz);
+STRH_API int strh_set_c_smago(strh_lattice* lat, float c_smago);
+STRH_API int strh_set_inflow_uniform(strh_lattice* lat, int enabled);
+/* sponge = (x0, width, tau); enabled = 0 restores `sponge = nil`. */
+STRH_API int strh_set_sponge(strh_lattice* lat, int enabled, float x0, float width, float tau);
+/* Noble-Torczynski solid fractions, nx*ny*nz floats (Sim.swift:413-421). */
+STRH_API int strh_set_solid_fractions(strh_lattice* lat, const float* eps);
+
+/* ---- initialisation: Simulation.initField (Sim.swift:424-439) ------------
+ * mode 0 rest, 1 2-D Taylor-Green, 2 3-D Taylor-Green vortex; tau = 1/omega. */
+STRH_API int strh_init_field(strh_lattice* lat, uint32_t mode, float amplitude);
+
+/* ---- stepping: Simulation.run (Sim.swift:288-331) ------------------------ */
+STRH_API int strh_run(strh_lattice* lat, int64_t steps);
+STRH_API int strh_steps_done(strh_lattice* lat, int64_t* steps);
+/* Accumulated GPU time of run(), Sim.swift:176-178. */
+STRH_API int strh_gpu_seconds(strh_lattice* lat, double* seconds);
+
+/* ---- observables ----------------------------------------------------------
+ * probe_moments: Sim.swift:334-353 (Kernels.metal:359-380). `out` receives
+ * 4*N floats (ux, uy, uz, rho) per cell. STRH_ERR_ODD_STEPS after an odd count. */
+STRH_API int strh_probe_moments(strh_lattice* lat, float* out);
+/* probe_force: Sim.swift:359-372. Advances 2 steps (force on the odd pass),
+ * then a fixed-order FP64 sum over z in [0,nz), y in [y0,y1], x in [x0,x1]
+ * (inclusive ranges). */
+STRH_API int strh_probe_force(strh_lattice* lat, int x0, int x1, int y0, int y1,
+                              double out[3]);
+/* stateDigest: Sim.swift:441-444 — SHA-256 of the 19*N*B raw DDF bytes in
+ * slot order i*N + n. */
+STRH_API int strh_state_digest(strh_lattice* lat, uint8_t out[32]);
+/* massSum: Sim.swift:447-458 — FP64 sum of all 19*N shifted DDFs (equals
+ * sum(rho - 1)), reduced on the GPU in a fixed order. */
+STRH_API int strh_mass_sum(strh_lattice* lat, double* out);
+/* New: FP64 sum over cells of |u|^2 (u as probe_moments computes it, the
+ * per-cell term in FP32 as Cases.swift:1242), reduced on the GPU in a fixed
+ * order. Requires an even step count. */
+STRH_API int strh_kinetic_energy(strh_lattice* lat, double* out);
+
+/* ---- raw state (the self-test writes fBuf directly, Cases.swift:90-99) --- */
+STRH_API int strh_download_ddf(strh_lattice* lat, void* out, size_t bytes);
+STRH_API int strh_upload_ddf(strh_lattice* lat, const void* in, size_t bytes);
+/* Per-cell momentum-exchange field of the last force step (4*N floats). */
+STRH_API int strh_download_force(strh_lattice* lat, float* out);
+
+/* ---- checkpoint / resume (new): device-side snapshot of the DDFs and the
+ * step count; restore rewinds both. One snapshot slot per lattice. */
+STRH_API int strh_snapshot(strh_lattice* lat);
+STRH_API int strh_restore(strh_lattice* lat);
+
+/* ---- host utilities (no counterpart in the reference, which hashes with
+ * CryptoKit): SHA-256 of a host buffer, and a switch that forces the portable
+ * block function instead of the x86 SHA extensions (returns 1 when the SHA
+ * extensions are in use after the call). */
+STRH_API int strh_sha256(const void* data, size_t len, uint8_t out[32]);
+STRH_API int strh_sha256_force_portable(int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STROUHAL_HIP_H */

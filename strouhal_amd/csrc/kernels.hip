@@ -1,0 +1,713 @@
+// D3Q19 lattice-Boltzmann device kernels for gfx950 (MI355X).
+//
+// The hot path is `step_kernel`: one AA-pattern time step, collide and stream
+// fused, in place (the reference's `step` + `collide`,
+// Sources/StrouhalCore/Kernels.metal:105-355). Design for CDNA4:
+//  * SoA DDF slots f[i*S + n] with the slot stride S padded to 64 cells, so
+//    every wave's load of one slot is one run of 64 consecutive values
+//    (256 B fp32 / 128 B fp16) — the even pass is perfectly coalesced; the
+//    odd pass shifts x by +-1 cell, which the L2 absorbs because one
+//    workgroup covers whole x-runs.
+//  * One thread per cell, 256-thread workgroups, block ids remapped so
+//    consecutive cell ranges land on the same XCD (each of the 8 XCDs has its
+//    own L2 and the hardware deals workgroups round-robin over them).
+//  * Specialised per (storage, parity, walls, NT, LES, forcing): the all-fluid
+//    periodic bulk reads and writes exactly 19 + 19 DDFs per cell and
+//    nothing else (152 B fp32 / 76 B fp16 per cell).
+//  * FP32 arithmetic in the reference's operation order, no contraction
+//    (built with -ffp-contract=off); the only fused multiply-add is the one
+//    the reference writes (Kernels.metal:181). Terms the reference multiplies
+//    by a zero lattice-velocity component are dropped at compile time; the
+//    partial sums they would be added to can never be -0 (each starts as
+//    +0 + x), so dropping them leaves every result bit unchanged.
+//  * No atomics anywhere; reductions run in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace strh {
+
+// ---- storage -------------------------------------------------------------
+struct StF32 {
+    using T = float;
+    __device__ static float ld(T v) { return v; }
+    __device__ static T st(float v) { return v; }
+};
+struct StF16 {
+    using T = _Float16;
+    __device__ static float ld(T v) { return (float)v; }       // exact widening
+    // IEEE RNE of the binary32 value. The empty asm pins v as a rounded FP32
+    // register: without it the compiler fuses the producing fmaf and this
+    // conversion into one v_fma_mixlo_f16 (a single rounding to f16), which
+    // differs from the reference's (half)(float) double rounding.
+    __device__ static T st(float v) {
+        asm("" : "+v"(v));
+        return (_Float16)v;
+    }
+};
+struct StBF16 {
+    using T = uint16_t;
+    __device__ static float ld(T v) { return __uint_as_float((uint32_t)v << 16); }
+    __device__ static T st(float v) {                          // IEEE RNE
+        uint32_t u = __float_as_uint(v);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (uint16_t)(u >> 16);
+    }
+};
+
+// ---- index helpers ---------------------------------------------------------
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv d) {
+    return (__umulhi(n, d.m) + n) >> d.s;
+}
+
+// Workgroup b of G runs on XCD b % 8. Give each XCD a contiguous range of
+// logical blocks so neighbouring cell ranges share an L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
+    const uint32_t q = G >> 3, r = G & 7u, xcd = b & 7u, k = b >> 3;
+    return xcd < r ? xcd * (q + 1u) + k : r * (q + 1u) + (xcd - r) * q + k;
+}
+
+template <int C> __device__ __forceinline__ int pick(int m, int p) {
+    return C < 0 ? m : (C > 0 ? p : 0);
+}
+
+// s + c*v for c in {-1, 0, 1}, as the reference's `s += (float)C * v` when
+// s can never be -0 (dropping +-0 terms and writing (-1)*v as a subtraction
+// are then exact).
+template <int C> __device__ __forceinline__ float acc(float s, float v) {
+    return C > 0 ? s + v : (C < 0 ? s - v : s);
+}
+
+// (cx*ux + cy*uy) + cz*uz with the zero terms dropped (c.u of direction i).
+template <int I> __device__ __forceinline__ float cdot(float ux, float uy, float uz) {
+    constexpr int a = kCx[I], b = kCy[I], c = kCz[I];
+    float s;
+    if (a != 0) {
+        s = a > 0 ? ux : -ux;
+        s = acc<b>(s, uy);
+        s = acc<c>(s, uz);
+    } else if (b != 0) {
+        s = b > 0 ? uy : -uy;
+        s = acc<c>(s, uz);
+    } else {
+        s = c > 0 ? uz : -uz;
+    }
+    return s;
+}
+
+template <int I> __device__ __forceinline__ constexpr float wgt() {
+    return I == 0 ? (1.0f / 3.0f) : (I <= 6 ? (1.0f / 18.0f) : (1.0f / 36.0f));
+}
+
+// Moving-wall velocity, Kernels.metal:81-87.
+__device__ __forceinline__ void wall_vel(uint8_t wf, int wy, const StepParams& p,
+                                         float& ux, float& uy, float& uz) {
+    if (wf == kLid) { ux = p.ulidX; uy = p.ulidY; uz = p.ulidZ; return; }
+    uy = 0.0f; uz = 0.0f;
+    if (p.inflowUniform != 0) { ux = p.uin; return; }
+    const float H = (float)p.ny - 2.0f;
+    const float yd = (float)wy - 0.5f;
+    const float v = 4.0f * p.uin * yd * (H - yd) / (H * H);
+    ux = v > 0.0f ? v : 0.0f;
+}
+
+// ---- collide: Kernels.metal:105-228 -------------------------------------
+template <bool LES> __device__ __forceinline__ void les_rates(
+    const float (&fh)[19], const StepParams& p, float rhom1, float rho, float inv,
+    float ux, float uy, float uz, float u2, float& wp, float& wm) {
+    float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
+#define STRH_LES_TERM(I)                                                              \
+    {                                                                                 \
+        const float cu = 3.0f * cdot<I>(ux, uy, uz);                                  \
+        const float feq = wgt<I>() * (rhom1 + rho * (cu + 0.5f * cu * cu - 1.5f * u2)); \
+        const float d = fh[I] - feq;                                                  \
+        pxx = acc<kCx[I] * kCx[I]>(pxx, d);                                           \
+        pyy = acc<kCy[I] * kCy[I]>(pyy, d);                                           \
+        pzz = acc<kCz[I] * kCz[I]>(pzz, d);                                           \
+        pxy = acc<kCx[I] * kCy[I]>(pxy, d);                                           \
+        pxz = acc<kCx[I] * kCz[I]>(pxz, d);                                           \
+        pyz = acc<kCy[I] * kCz[I]>(pyz, d);                                           \
+    }
+    STRH_LES_TERM(1) STRH_LES_TERM(2) STRH_LES_TERM(3) STRH_LES_TERM(4) STRH_LES_TERM(5)
+    STRH_LES_TERM(6) STRH_LES_TERM(7) STRH_LES_TERM(8) STRH_LES_TERM(9) STRH_LES_TERM(10)
+    STRH_LES_TERM(11) STRH_LES_TERM(12) STRH_LES_TERM(13) STRH_LES_TERM(14) STRH_LES_TERM(15)
+    STRH_LES_TERM(16) STRH_LES_TERM(17) STRH_LES_TERM(18)
+#undef STRH_LES_TERM
+    const float Q = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz
+                          + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const float tau0 = 1.0f / wp;
+    const float tauEff = 0.5f * (tau0 + sqrtf(tau0 * tau0 + 25.455844f * p.cSmago * Q * inv));
+    if (p.omegaMinus == p.omega) {
+        wp = 1.0f / tauEff;
+        wm = wp;
+    } else {
+        const float lambda = (1.0f / wp - 0.5f) * (1.0f / wm - 0.5f);
+        wp = 1.0f / tauEff;
+        wm = 1.0f / (0.5f + lambda / (tauEff - 0.5f));
+    }
+}
+
+// One symmetric/antisymmetric pair (I, I+1) of the eps == 0 path
+// (Kernels.metal:183-197).
+template <int I, bool FORCE> __device__ __forceinline__ void pair_golden(
+    float (&fh)[19], const StepParams& p, float rhom1, float rho, float ux, float uy,
+    float uz, float u2, float wp, float wm, float ap, float am, float uF) {
+    constexpr float W = wgt<I>();
+    const float cu = 3.0f * cdot<I>(ux, uy, uz);
+    const float feqp = W * (rhom1 + rho * (0.5f * cu * cu - 1.5f * u2));
+    const float feqm = W * rho * cu;
+    const float fp = 0.5f * (fh[I] + fh[I + 1]);
+    const float fm = 0.5f * (fh[I] - fh[I + 1]);
+    const float dp = wp * (feqp - fp);
+    const float dm = wm * (feqm - fm);
+    if (FORCE) {
+        const float sm = W * am * p.threeCF[I >> 1];
+        const float sp = W * ap * (3.0f * cu * p.cF[I >> 1] - 3.0f * uF);
+        fh[I] += dp + dm + sp + sm;
+        fh[I + 1] += dp - dm + sp - sm;
+    } else {
+        fh[I] += dp + dm;
+        fh[I + 1] += dp - dm;
+    }
+}
+
+// One pair of the Noble-Torczynski path (Kernels.metal:210-227).
+template <int I, bool FORCE> __device__ __forceinline__ void pair_nt(
+    float (&fh)[19], const StepParams& p, float rhom1, float rho, float ux, float uy,
+    float uz, float u2, float wp, float wm, float ap, float am, float uF, float B,
+    float oneMinusB, float& ntx, float& nty, float& ntz) {
+    constexpr float W = wgt<I>();
+    const float cu = 3.0f * cdot<I>(ux, uy, uz);
+    const float feqp = W * (rhom1 + rho * (0.5f * cu * cu - 1.5f * u2));
+    const float feqm = W * rho * cu;
+    const float fp = 0.5f * (fh[I] + fh[I + 1]);
+    const float fm = 0.5f * (fh[I] - fh[I + 1]);
+    const float dp = wp * (feqp - fp);
+    const float dm = wm * (feqm - fm);
+    const float bounce = B * (fh[I + 1] - fh[I]);
+    if (FORCE) {
+        const float sm = W * am * p.threeCF[I >> 1];
+        const float sp = W * ap * (3.0f * cu * p.cF[I >> 1] - 3.0f * uF);
+        fh[I] += oneMinusB * (dp + dm + sp + sm) + bounce;
+        fh[I + 1] += oneMinusB * (dp - dm + sp - sm) - bounce;
+    } else {
+        fh[I] += oneMinusB * (dp + dm) + bounce;
+        fh[I + 1] += oneMinusB * (dp - dm) - bounce;
+    }
+    // ntF -= 2 c_i bounce; (2*(-1))*b == -(2*b) exactly.
+    const float tb = 2.0f * bounce;
+    ntx = acc<-kCx[I]>(ntx, tb);
+    nty = acc<-kCy[I]>(nty, tb);
+    ntz = acc<-kCz[I]>(ntz, tb);
+}
+
+template <bool LES, bool FORCE, bool NT>
+__device__ __forceinline__ void collide(float (&fh)[19], const StepParams& p, int x,
+                                        float eps, float& ntx, float& nty, float& ntz) {
+    float wp = p.omega, wm = p.omegaMinus;
+    if ((float)x > p.spongeX0) {                                   // 112-123
+        const float frac = fminf(((float)x - p.spongeX0) * p.spongeInvW, 1.0f);
+        const float tau0 = 1.0f / wp;
+        const float tauS = tau0 + frac * (p.spongeTau - tau0);
+        if (p.omegaMinus == p.omega) {
+            wp = 1.0f / tauS;
+            wm = wp;
+        } else {
+            const float lambda = (1.0f / wp - 0.5f) * (1.0f / wm - 0.5f);
+            wp = 1.0f / tauS;
+            wm = 1.0f / (0.5f + lambda / (tauS - 0.5f));
+        }
+    }
+    float rhom1 = 0.0f;                                            // 124-128
+#pragma unroll
+    for (int i = 0; i < 19; i++) rhom1 += fh[i];
+    const float rho = 1.0f + rhom1;
+    const float inv = 1.0f / rho;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;                         // 129-134
+#define STRH_MOM(I)                        \
+    px = acc<kCx[I]>(px, fh[I]);           \
+    py = acc<kCy[I]>(py, fh[I]);           \
+    pz = acc<kCz[I]>(pz, fh[I]);
+    STRH_MOM(1) STRH_MOM(2) STRH_MOM(3) STRH_MOM(4) STRH_MOM(5) STRH_MOM(6)
+    STRH_MOM(7) STRH_MOM(8) STRH_MOM(9) STRH_MOM(10) STRH_MOM(11) STRH_MOM(12)
+    STRH_MOM(13) STRH_MOM(14) STRH_MOM(15) STRH_MOM(16) STRH_MOM(17) STRH_MOM(18)
+#undef STRH_MOM
+    float ux, uy, uz;                                              // 135-138
+    if (FORCE) {
+        ux = (px + 0.5f * p.fx) * inv;
+        uy = (py + 0.5f * p.fy) * inv;
+        uz = (pz + 0.5f * p.fz) * inv;
+    } else {  // px + 0.5f*0 == px: px is never -0
+        ux = px * inv;
+        uy = py * inv;
+        uz = pz * inv;
+    }
+    const float u2 = ux * ux + uy * uy + uz * uz;
+    if (LES) {                                                     // 140-167
+        if (p.cSmago > 0.0f) les_rates<LES>(fh, p, rhom1, rho, inv, ux, uy, uz, u2, wp, wm);
+    }
+    float uF = 0.0f, ap = 0.0f, am = 0.0f;                         // 169-171
+    if (FORCE) {
+        uF = ux * p.fx + uy * p.fy + uz * p.fz;
+        ap = 1.0f - 0.5f * wp;
+        am = 1.0f - 0.5f * wm;
+    }
+    const float feq0 = (1.0f / 3.0f) * (rhom1 - 1.5f * rho * u2);
+    if (!NT || eps == 0.0f) {                                      // 177-199
+        if (FORCE) {
+            const float s0 = (1.0f / 3.0f) * ap * (-3.0f * uF);
+            fh[0] = fmaf(wp, feq0 - fh[0], fh[0]) + s0;
+        } else {
+            fh[0] = fmaf(wp, feq0 - fh[0], fh[0]);
+        }
+#define STRH_PAIR(I) pair_golden<I, FORCE>(fh, p, rhom1, rho, ux, uy, uz, u2, wp, wm, ap, am, uF);
+        STRH_PAIR(1) STRH_PAIR(3) STRH_PAIR(5) STRH_PAIR(7) STRH_PAIR(9)
+        STRH_PAIR(11) STRH_PAIR(13) STRH_PAIR(15) STRH_PAIR(17)
+#undef STRH_PAIR
+        return;
+    }
+    if (NT) {                                                      // 201-227
+        const float tw = 1.0f / wp - 0.5f;
+        const float B = eps * tw / ((1.0f - eps) + tw);
+        const float oneMinusB = 1.0f - B;
+        if (FORCE) {
+            const float s0 = (1.0f / 3.0f) * ap * (-3.0f * uF);
+            fh[0] += oneMinusB * (wp * (feq0 - fh[0]) + s0);
+        } else {
+            fh[0] += oneMinusB * (wp * (feq0 - fh[0]));
+        }
+#define STRH_PAIR(I) pair_nt<I, FORCE>(fh, p, rhom1, rho, ux, uy, uz, u2, wp, wm, ap, am, uF, B, oneMinusB, ntx, nty, ntz);
+        STRH_PAIR(1) STRH_PAIR(3) STRH_PAIR(5) STRH_PAIR(7) STRH_PAIR(9)
+        STRH_PAIR(11) STRH_PAIR(13) STRH_PAIR(15) STRH_PAIR(17)
+#undef STRH_PAIR
+    }
+}
+
+// ---- the fused step: Kernels.metal:230-355 ------------------------------
+struct Offsets {
+    int xm, xp, ym, yp, zm, zp;
+    template <int CX, int CY, int CZ> __device__ __forceinline__ int at() const {
+        return pick<CX>(xm, xp) + pick<CY>(ym, yp) + pick<CZ>(zm, zp);
+    }
+};
+
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+__global__ void __launch_bounds__(256)
+step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
+            const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
+            float4* __restrict__ force, const float* __restrict__ epsBuf, StepParams p) {
+    using T = typename St::T;
+    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * 256u + threadIdx.x;
+    if (n >= p.N) return;
+    const size_t S = p.S;
+    T* fs[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * S;
+
+    float eps = 0.0f;
+    float fh[19];
+    float ntx = 0.0f, nty = 0.0f, ntz = 0.0f;
+
+    if (PARITY == 0) {                                             // 252-260
+        if (WALLS && flags[n] != kFluid) return;
+        if (NT && p.useEps) eps = epsBuf[n];
+        const uint32_t t = fdiv(n, p.divx);
+        const int x = (int)(n - t * (uint32_t)p.nx);
+#pragma unroll
+        for (int i = 0; i < 19; i++) fh[i] = St::ld(fs[i][n]);
+        collide<LES, FORCE, NT>(fh, p, x, eps, ntx, nty, ntz);
+        fs[0][n] = St::st(fh[0]);
+#pragma unroll
+        for (int i = 1; i < 19; i++) fs[kOpp[i]][n] = St::st(fh[i]);
+        if (NT && p.writeForce != 0 && eps > 0.0f) force[n] = make_float4(ntx, nty, ntz, 0.0f);
+        return;
+    }
+
+    // -------- odd: streaming pass (neighbours + walls), 261-354 --------
+    uint32_t sM = 0, lM = 0;
+    if (WALLS) {
+        const uint32_t lw = link[n];
+        if (lw & kNotFluidBit) return;
+        sM = lw & 0x3ffffu;
+        if (sM != 0u) lM = lidm[n];
+    }
+    if (NT && p.useEps) eps = epsBuf[n];
+    const uint32_t t = fdiv(n, p.divx);
+    const int x = (int)(n - t * (uint32_t)p.nx);
+    const uint32_t z = fdiv(t, p.divy);
+    const int y = (int)(t - z * (uint32_t)p.ny);
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int plane = nx * ny;
+    Offsets o;
+    o.xm = (x == 0) ? nx - 1 : -1;
+    o.xp = (x == nx - 1) ? 1 - nx : 1;
+    o.ym = (y == 0) ? (ny - 1) * nx : -nx;
+    o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
+    o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
+    o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
+    const int ni = (int)n;
+
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    uint32_t deferred = 0u;
+    fh[0] = St::ld(fs[0][n]);
+#define STRH_PULL(I)                                                                  \
+    {                                                                                 \
+        constexpr int srcBit = kOpp[I] - 1;                                           \
+        if (WALLS && ((sM >> srcBit) & 1u)) {                                         \
+            fh[I] = St::ld(fs[I][n]);                                                 \
+            if ((lM >> srcBit) & 1u) deferred |= 1u << I;                             \
+            if (p.writeForce != 0) {                                                  \
+                ax = acc<-kCx[I]>(ax, fh[I]);                                         \
+                ay = acc<-kCy[I]>(ay, fh[I]);                                         \
+                az = acc<-kCz[I]>(az, fh[I]);                                         \
+            }                                                                         \
+        } else {                                                                      \
+            fh[I] = St::ld(fs[kOpp[I]][(uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>())]); \
+        }                                                                             \
+    }
+    STRH_PULL(1) STRH_PULL(2) STRH_PULL(3) STRH_PULL(4) STRH_PULL(5) STRH_PULL(6)
+    STRH_PULL(7) STRH_PULL(8) STRH_PULL(9) STRH_PULL(10) STRH_PULL(11) STRH_PULL(12)
+    STRH_PULL(13) STRH_PULL(14) STRH_PULL(15) STRH_PULL(16) STRH_PULL(17) STRH_PULL(18)
+#undef STRH_PULL
+
+    if (WALLS && deferred != 0u) {                                 // 292-314
+#define STRH_DEFER(I)                                                                 \
+        if ((deferred >> I) & 1u) {                                                   \
+            const int sy = y - kCy[I] < 0 ? ny - 1 : (y - kCy[I] >= ny ? 0 : y - kCy[I]); \
+            const uint32_t s = (uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>());    \
+            const uint8_t wf = flags[s];                                              \
+            if (wf == kOutflow) {                                                     \
+                fh[I] = -fh[I];                                                       \
+            } else {                                                                  \
+                float uwx, uwy, uwz;                                                  \
+                wall_vel(wf, sy, p, uwx, uwy, uwz);                                   \
+                const float cu = (float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz; \
+                const float corr = 6.0f * wgt<I>() * cu;                              \
+                fh[I] += corr;                                                        \
+                if (p.writeForce != 0) {                                              \
+                    ax = acc<-kCx[I]>(ax, corr);                                      \
+                    ay = acc<-kCy[I]>(ay, corr);                                      \
+                    az = acc<-kCz[I]>(az, corr);                                      \
+                }                                                                     \
+            }                                                                         \
+        }
+        STRH_DEFER(1) STRH_DEFER(2) STRH_DEFER(3) STRH_DEFER(4) STRH_DEFER(5) STRH_DEFER(6)
+        STRH_DEFER(7) STRH_DEFER(8) STRH_DEFER(9) STRH_DEFER(10) STRH_DEFER(11) STRH_DEFER(12)
+        STRH_DEFER(13) STRH_DEFER(14) STRH_DEFER(15) STRH_DEFER(16) STRH_DEFER(17) STRH_DEFER(18)
+#undef STRH_DEFER
+    }
+
+    collide<LES, FORCE, NT>(fh, p, x, eps, ntx, nty, ntz);            // 315-316
+    if (NT) {
+        ax += ntx;
+        ay += nty;
+        az += ntz;
+    }
+    fs[0][n] = St::st(fh[0]);                                      // 317-350
+#define STRH_PUSH(I)                                                                  \
+    {                                                                                 \
+        constexpr int dstBit = I - 1;                                                 \
+        if (WALLS && ((sM >> dstBit) & 1u)) {                                         \
+            float corr = 0.0f;                                                        \
+            float sign = 1.0f;                                                        \
+            if ((lM >> dstBit) & 1u) {                                                \
+                const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
+                const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>());   \
+                const uint8_t wf = flags[d];                                          \
+                if (wf == kOutflow) {                                                 \
+                    sign = -1.0f;                                                     \
+                } else {                                                              \
+                    float uwx, uwy, uwz;                                              \
+                    wall_vel(wf, dy, p, uwx, uwy, uwz);                               \
+                    corr = 6.0f * wgt<I>()                                            \
+                         * ((float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz); \
+                }                                                                     \
+            }                                                                         \
+            fs[kOpp[I]][n] = St::st(sign * fh[I] - corr);                             \
+            if (p.writeForce != 0) {                                                  \
+                const float v = sign * fh[I] + 2.0f * wgt<I>();                       \
+                ax = acc<kCx[I]>(ax, v);                                              \
+                ay = acc<kCy[I]>(ay, v);                                              \
+                az = acc<kCz[I]>(az, v);                                              \
+            }                                                                         \
+        } else {                                                                      \
+            fs[I][(uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>())] = St::st(fh[I]);   \
+        }                                                                             \
+    }
+    STRH_PUSH(1) STRH_PUSH(2) STRH_PUSH(3) STRH_PUSH(4) STRH_PUSH(5) STRH_PUSH(6)
+    STRH_PUSH(7) STRH_PUSH(8) STRH_PUSH(9) STRH_PUSH(10) STRH_PUSH(11) STRH_PUSH(12)
+    STRH_PUSH(13) STRH_PUSH(14) STRH_PUSH(15) STRH_PUSH(16) STRH_PUSH(17) STRH_PUSH(18)
+#undef STRH_PUSH
+    if (p.writeForce != 0 && (sM != 0u || eps > 0.0f))               // 351-353
+        force[n] = make_float4(ax, ay, az, 0.0f);
+}
+
+// ---- moments probe: Kernels.metal:359-380 (literal loop order) ----------
+template <class St>
+__global__ void __launch_bounds__(256)
+moments_kernel(const typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
+               float4* __restrict__ out, uint32_t N, uint32_t S, float fx, float fy, float fz) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N) return;
+    if (flags[n] != kFluid) { out[n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); return; }
+    float rhom1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const float v = St::ld(f[(size_t)i * S + n]);
+        rhom1 += v;
+        px += (float)kCx[i] * v;
+        py += (float)kCy[i] * v;
+        pz += (float)kCz[i] * v;
+    }
+    const float rho = 1.0f + rhom1;
+    out[n] = make_float4((px + 0.5f * fx) / rho, (py + 0.5f * fy) / rho,
+                         (pz + 0.5f * fz) / rho, rho);
+}
+
+// ---- field initialiser: Kernels.metal:386-453 ----------------------------
+template <class St>
+__global__ void __launch_bounds__(256)
+init_kernel(typename St::T* __restrict__ f, uint32_t nx, uint32_t ny, uint32_t nz,
+            uint32_t N, uint32_t S, uint32_t mode, float amplitude, float tau) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N) return;
+    const float PI = 3.14159265358979323846f;
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    float sxx = 0.0f, rhom1 = 0.0f, sxz = 0.0f, syz = 0.0f;
+    if (mode == 1u) {
+        const uint32_t x = n % nx, t = n / nx, y = t % ny;
+        const float kx = 2.0f * PI / (float)nx;
+        const float ky = 2.0f * PI / (float)ny;
+        const float xa = (float)x + 0.5f, ya = (float)y + 0.5f;
+        ux = amplitude * sinf(kx * xa) * cosf(ky * ya);
+        uy = -amplitude * cosf(kx * xa) * sinf(ky * ya);
+        sxx = amplitude * kx * cosf(kx * xa) * cosf(ky * ya);
+        rhom1 = 0.75f * amplitude * amplitude * (cosf(2.0f * kx * xa) + cosf(2.0f * ky * ya));
+    }
+    if (mode == 2u) {
+        const uint32_t x = n % nx, t = n / nx, y = t % ny, z = t / ny;
+        const float kx = 2.0f * PI / (float)nx;
+        const float ky = 2.0f * PI / (float)ny;
+        const float kz = 2.0f * PI / (float)nz;
+        const float xa = kx * ((float)x + 0.5f);
+        const float ya = ky * ((float)y + 0.5f);
+        const float za = kz * ((float)z + 0.5f);
+        ux = amplitude * sinf(xa) * cosf(ya) * cosf(za);
+        uy = -amplitude * cosf(xa) * sinf(ya) * cosf(za);
+        sxx = amplitude * kx * cosf(xa) * cosf(ya) * cosf(za);
+        sxz = -0.5f * amplitude * kz * sinf(xa) * cosf(ya) * sinf(za);
+        syz = 0.5f * amplitude * kz * cosf(xa) * sinf(ya) * sinf(za);
+        rhom1 = 0.1875f * amplitude * amplitude
+              * (cosf(2.0f * xa) + cosf(2.0f * ya)) * (cosf(2.0f * za) + 2.0f);
+    }
+    const float rho = 1.0f + rhom1;
+    const float u2 = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const float W = i == 0 ? (1.0f / 3.0f) : (i <= 6 ? (1.0f / 18.0f) : (1.0f / 36.0f));
+        const float cu = 3.0f * ((float)kCx[i] * ux + (float)kCy[i] * uy + (float)kCz[i] * uz);
+        const float feq = W * (rhom1 + rho * (cu + 0.5f * cu * cu - 1.5f * u2));
+        const float cc = (float)(kCx[i] * kCx[i] - kCy[i] * kCy[i]);
+        const float scc = sxx * cc + 2.0f * sxz * (float)(kCx[i] * kCz[i])
+                        + 2.0f * syz * (float)(kCy[i] * kCz[i]);
+        const float fneq = -3.0f * W * tau * scc;
+        f[(size_t)i * S + n] = St::st(feq + fneq);
+    }
+}
+
+// ---- fixed-order FP64 reductions ------------------------------------------
+// The summed range is split into `segs` segments of M elements (the 19 DDF
+// slots for the mass sum, one segment otherwise). Stage 1: block (b, g) sums
+// elements [b*chunk, min((b+1)*chunk, M)) of segment g; thread t takes every
+// 256th element of the chunk in increasing order, then the block folds its
+// 256 partials with a fixed binary tree through LDS. Stage 2 folds the
+// segment-major partials the same way. The order depends only on the lattice
+// size and the fixed chunking — never on timing — so every result is bitwise
+// reproducible run to run.
+__device__ __forceinline__ double block_tree_sum(double v, double* lds) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) lds[threadIdx.x] = lds[threadIdx.x] + lds[threadIdx.x + s];
+        __syncthreads();
+    }
+    return lds[0];
+}
+
+template <class Elem>
+__global__ void __launch_bounds__(256)
+reduce_stage1(Elem e, uint32_t M, uint32_t chunk, double* __restrict__ partials) {
+    __shared__ double lds[256];
+    const uint32_t lo = blockIdx.x * chunk;
+    const uint32_t hi = (M - lo) > chunk ? lo + chunk : M;
+    double acc = 0.0;
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += 256u) acc += e(blockIdx.y, k);
+    const double s = block_tree_sum(acc, lds);
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256)
+reduce_stage2(const double* __restrict__ partials, uint32_t P, double* __restrict__ out) {
+    __shared__ double lds[256];
+    double acc = 0.0;
+    for (uint32_t k = threadIdx.x; k < P; k += 256u) acc += partials[k];
+    const double s = block_tree_sum(acc, lds);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+template <class St> struct MassElem {
+    const typename St::T* f;
+    uint32_t S;
+    __device__ double operator()(uint32_t slot, uint32_t n) const {
+        return (double)St::ld(f[(size_t)slot * S + n]);
+    }
+};
+
+struct BoxForceElem {
+    const float4* force;
+    uint32_t nx, ny, x0, y0;
+    FastDiv bw, bh;
+    int comp;
+    __device__ double operator()(uint32_t, uint32_t k) const {
+        const uint32_t r = fdiv(k, bw);
+        const uint32_t xx = k - r * bw.d;
+        const uint32_t z = fdiv(r, bh);
+        const uint32_t yy = r - z * bh.d;
+        const float4 v = force[((size_t)z * ny + (y0 + yy)) * nx + (x0 + xx)];
+        return (double)(comp == 0 ? v.x : (comp == 1 ? v.y : v.z));
+    }
+};
+
+struct KineticElem {
+    const float4* mom;
+    __device__ double operator()(uint32_t, uint32_t k) const {
+        const float4 v = mom[k];
+        return (double)(v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+};
+
+__global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p, uint64_t words) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (k < words) p[k] = 0u;
+}
+
+// ---- launchers -------------------------------------------------------------
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
+    hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(L.p.blocks),
+                       dim3(256), 0, s, (typename St::T*)L.f, L.flags, L.link, L.lidm,
+                       (float4*)L.force, L.eps, L.p);
+    return hipGetLastError();
+}
+
+template <class St, int PARITY, bool WALLS, bool NT, bool LES>
+static hipError_t pick_force(const StepLaunch& L, hipStream_t s) {
+    return L.has_force ? launch_step_t<St, PARITY, WALLS, NT, LES, true>(L, s)
+                       : launch_step_t<St, PARITY, WALLS, NT, LES, false>(L, s);
+}
+template <class St, int PARITY, bool WALLS, bool NT>
+static hipError_t pick_les(const StepLaunch& L, hipStream_t s) {
+    return L.les ? pick_force<St, PARITY, WALLS, NT, true>(L, s)
+                 : pick_force<St, PARITY, WALLS, NT, false>(L, s);
+}
+template <class St, int PARITY, bool WALLS>
+static hipError_t pick_nt(const StepLaunch& L, hipStream_t s) {
+    return L.nt ? pick_les<St, PARITY, WALLS, true>(L, s) : pick_les<St, PARITY, WALLS, false>(L, s);
+}
+template <class St, int PARITY>
+static hipError_t pick_walls(const StepLaunch& L, hipStream_t s) {
+    return L.walls ? pick_nt<St, PARITY, true>(L, s) : pick_nt<St, PARITY, false>(L, s);
+}
+template <class St>
+static hipError_t pick_parity(const StepLaunch& L, hipStream_t s) {
+    return L.p.parity == 0 ? pick_walls<St, 0>(L, s) : pick_walls<St, 1>(L, s);
+}
+
+hipError_t launch_step(const StepLaunch& L, hipStream_t s) {
+    switch (L.precision) {
+        case kFP32: return pick_parity<StF32>(L, s);
+        case kFP16S: return pick_parity<StF16>(L, s);
+        case kBF16: return pick_parity<StBF16>(L, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+static inline uint32_t grid_for(uint64_t n) { return (uint32_t)((n + 255u) / 256u); }
+
+hipError_t launch_moments(int precision, const void* f, const uint8_t* flags, float* out,
+                          uint32_t N, uint32_t S, float fx, float fy, float fz, hipStream_t s) {
+    const dim3 g(grid_for(N)), b(256);
+    switch (precision) {
+        case kFP32: hipLaunchKernelGGL(moments_kernel<StF32>, g, b, 0, s, (const float*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
+        case kFP16S: hipLaunchKernelGGL(moments_kernel<StF16>, g, b, 0, s, (const _Float16*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
+        case kBF16: hipLaunchKernelGGL(moments_kernel<StBF16>, g, b, 0, s, (const uint16_t*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_init(int precision, void* f, uint32_t nx, uint32_t ny, uint32_t nz,
+                       uint32_t S, uint32_t mode, float amplitude, float tau, hipStream_t s) {
+    const uint32_t N = nx * ny * nz;
+    const dim3 g(grid_for(N)), b(256);
+    switch (precision) {
+        case kFP32: hipLaunchKernelGGL(init_kernel<StF32>, g, b, 0, s, (float*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
+        case kFP16S: hipLaunchKernelGGL(init_kernel<StF16>, g, b, 0, s, (_Float16*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
+        case kBF16: hipLaunchKernelGGL(init_kernel<StBF16>, g, b, 0, s, (uint16_t*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class Elem>
+static hipError_t reduce(const Elem& e, uint32_t M, uint32_t segs, double* partials,
+                         uint32_t max_partials, double* out, hipStream_t s) {
+    uint32_t Pb = (M + 4095u) / 4096u;               // >= 16 elements per thread
+    if (Pb * segs > max_partials) Pb = max_partials / segs;
+    if (Pb == 0) Pb = 1;
+    const uint32_t chunk = (uint32_t)(((uint64_t)M + Pb - 1) / Pb);
+    hipLaunchKernelGGL(reduce_stage1<Elem>, dim3(Pb, segs), dim3(256), 0, s, e, M, chunk, partials);
+    hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(256), 0, s, partials, Pb * segs, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mass_sum(int precision, const void* f, uint32_t N, uint32_t S,
+                           double* partials, uint32_t max_partials, double* out, hipStream_t s) {
+    switch (precision) {
+        case kFP32: return reduce(MassElem<StF32>{(const float*)f, S}, N, 19, partials, max_partials, out, s);
+        case kFP16S: return reduce(MassElem<StF16>{(const _Float16*)f, S}, N, 19, partials, max_partials, out, s);
+        case kBF16: return reduce(MassElem<StBF16>{(const uint16_t*)f, S}, N, 19, partials, max_partials, out, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_box_force_sum(const float* force, uint32_t nx, uint32_t ny, uint32_t nz,
+                                uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
+                                double* partials, uint32_t max_partials, double* out3,
+                                hipStream_t s) {
+    const uint32_t bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+    const uint32_t M = bw * bh * nz;
+    for (int c = 0; c < 3; c++) {
+        BoxForceElem e{(const float4*)force, nx, ny, x0, y0, make_fastdiv(bw), make_fastdiv(bh), c};
+        hipError_t err = reduce(e, M, 1, partials, max_partials, out3 + c, s);
+        if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_kinetic_sum(const float* moments, uint32_t N, double* partials,
+                              uint32_t max_partials, double* out, hipStream_t s) {
+    return reduce(KineticElem{(const float4*)moments}, N, 1, partials, max_partials, out, s);
+}
+
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
+    const uint64_t words = bytes / 4;
+    if (words == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, s,
+                       (uint32_t*)p, words);
+    return hipGetLastError();
+}
+
+}  // namespace strh

@@ -1,0 +1,630 @@
+// Host runtime of the solver and its C ABI (include/strouhal_hip.h).
+//
+// The MI355X counterpart of the reference's `GPU` + `Simulation`
+// (Sources/StrouhalCore/Sim.swift:60-459): lattice construction with the
+// neighbour-mask builder, the per-step parameter block and cosine ramp, the
+// stepping loop, the probes and the SHA-256 state digest. Every call reports
+// failure through a status code and a thread-local message; nothing aborts.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/strouhal_hip.h"
+#include "kernels.h"
+#include "lbm.h"
+#include "sha256.h"
+
+using namespace strh;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(STRH_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                        \
+    do {                                           \
+        hipError_t _e = (expr);                    \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+constexpr uint32_t kMaxPartials = 19 * 512;
+// The largest lattice: 19*N must index the mass reduction in 32 bits and the
+// per-slot byte offsets of the step kernel must fit 32 bits.
+constexpr uint64_t kMaxCells = 200000000ull;
+
+}  // namespace
+
+struct strh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string name;
+};
+
+struct strh_lattice {
+    strh_ctx* ctx = nullptr;
+    int precision = kFP32;
+    int nx = 0, ny = 0, nz = 0;
+    uint32_t N = 0, S = 0;
+    size_t B = 4;               // bytes per stored DDF
+    void* f = nullptr;          // 19 slots of S elements
+    uint8_t* flags = nullptr;   // N raw Cell values
+    uint32_t* link = nullptr;   // N link words (walls only)
+    uint32_t* lidm = nullptr;   // N moving-wall masks (walls only)
+    float* force = nullptr;     // 4N floats (wants_forces)
+    float* eps = nullptr;       // N solid fractions (after set_solid_fractions)
+    float* moments = nullptr;   // 4N floats, allocated on first probe
+    double* partials = nullptr; // reduction scratch
+    double* result = nullptr;   // 3 doubles
+    void* snap = nullptr;       // snapshot of f
+    int64_t snapSteps = -1;
+    bool walls = false, useEps = false, wantsForces = false;
+    // Sim.swift:164-172
+    float omega = 1.0f, omegaMinus = 1.0f;
+    float lid[3] = {0, 0, 0};
+    float uinTarget = 0.0f;
+    int64_t rampSteps = 0;
+    float bodyForce[3] = {0, 0, 0};
+    float cSmago = 0.0f;
+    bool inflowUniform = false;
+    bool spongeOn = false;
+    float spongeX0 = 0, spongeWidth = 1, spongeTau = 1;
+    int64_t stepsDone = 0;
+    double gpuSeconds = 0.0;
+};
+
+namespace {
+
+size_t slot_bytes(const strh_lattice* L) { return (size_t)L->S * L->B; }
+
+int set_device(strh_ctx* c) {
+    HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
+    return STRH_OK;
+}
+
+// Sim.swift:258-266 — C1 cosine ramp in double, narrowed to float.
+float ramp(const strh_lattice* L, int64_t t) {
+    if (L->rampSteps <= 0) return 1.0f;
+    const double x = std::min(1.0, (double)t / (double)L->rampSteps);
+    return (float)(0.5 * (1.0 - cos(M_PI * x)));
+}
+
+// Sim.swift:268-281 plus the launch geometry.
+StepParams make_params(const strh_lattice* L, int64_t t, bool writeForce) {
+    StepParams p;
+    memset(&p, 0, sizeof(p));
+    const float r = ramp(L, t);
+    p.nx = L->nx; p.ny = L->ny; p.nz = L->nz;
+    p.parity = (int)(t & 1);
+    p.N = L->N; p.S = L->S;
+    p.omega = L->omega; p.omegaMinus = L->omegaMinus;
+    p.uin = L->uinTarget * r;
+    p.cSmago = L->cSmago;
+    p.fx = L->bodyForce[0]; p.fy = L->bodyForce[1]; p.fz = L->bodyForce[2];
+    p.writeForce = writeForce ? 1 : 0;
+    p.ulidX = L->lid[0] * r; p.ulidY = L->lid[1] * r; p.ulidZ = L->lid[2] * r;
+    p.inflowUniform = L->inflowUniform ? 1 : 0;
+    if (L->spongeOn) {
+        p.spongeX0 = L->spongeX0;
+        p.spongeInvW = 1.0f / L->spongeWidth;
+        p.spongeTau = L->spongeTau;
+    } else {
+        p.spongeX0 = 1e9f; p.spongeInvW = 0.0f; p.spongeTau = 1.0f;
+    }
+    p.useEps = L->useEps ? 1 : 0;
+    for (int k = 0; k < 9; k++) {
+        const int i = 2 * k + 1;
+        const float cF = (float)kCx[i] * p.fx + (float)kCy[i] * p.fy + (float)kCz[i] * p.fz;
+        p.cF[k] = cF;
+        p.threeCF[k] = 3.0f * cF;
+    }
+    p.divx = make_fastdiv((uint32_t)L->nx);
+    p.divy = make_fastdiv((uint32_t)L->ny);
+    p.blocks = (L->N + 255u) / 256u;
+    return p;
+}
+
+StepLaunch make_launch(const strh_lattice* L, int64_t t, bool writeForce) {
+    StepLaunch s;
+    s.precision = L->precision;
+    s.f = L->f;
+    s.flags = L->flags;
+    s.link = L->link;
+    s.lidm = L->lidm;
+    s.force = L->force;
+    s.eps = L->eps;
+    s.walls = L->walls;
+    s.nt = L->useEps;
+    s.les = L->cSmago > 0.0f;
+    s.has_force = !(L->bodyForce[0] == 0.0f && L->bodyForce[1] == 0.0f && L->bodyForce[2] == 0.0f);
+    s.p = make_params(L, t, writeForce);
+    return s;
+}
+
+// Sim.swift:235-255 over z-slabs on host threads: bit (i-1) of the link word
+// is set when neighbour n + c_i (periodic) is not fluid; the moving-wall
+// family (LID / INFLOW / OUTFLOW) also sets it in the lid mask.
+void build_masks(const uint8_t* fl, int nx, int ny, int nz, std::vector<uint32_t>& link,
+                 std::vector<uint32_t>& lidm) {
+    const size_t N = (size_t)nx * ny * nz;
+    link.assign(N, 0u);
+    lidm.assign(N, 0u);
+    auto work = [&](int z0, int z1) {
+        for (int z = z0; z < z1; z++) for (int y = 0; y < ny; y++) for (int x = 0; x < nx; x++) {
+            const size_t idx = ((size_t)z * ny + y) * nx + x;
+            if (fl[idx] != kFluid) { link[idx] = kNotFluidBit; continue; }
+            uint32_t s = 0, l = 0;
+            for (int i = 1; i < 19; i++) {
+                const int xn = (x + kCx[i] + nx) % nx;
+                const int yn = (y + kCy[i] + ny) % ny;
+                const int zn = (z + kCz[i] + nz) % nz;
+                const uint8_t c = fl[((size_t)zn * ny + yn) * nx + xn];
+                if (c == kSolid) {
+                    s |= 1u << (i - 1);
+                } else if (c == kLid || c == kInflow || c == kOutflow) {
+                    s |= 1u << (i - 1);
+                    l |= 1u << (i - 1);
+                }
+            }
+            link[idx] = s;
+            lidm[idx] = l;
+        }
+    };
+    const int nt = std::max(1, std::min<int>(nz, (int)std::min(16u, std::thread::hardware_concurrency())));
+    if (nt == 1 || N < (1u << 18)) { work(0, nz); return; }
+    std::vector<std::thread> ts;
+    for (int k = 0; k < nt; k++) ts.emplace_back(work, nz * k / nt, nz * (k + 1) / nt);
+    for (auto& t : ts) t.join();
+}
+
+void free_lattice(strh_lattice* L) {
+    if (!L) return;
+    if (L->ctx) hipSetDevice(L->ctx->device);
+    void* ptrs[] = {L->f, L->flags, L->link, L->lidm, L->force, L->eps, L->moments,
+                    L->partials, L->result, L->snap};
+    for (void* p : ptrs) if (p) hipFree(p);
+    delete L;
+}
+
+int sync(strh_lattice* L, const char* what) {
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream), what);
+    return STRH_OK;
+}
+
+// Launch `steps` steps starting at stepsDone; force accumulation on absolute
+// step `forceStep` (or none when < 0). Sim.swift:288-331.
+int run_steps(strh_lattice* L, int64_t steps, int64_t forceStep) {
+    if (steps < 0) return fail(STRH_ERR_INVALID, "negative step count");
+    if (steps == 0) return STRH_OK;
+    strh_ctx* c = L->ctx;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+    for (int64_t s = 0; s < steps; s++) {
+        const int64_t t = L->stepsDone + s;
+        const StepLaunch sl = make_launch(L, t, t == forceStep);
+        HIP_TRY(launch_step(sl, c->stream), "step kernel launch");
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(c->ev1), "step kernels");
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+    L->gpuSeconds += (double)ms * 1e-3;
+    L->stepsDone += steps;
+    return STRH_OK;
+}
+
+int ensure_moments(strh_lattice* L) {
+    if (L->moments) return STRH_OK;
+    HIP_TRY(hipMalloc(&L->moments, (size_t)L->N * 16), "hipMalloc(moments)");
+    return STRH_OK;
+}
+
+int compute_moments(strh_lattice* L) {
+    int rc = ensure_moments(L);
+    if (rc) return rc;
+    HIP_TRY(launch_moments(L->precision, L->f, L->flags, L->moments, L->N, L->S,
+                           L->bodyForce[0], L->bodyForce[1], L->bodyForce[2], L->ctx->stream),
+            "moments kernel launch");
+    return STRH_OK;
+}
+
+int check_lat(strh_lattice* L) {
+    if (!L || !L->ctx) return fail(STRH_ERR_INVALID, "null lattice handle");
+    return set_device(L->ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* strh_last_error(void) { return g_error.c_str(); }
+
+int strh_ctx_create(int device_ordinal, strh_ctx** out) {
+    if (!out) return fail(STRH_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(STRH_ERR_NO_DEVICE, "no HIP device");
+    if (device_ordinal < 0 || device_ordinal >= count)
+        return fail(STRH_ERR_NO_DEVICE, "device ordinal %d out of range (%d devices)", device_ordinal, count);
+    HIP_TRY(hipSetDevice(device_ordinal), "hipSetDevice");
+    strh_ctx* c = new strh_ctx();
+    c->device = device_ordinal;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess) c->name = prop.gcnArchName;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(STRH_ERR_DEVICE, "stream/event creation failed");
+    }
+    *out = c;
+    return STRH_OK;
+}
+
+int strh_ctx_destroy(strh_ctx* c) {
+    if (!c) return STRH_OK;
+    hipSetDevice(c->device);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return STRH_OK;
+}
+
+int strh_device_name(strh_ctx* c, char* out, size_t cap) {
+    if (!c || !out || cap == 0) return fail(STRH_ERR_INVALID, "bad arguments");
+    snprintf(out, cap, "%s", c->name.c_str());
+    return STRH_OK;
+}
+
+int strh_lattice_create(strh_ctx* c, const strh_lattice_desc* d, const uint8_t* flags,
+                        strh_lattice** out) {
+    if (!c || !d || !flags || !out) return fail(STRH_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(STRH_ERR_INVALID, "lattice dimensions must be positive");
+    const uint64_t N = (uint64_t)d->nx * d->ny * d->nz;
+    if (N > kMaxCells) return fail(STRH_ERR_INVALID, "lattice of %llu cells exceeds the %llu-cell limit",
+                                   (unsigned long long)N, (unsigned long long)kMaxCells);
+    if (d->precision != STRH_FP32 && d->precision != STRH_FP16S && d->precision != STRH_BF16)
+        return fail(STRH_ERR_INVALID, "unknown precision %d", d->precision);
+    for (uint64_t n = 0; n < N; n++)
+        if (flags[n] > STRH_OUTFLOW) return fail(STRH_ERR_INVALID, "invalid cell flag %d at cell %llu", flags[n], (unsigned long long)n);
+    int rc = set_device(c);
+    if (rc) return rc;
+
+    strh_lattice* L = new strh_lattice();
+    L->ctx = c;
+    L->precision = d->precision;
+    L->nx = d->nx; L->ny = d->ny; L->nz = d->nz;
+    L->N = (uint32_t)N;
+    L->S = (uint32_t)((N + 63u) & ~63ull);
+    L->B = d->precision == STRH_FP32 ? 4 : 2;
+    L->omega = d->omega;
+    L->omegaMinus = d->omega_minus < 0.0f ? d->omega : d->omega_minus;
+    for (int k = 0; k < 3; k++) { L->lid[k] = d->lid[k]; L->bodyForce[k] = d->force[k]; }
+    L->uinTarget = d->uin;
+    L->rampSteps = d->ramp_steps;
+    L->cSmago = d->c_smago;
+    L->wantsForces = d->wants_forces != 0;
+
+    auto bail = [&](hipError_t e, const char* what) {
+        free_lattice(L);
+        return e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation
+                   ? fail(STRH_ERR_ALLOC, "%s: %s", what, hipGetErrorString(e))
+                   : hip_fail(e, what);
+    };
+    hipError_t e;
+    const size_t fbytes = 19 * slot_bytes(L);
+    if ((e = hipMalloc(&L->f, fbytes)) != hipSuccess) return bail(e, "hipMalloc(f)");
+    // Shifted rest state is exactly 0 in every precision (Sim.swift:225).
+    if ((e = hipMemsetAsync(L->f, 0, fbytes, c->stream)) != hipSuccess) return bail(e, "hipMemset(f)");
+    if ((e = hipMalloc(&L->flags, N)) != hipSuccess) return bail(e, "hipMalloc(flags)");
+    if ((e = hipMemcpy(L->flags, flags, N, hipMemcpyHostToDevice)) != hipSuccess) return bail(e, "copy flags");
+    bool anyWall = false;
+    for (uint64_t n = 0; n < N && !anyWall; n++) anyWall = flags[n] != STRH_FLUID;
+    L->walls = anyWall;
+    if (anyWall) {
+        std::vector<uint32_t> link, lidm;
+        build_masks(flags, d->nx, d->ny, d->nz, link, lidm);
+        if ((e = hipMalloc(&L->link, N * 4)) != hipSuccess) return bail(e, "hipMalloc(link)");
+        if ((e = hipMalloc(&L->lidm, N * 4)) != hipSuccess) return bail(e, "hipMalloc(lidm)");
+        if ((e = hipMemcpy(L->link, link.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess) return bail(e, "copy link");
+        if ((e = hipMemcpy(L->lidm, lidm.data(), N * 4, hipMemcpyHostToDevice)) != hipSuccess) return bail(e, "copy lidm");
+    }
+    if (L->wantsForces) {
+        if ((e = hipMalloc(&L->force, N * 16)) != hipSuccess) return bail(e, "hipMalloc(force)");
+        if ((e = hipMemsetAsync(L->force, 0, N * 16, c->stream)) != hipSuccess) return bail(e, "hipMemset(force)");
+    }
+    if ((e = hipMalloc(&L->partials, kMaxPartials * sizeof(double))) != hipSuccess) return bail(e, "hipMalloc(partials)");
+    if ((e = hipMalloc(&L->result, 4 * sizeof(double))) != hipSuccess) return bail(e, "hipMalloc(result)");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(e, "lattice init");
+    *out = L;
+    return STRH_OK;
+}
+
+int strh_lattice_destroy(strh_lattice* L) {
+    free_lattice(L);
+    return STRH_OK;
+}
+
+int strh_lattice_cells(strh_lattice* L, int64_t* cells) {
+    if (!L || !cells) return fail(STRH_ERR_INVALID, "null argument");
+    *cells = L->N;
+    return STRH_OK;
+}
+
+int strh_ddf_bytes(strh_lattice* L, size_t* bytes) {
+    if (!L || !bytes) return fail(STRH_ERR_INVALID, "null argument");
+    *bytes = 19 * (size_t)L->N * L->B;
+    return STRH_OK;
+}
+
+int strh_set_omega(strh_lattice* L, float omega, float omega_minus) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    L->omega = omega;
+    L->omegaMinus = omega_minus < 0.0f ? omega : omega_minus;
+    return STRH_OK;
+}
+
+int strh_set_lid_velocity(strh_lattice* L, float ux, float uy, float uz) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    L->lid[0] = ux; L->lid[1] = uy; L->lid[2] = uz;
+    return STRH_OK;
+}
+
+int strh_set_uin(strh_lattice* L, float uin) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    L->uinTarget = uin;
+    return STRH_OK;
+}
+
+int strh_set_ramp_steps(strh_lattice* L, int64_t r) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    L->rampSteps = r;
+    return STRH_OK;
+}
+
+int strh_set_force(strh_lattice* L, float fx, float fy, float fz) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    L->bodyForce[0] = fx; L->bodyForce[1] = fy; L->bodyForce[2] = fz;
+    return STRH_OK;
+}
+
+int strh_set_c_smago(strh_lattice* L, float c) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    L->cSmago = c;
+    return STRH_OK;
+}
+
+int strh_set_inflow_uniform(strh_lattice* L, int enabled) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    L->inflowUniform = enabled != 0;
+    return STRH_OK;
+}
+
+int strh_set_sponge(strh_lattice* L, int enabled, float x0, float width, float tau) {
+    if (!L) return fail(STRH_ERR_INVALID, "null lattice handle");
+    if (enabled && !(width > 0.0f)) return fail(STRH_ERR_INVALID, "sponge width must be positive");
+    L->spongeOn = enabled != 0;
+    L->spongeX0 = x0; L->spongeWidth = width; L->spongeTau = tau;
+    return STRH_OK;
+}
+
+int strh_set_solid_fractions(strh_lattice* L, const float* eps) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!eps) return fail(STRH_ERR_INVALID, "null eps");
+    if (!L->eps) HIP_TRY(hipMalloc(&L->eps, (size_t)L->N * 4), "hipMalloc(eps)");
+    HIP_TRY(hipMemcpy(L->eps, eps, (size_t)L->N * 4, hipMemcpyHostToDevice), "copy eps");
+    L->useEps = true;
+    return STRH_OK;
+}
+
+int strh_init_field(strh_lattice* L, uint32_t mode, float amplitude) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (mode > 2) return fail(STRH_ERR_INVALID, "unknown init mode %u", mode);
+    const float tau = 1.0f / L->omega;
+    HIP_TRY(launch_init(L->precision, L->f, (uint32_t)L->nx, (uint32_t)L->ny, (uint32_t)L->nz,
+                        L->S, mode, amplitude, tau, L->ctx->stream), "init kernel launch");
+    return sync(L, "init kernel");
+}
+
+int strh_run(strh_lattice* L, int64_t steps) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    return run_steps(L, steps, -1);
+}
+
+int strh_steps_done(strh_lattice* L, int64_t* steps) {
+    if (!L || !steps) return fail(STRH_ERR_INVALID, "null argument");
+    *steps = L->stepsDone;
+    return STRH_OK;
+}
+
+int strh_gpu_seconds(strh_lattice* L, double* seconds) {
+    if (!L || !seconds) return fail(STRH_ERR_INVALID, "null argument");
+    *seconds = L->gpuSeconds;
+    return STRH_OK;
+}
+
+int strh_probe_moments(strh_lattice* L, float* out) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    if (L->stepsDone % 2 != 0) return fail(STRH_ERR_ODD_STEPS, "moments probe requires even step count");
+    if ((rc = compute_moments(L))) return rc;
+    HIP_TRY(hipMemcpyAsync(out, L->moments, (size_t)L->N * 16, hipMemcpyDeviceToHost, L->ctx->stream), "copy moments");
+    return sync(L, "moments probe");
+}
+
+int strh_probe_force(strh_lattice* L, int x0, int x1, int y0, int y1, double out[3]) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    if (!L->wantsForces) return fail(STRH_ERR_NO_FORCES, "lattice created without wants_forces");
+    if (L->stepsDone % 2 != 0) return fail(STRH_ERR_ODD_STEPS, "force probe requires even step count");
+    if (x0 < 0 || x1 >= L->nx || x0 > x1 || y0 < 0 || y1 >= L->ny || y0 > y1)
+        return fail(STRH_ERR_INVALID, "probe box x[%d,%d] y[%d,%d] outside %dx%d", x0, x1, y0, y1, L->nx, L->ny);
+    HIP_TRY(launch_fill_zero(L->force, (size_t)L->N * 16, L->ctx->stream), "zero force");
+    if ((rc = run_steps(L, 2, L->stepsDone + 1))) return rc;
+    HIP_TRY(launch_box_force_sum(L->force, (uint32_t)L->nx, (uint32_t)L->ny, (uint32_t)L->nz,
+                                 (uint32_t)x0, (uint32_t)x1, (uint32_t)y0, (uint32_t)y1,
+                                 L->partials, kMaxPartials, L->result, L->ctx->stream),
+            "force reduction launch");
+    HIP_TRY(hipMemcpyAsync(out, L->result, 3 * sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy force");
+    return sync(L, "force probe");
+}
+
+int strh_mass_sum(strh_lattice* L, double* out) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    HIP_TRY(launch_mass_sum(L->precision, L->f, L->N, L->S, L->partials, kMaxPartials, L->result,
+                            L->ctx->stream), "mass reduction launch");
+    HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy mass");
+    return sync(L, "mass sum");
+}
+
+int strh_kinetic_energy(strh_lattice* L, double* out) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    if (L->stepsDone % 2 != 0) return fail(STRH_ERR_ODD_STEPS, "kinetic energy requires even step count");
+    if ((rc = compute_moments(L))) return rc;
+    HIP_TRY(launch_kinetic_sum(L->moments, L->N, L->partials, kMaxPartials, L->result, L->ctx->stream),
+            "kinetic reduction launch");
+    HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy energy");
+    return sync(L, "kinetic energy");
+}
+
+// Canonical image: 19 slots of N values, slot-major (Sim.swift:441-444 hashes
+// exactly these bytes). Device slots are padded to S; copy with a pitch.
+int strh_download_ddf(strh_lattice* L, void* out, size_t bytes) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    const size_t row = (size_t)L->N * L->B;
+    if (!out || bytes != 19 * row) return fail(STRH_ERR_INVALID, "download needs exactly %zu bytes", 19 * row);
+    HIP_TRY(hipMemcpy2DAsync(out, row, L->f, slot_bytes(L), row, 19, hipMemcpyDeviceToHost, L->ctx->stream), "download ddf");
+    return sync(L, "download ddf");
+}
+
+int strh_upload_ddf(strh_lattice* L, const void* in, size_t bytes) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    const size_t row = (size_t)L->N * L->B;
+    if (!in || bytes != 19 * row) return fail(STRH_ERR_INVALID, "upload needs exactly %zu bytes", 19 * row);
+    HIP_TRY(hipMemcpy2DAsync(L->f, slot_bytes(L), in, row, row, 19, hipMemcpyHostToDevice, L->ctx->stream), "upload ddf");
+    return sync(L, "upload ddf");
+}
+
+int strh_download_force(strh_lattice* L, float* out) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    if (!L->wantsForces) return fail(STRH_ERR_NO_FORCES, "lattice created without wants_forces");
+    HIP_TRY(hipMemcpyAsync(out, L->force, (size_t)L->N * 16, hipMemcpyDeviceToHost, L->ctx->stream), "copy force field");
+    return sync(L, "download force");
+}
+
+// SHA-256 over the canonical image, streamed through two pinned staging
+// buffers so the next chunk's copy overlaps hashing of the current one.
+int strh_state_digest(strh_lattice* L, uint8_t out[32]) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    const size_t row = (size_t)L->N * L->B;
+    const size_t total = 19 * row;
+    const size_t chunk = std::min<size_t>(total, (size_t)32 << 20);
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        for (int k = 0; k < 2; k++) {
+            if (stage[k]) hipHostFree(stage[k]);
+            if (done[k]) hipEventDestroy(done[k]);
+        }
+    };
+    for (int k = 0; k < 2; k++) {
+        if (hipHostMalloc(&stage[k], chunk, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&done[k], hipEventDisableTiming) != hipSuccess) {
+            cleanup();
+            return fail(STRH_ERR_ALLOC, "pinned staging allocation failed");
+        }
+    }
+    // Canonical byte offset o maps to slot o / row, in-slot offset o % row.
+    auto issue = [&](size_t off, size_t len, int k) -> hipError_t {
+        size_t done_bytes = 0;
+        while (done_bytes < len) {
+            const size_t o = off + done_bytes;
+            const size_t slot = o / row, in = o % row;
+            const size_t take = std::min(len - done_bytes, row - in);
+            hipError_t e = hipMemcpyAsync((char*)stage[k] + done_bytes,
+                                          (const char*)L->f + slot * slot_bytes(L) + in, take,
+                                          hipMemcpyDeviceToHost, L->ctx->stream);
+            if (e != hipSuccess) return e;
+            done_bytes += take;
+        }
+        return hipEventRecord(done[k], L->ctx->stream);
+    };
+    Sha256 sha;
+    size_t off = 0;
+    int k = 0;
+    hipError_t e = issue(0, std::min(chunk, total), 0);
+    while (e == hipSuccess && off < total) {
+        const size_t len = std::min(chunk, total - off);
+        const size_t next = off + len;
+        if (next < total) e = issue(next, std::min(chunk, total - next), k ^ 1);
+        if (e != hipSuccess) break;
+        e = hipEventSynchronize(done[k]);
+        if (e != hipSuccess) break;
+        sha.update(stage[k], len);
+        off = next;
+        k ^= 1;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(L->ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return hip_fail(e, "state digest");
+    sha.finish(out);
+    return STRH_OK;
+}
+
+int strh_snapshot(strh_lattice* L) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    const size_t fbytes = 19 * slot_bytes(L);
+    if (!L->snap) HIP_TRY(hipMalloc(&L->snap, fbytes), "hipMalloc(snapshot)");
+    HIP_TRY(hipMemcpyAsync(L->snap, L->f, fbytes, hipMemcpyDeviceToDevice, L->ctx->stream), "snapshot");
+    L->snapSteps = L->stepsDone;
+    return sync(L, "snapshot");
+}
+
+int strh_restore(strh_lattice* L) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!L->snap || L->snapSteps < 0) return fail(STRH_ERR_INVALID, "no snapshot taken");
+    HIP_TRY(hipMemcpyAsync(L->f, L->snap, 19 * slot_bytes(L), hipMemcpyDeviceToDevice, L->ctx->stream), "restore");
+    L->stepsDone = L->snapSteps;
+    return sync(L, "restore");
+}
+
+}  // extern "C"

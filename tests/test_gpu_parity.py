@@ -1,0 +1,291 @@
+"""GPU parity: the HIP step kernel against the FP32 oracle, bit for bit.
+
+Every case starts both implementations from the same raw DDF image, runs the
+same number of steps (odd and even counts, so both AA passes end a run) and
+compares the raw storage bits. The FP32 oracle restates
+Sources/StrouhalCore/Kernels.metal:105-355 in the reference's operation
+order with no contraction, as the kernel is built.
+"""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+import pytest
+
+from tests.helpers import (CX, CY, CZ, STORAGE, assert_bit_exact, make_pair, random_state,
+                           to_storage)
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ["fp32", "fp16s", "bf16"]
+
+
+def _flags(kind, nx, ny, nz, seed=0):
+    fl = np.zeros((nz, ny, nx), dtype=np.uint8)
+    if kind == "periodic":
+        return fl
+    if kind == "cavity":
+        fl[:, 0, :] = 1
+        fl[:, :, 0] = 1
+        fl[:, :, nx - 1] = 1
+        fl[:, ny - 1, :] = 2
+        return fl
+    if kind == "channel":
+        fl[:, 0, :] = 1
+        fl[:, ny - 1, :] = 1
+        fl[:, :, 0] = 3
+        fl[:, :, nx - 1] = 3
+        return fl
+    if kind == "mixed":
+        # every flag kind, scattered, including OUTFLOW (dormant ABB) walls
+        rng = np.random.default_rng(seed)
+        fl = rng.choice([0, 0, 0, 0, 0, 0, 1, 2, 3, 4], size=(nz, ny, nx)).astype(np.uint8)
+        return fl
+    raise ValueError(kind)
+
+
+def _run_compare(g, r, steps_list, precision):
+    for steps in steps_list:
+        g.run(steps)
+        r.run(steps)
+        assert g.steps_done == r.steps_done
+        assert_bit_exact(g.download_ddf(), r.download_ddf(), f"DDFs after {g.steps_done} steps")
+
+
+def _seed_both(g, r, precision, seed):
+    img = to_storage(random_state(g.cells, seed), precision)
+    g.upload_ddf(img)
+    r.upload_ddf(img if precision != "fp16s" else img.view(np.uint16))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "uploaded DDFs")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("shape", [(16, 12, 10), (13, 7, 5), (70, 3, 1), (64, 64, 1)])
+def test_periodic_srt_bulk(gpu, precision, shape):
+    nx, ny, nz = shape
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.7)
+    _seed_both(g, r, precision, seed=nx * 7 + ny)
+    _run_compare(g, r, [1, 2, 5], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_periodic_trt_guo_force(gpu, precision):
+    g, r = make_pair(gpu, precision, 18, 14, 9, 1.25, 1.6, force=(2e-5, -1e-5, 3e-6))
+    _seed_both(g, r, precision, seed=3)
+    _run_compare(g, r, [3, 4], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_les_srt_and_trt(gpu, precision):
+    for om, omm in [(1.95, None), (1.9, 1.3)]:
+        g, r = make_pair(gpu, precision, 17, 11, 6, om, omm, c_smago=0.04, force=(1e-6, 0, 0))
+        _seed_both(g, r, precision, seed=11)
+        _run_compare(g, r, [2, 3], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_cavity_moving_lid_with_ramp(gpu, precision):
+    n = 20
+    g, r = make_pair(gpu, precision, n + 2, n + 2, 3, 1.6, 1.1, lid=(0.1, 0.02, 0.0),
+                     ramp_steps=7, flags=_flags("cavity", n + 2, n + 2, 3))
+    _seed_both(g, r, precision, seed=5)
+    _run_compare(g, r, [1, 4, 6], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("uniform", [False, True])
+def test_channel_inflow_sponge(gpu, precision, uniform):
+    nx, ny = 40, 14
+    g, r = make_pair(gpu, precision, nx, ny, 2, 1.5, 1.7, uin=0.06, ramp_steps=5,
+                     flags=_flags("channel", nx, ny, 2))
+    g.inflow_uniform = uniform
+    r.inflow_uniform = uniform
+    g.sponge = (25.0, 10.0, 1.0)
+    r.sponge = g.sponge
+    _seed_both(g, r, precision, seed=7)
+    _run_compare(g, r, [3, 3], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_mixed_flags_outflow_abb(gpu, precision):
+    nx, ny, nz = 15, 9, 7
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.4, None, lid=(0.05, 0.0, 0.01), uin=0.04,
+                     flags=_flags("mixed", nx, ny, nz, seed=2))
+    _seed_both(g, r, precision, seed=9)
+    _run_compare(g, r, [1, 1, 4], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_noble_torczynski_with_forces(gpu, precision):
+    nx, ny, nz = 24, 18, 4
+    rng = np.random.default_rng(4)
+    eps = np.zeros((nz, ny, nx), dtype=np.float32)
+    eps[:, 5:12, 6:14] = rng.uniform(0.0, 1.0, size=(nz, 7, 8)).astype(np.float32)
+    eps[:, 8, 9] = 1.0
+    eps[:, 6, 7] = 0.0
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.3, 1.5, uin=0.05, force=(1e-5, 0, 0),
+                     flags=_flags("channel", nx, ny, nz), wants_forces=True)
+    g.set_solid_fractions(eps)
+    r.set_solid_fractions(eps)
+    _seed_both(g, r, precision, seed=13)
+    _run_compare(g, r, [2], precision)
+    # force probe: one even+odd pair with momentum exchange on the odd pass
+    fg = g.probe_force((3, 18), (2, 15))
+    fr = r.probe_force((3, 18), (2, 15))
+    assert g.steps_done == r.steps_done
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+    field_g = g.download_force()
+    field_r = r.force_field().astype(np.float32)
+    assert_bit_exact(field_g, field_r, "per-cell force")
+    # box sum: FP64, fixed tree order on the GPU vs sequential on the CPU
+    scale = np.abs(field_r[:, :3].astype(np.float64)).sum() + 1e-300
+    assert np.all(np.abs(fg - fr) <= 1e-13 * scale), (fg, fr)
+
+
+def test_streaming_lone_ddf(gpu):
+    """Cases.swift:85-125: omega = 0, each DDF lands at c0 + T c_i, 18 nonzero slots."""
+    from strouhal_amd import Simulation
+    for precision in PRECISIONS:
+        n = 16
+        sim = Simulation(gpu, precision, nx=n, ny=n, nz=n, omega=0.0)
+        N = sim.cells
+        f = sim.download_ddf()
+        c0 = (8 * n + 8) * n + 8
+        inj = to_storage(np.array([[i * 0.001] for i in range(19)]), precision)[:, 0]
+        for i in range(1, 19):
+            f[i, c0] = inj[i]
+        sim.upload_ddf(f)
+        T = 4
+        sim.run(T)
+        g = sim.download_ddf()
+        for i in range(1, 19):
+            dest = (((8 + T * CZ[i]) % n) * n + (8 + T * CY[i]) % n) * n + (8 + T * CX[i]) % n
+            assert g[i, dest].tobytes() == inj[i].tobytes(), (precision, i)
+        assert np.count_nonzero(g.view(np.uint16 if g.dtype.itemsize == 2 else np.uint32)) == 18
+        assert N == n ** 3
+
+
+def test_rest_fixed_points(gpu):
+    """Cases.swift:58-83: rest states are bitwise fixed points (periodic and cavity)."""
+    from strouhal_amd import Simulation
+    from strouhal_amd.cases import cavity_flags
+    for precision in PRECISIONS:
+        sim = Simulation(gpu, precision, nx=32, ny=32, nz=32, omega=1.7)
+        d0 = sim.state_digest
+        sim.run(100)
+        assert sim.state_digest == d0
+        sim = Simulation(gpu, precision, nx=34, ny=34, nz=1, omega=1.7, flags=cavity_flags(32))
+        d0 = sim.state_digest
+        sim.run(100)
+        assert sim.state_digest == d0
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_digest_matches_hashlib_and_oracle(gpu, precision):
+    g, r = make_pair(gpu, precision, 33, 9, 5, 1.8)
+    _seed_both(g, r, precision, seed=21)
+    g.run(3)
+    r.run(3)
+    img = g.download_ddf()
+    assert g.state_digest == hashlib.sha256(img.tobytes()).hexdigest() == r.state_digest
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_moments_and_mass_and_energy(gpu, precision):
+    g, r = make_pair(gpu, precision, 21, 13, 7, 1.6, force=(1e-5, 2e-5, 0.0),
+                     flags=_flags("cavity", 21, 13, 7))
+    _seed_both(g, r, precision, seed=17)
+    g.run(4)
+    r.run(4)
+    mg = g.probe_moments()
+    mr = r.probe_moments().astype(np.float32)
+    assert_bit_exact(mg, mr, "moments")
+    ddf = r.ddf_float()
+    scale = np.abs(ddf).sum()
+    assert abs(g.mass_sum() - r.mass_sum()) <= 1e-13 * scale
+    ke_ref = float(((mg[:, 0] * mg[:, 0] + mg[:, 1] * mg[:, 1]) + mg[:, 2] * mg[:, 2])
+                   .astype(np.float64).sum())
+    assert abs(g.kinetic_energy() - ke_ref) <= 1e-13 * ke_ref
+    g.run(1)
+    from strouhal_amd import StrouhalError
+    with pytest.raises(StrouhalError):
+        g.probe_moments()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_init_field_matches_oracle(gpu, precision, mode):
+    """initField (Kernels.metal:393-453): GPU sinf/cosf vs glibc, within 4 ulp of storage."""
+    from oracle.oracle import OracleSim
+    from strouhal_amd import Simulation
+    nx, ny, nz = 24, 20, 12
+    g = Simulation(gpu, precision, nx=nx, ny=ny, nz=nz, omega=1.3)
+    r = OracleSim(nx, ny, nz, g.omega, arith="f32", storage=STORAGE[precision])
+    g.init_field(mode, 0.05)
+    r.init_field(mode, 0.05)
+    a = r.ddf_float()
+    gg = g.download_ddf()
+    if precision == "bf16":
+        b = (gg.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        ulp = 2.0 ** -7
+    elif precision == "fp16s":
+        b = gg.astype(np.float64)
+        ulp = 2.0 ** -10
+    else:
+        b = gg.astype(np.float64)
+        ulp = 2.0 ** -23
+    tol = 4 * ulp * np.maximum(np.abs(a), 1e-6)
+    assert np.all(np.abs(a - b) <= tol)
+
+
+def test_determinism_run_twice(gpu):
+    """Cases.swift:374-399: two identical runs give identical digests."""
+    from strouhal_amd import Simulation
+    from strouhal_amd.cases import cavity
+    for precision in ["fp32", "fp16s"]:
+        digests = []
+        for _ in range(2):
+            sim = cavity(gpu, precision, n=128, re=1000.0, lam=0.25)
+            sim.run(2000)
+            digests.append(sim.state_digest)
+        assert digests[0] == digests[1]
+        d2 = []
+        for _ in range(2):
+            sim = Simulation(gpu, precision, nx=64, ny=64, nz=64, omega=1.9)
+            sim.init_field(1, 0.05)
+            sim.run(100)
+            d2.append((sim.state_digest, sim.mass_sum(), sim.kinetic_energy()))
+        assert d2[0] == d2[1]
+
+
+def test_snapshot_restore(gpu):
+    from strouhal_amd import Simulation
+    sim = Simulation(gpu, "fp32", nx=20, ny=20, nz=20, omega=1.8)
+    sim.init_field(2, 0.05)
+    sim.run(4)
+    sim.snapshot()
+    d = sim.state_digest
+    sim.run(6)
+    after = sim.state_digest
+    sim.restore()
+    assert sim.steps_done == 4 and sim.state_digest == d
+    sim.run(6)
+    assert sim.state_digest == after
+
+
+def test_error_paths(gpu):
+    from strouhal_amd import Simulation, StrouhalError
+    sim = Simulation(gpu, "fp32", nx=8, ny=8, nz=8, omega=1.0)
+    with pytest.raises(StrouhalError) as e:
+        sim.probe_force((0, 3), (0, 3))
+    assert e.value.code == 6
+    with pytest.raises(StrouhalError):
+        sim.upload_ddf(np.zeros((19, 10), dtype=np.float32))
+    with pytest.raises(StrouhalError):
+        sim.init_field(7, 0.1)
+    with pytest.raises(StrouhalError):
+        Simulation(gpu, "fp32", nx=0, ny=8, nz=8, omega=1.0)
+    sim2 = Simulation(gpu, "fp32", nx=8, ny=8, nz=8, omega=1.0, wants_forces=True)
+    with pytest.raises(StrouhalError):
+        sim2.probe_force((0, 9), (0, 3))

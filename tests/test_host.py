@@ -1,0 +1,128 @@
+"""Host-side checks (CPU only): the C-ABI library loads and exports every
+declared symbol, SHA-256 known answers, host geometry builders."""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "strouhal_hip.h")).read()
+    return sorted(set(re.findall(r"STRH_API\s+[\w\s\*]+?\b(strh_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    from strouhal_amd import _lib
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for name in syms:
+        assert hasattr(L, name), name
+    assert sorted(_lib.EXPORTS) == syms
+
+
+def test_no_device_reports_status_not_abort():
+    from strouhal_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = L.strh_ctx_create(0, ctypes.byref(h))
+    if rc == 0:
+        L.strh_ctx_destroy(h)
+    else:
+        assert rc == 2
+        assert b"device" in L.strh_last_error()
+    # null arguments are reported, never dereferenced
+    assert L.strh_run(None, 1) == 1
+    assert L.strh_mass_sum(None, None) == 1
+
+
+KATS = {
+    b"": "e3b0c44298fc1c149afbf4c8996fb92427ae41e4649b934ca495991b7852b855",
+    b"abc": "ba7816bf8f01cfea414140de5dae2223b00361a396177a9cb410ff61f20015ad",
+    b"abcdbcdecdefdefgefghfghighijhijkijkljklmklmnlmnomnopnopq":
+        "248d6a61d20638b8e5c026930c3e6039a33ce45964ff2167f6ecedd419db06c1",
+    b"a" * 1_000_000: "cdc76e5c9914fb9281a1c7e284d73e67f1809a48a497200e046d39ccc7112cd0",
+}
+
+
+@pytest.mark.parametrize("portable", [False, True])
+def test_sha256_known_answers(portable):
+    from strouhal_amd import sha256, sha256_force_portable
+    try:
+        sha256_force_portable(portable)
+        for msg, digest in KATS.items():
+            assert sha256(msg) == digest
+        rng = np.random.default_rng(0)
+        for n in [1, 55, 56, 63, 64, 65, 119, 120, 127, 128, 1000, 4097]:
+            data = rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
+            assert sha256(data) == hashlib.sha256(data).hexdigest()
+    finally:
+        sha256_force_portable(False)
+
+
+def _disk_literal(nx, ny, cx, cy, r):
+    """CaseSetups.swift:6-25 transcribed as scalar loops."""
+    eps = np.zeros(nx * ny, dtype=np.float32)
+    r2, sub = r * r, 16
+    lo, hi = int(cx - r) - 2, int(cx + r) + 2
+    loY, hiY = int(cy - r) - 2, int(cy + r) + 2
+    for y in range(max(0, loY), min(ny - 1, hiY) + 1):
+        for x in range(max(0, lo), min(nx - 1, hi) + 1):
+            inside = 0
+            for sy in range(sub):
+                for sx in range(sub):
+                    px = float(x) - 0.5 + (float(sx) + 0.5) / float(sub)
+                    py = float(y) - 0.5 + (float(sy) + 0.5) / float(sub)
+                    dx, dy = px - cx, py - cy
+                    if dx * dx + dy * dy <= r2:
+                        inside += 1
+            eps[y * nx + x] = np.float32(inside) / np.float32(sub * sub)
+    return eps.reshape(ny, nx)
+
+
+def _sphere_literal(n, c, r):
+    eps = np.zeros((n, n, n), dtype=np.float32)
+    r2, sub = r * r, 8
+    cx, cy, cz = c
+    for z in range(max(0, int(cz - r) - 2), min(n - 1, int(cz + r) + 2) + 1):
+        for y in range(max(0, int(cy - r) - 2), min(n - 1, int(cy + r) + 2) + 1):
+            for x in range(max(0, int(cx - r) - 2), min(n - 1, int(cx + r) + 2) + 1):
+                inside = 0
+                for sz in range(sub):
+                    for sy in range(sub):
+                        for sx in range(sub):
+                            px = float(x) - 0.5 + (float(sx) + 0.5) / float(sub)
+                            py = float(y) - 0.5 + (float(sy) + 0.5) / float(sub)
+                            pz = float(z) - 0.5 + (float(sz) + 0.5) / float(sub)
+                            dx, dy, dz = px - cx, py - cy, pz - cz
+                            if dx * dx + dy * dy + dz * dz <= r2:
+                                inside += 1
+                eps[z, y, x] = np.float32(inside) / np.float32(sub ** 3)
+    return eps
+
+
+def test_disk_and_sphere_fractions_match_the_loops():
+    from strouhal_amd.cases import disk_solid_fractions, sphere_solid_fractions
+    d = disk_solid_fractions(20, 18, 9.3, 8.7, 4.5)
+    assert np.array_equal(d, _disk_literal(20, 18, 9.3, 8.7, 4.5))
+    assert abs(float(d.sum()) - np.pi * 4.5 ** 2) < 0.05
+    s = sphere_solid_fractions(10, 10, 10, 5.2, 4.9, 5.05, 2.6)
+    assert np.array_equal(s, _sphere_literal(10, (5.2, 4.9, 5.05), 2.6))
+    assert abs(float(s.sum()) - 4.0 / 3.0 * np.pi * 2.6 ** 3) < 0.1
+
+
+def test_trt_omegas_and_flags():
+    from strouhal_amd import Cell, flags_array, trt_omegas
+    wp, wm = trt_omegas(0.8, 3.0 / 16.0)
+    assert wp == float(np.float32(1.25))
+    assert wm == float(np.float32(1.0 / (0.5 + 0.1875 / 0.3)))
+    assert trt_omegas(0.6, None) == (float(np.float32(1 / 0.6)),) * 2
+    fl = flags_array(lambda x, y, z: np.where(y == 0, Cell.solid, Cell.fluid), 5, 4, 3)
+    assert fl.shape == (3, 4, 5) and fl[:, 0, :].min() == 1 and fl[:, 1:, :].max() == 0
