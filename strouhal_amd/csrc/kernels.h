@@ -24,10 +24,13 @@ struct StepLaunch {
     bool nt;         // Noble-Torczynski field installed
     bool les;        // cSmago > 0
     bool has_force;  // body force non-zero
+    uint32_t rows_bs;  // 256 / 512: odd pass uses LDS row staging; 0: direct
     StepParams p;
 };
 
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);
+// Workgroup size of the LDS row-staged odd pass for this lattice, or 0.
+uint32_t rows_block_size(uint32_t nx, uint64_t N);
 hipError_t launch_moments(int precision, const void* f, const uint8_t* flags, float* out,
                           uint32_t N, uint32_t S, float fx, float fy, float fz, hipStream_t s);
 hipError_t launch_init(int precision, void* f, uint32_t nx, uint32_t ny, uint32_t nz,

@@ -28,25 +28,27 @@
 namespace strh {
 
 // ---- storage -------------------------------------------------------------
-struct StF32 {
+// Conversion between the stored DDF and the FP32 arithmetic value
+// ((float)f[...] and (FPXX)fh[i] in Kernels.metal:254-257).
+struct ConvF32 {
     using T = float;
     __device__ static float ld(T v) { return v; }
     __device__ static T st(float v) { return v; }
 };
-struct StF16 {
-    using T = _Float16;
-    __device__ static float ld(T v) { return (float)v; }       // exact widening
+struct ConvF16 {
+    using T = uint16_t;   // binary16 bits
+    __device__ static float ld(T v) { return (float)__builtin_bit_cast(_Float16, v); }  // exact
     // IEEE RNE of the binary32 value. The empty asm pins v as a rounded FP32
     // register: without it the compiler fuses the producing fmaf and this
     // conversion into one v_fma_mixlo_f16 (a single rounding to f16), which
     // differs from the reference's (half)(float) double rounding.
     __device__ static T st(float v) {
         asm("" : "+v"(v));
-        return (_Float16)v;
+        return __builtin_bit_cast(uint16_t, (_Float16)v);
     }
 };
-struct StBF16 {
-    using T = uint16_t;
+struct ConvBF16 {
+    using T = uint16_t;   // bfloat16 bits
     __device__ static float ld(T v) { return __uint_as_float((uint32_t)v << 16); }
     __device__ static T st(float v) {                          // IEEE RNE
         uint32_t u = __float_as_uint(v);
@@ -55,6 +57,38 @@ struct StBF16 {
         return (uint16_t)(u >> 16);
     }
 };
+
+// DDF memory access at a 32-bit byte offset from a slot base (a uniform
+// pointer), so loads and stores use the scalar-base + 32-bit vector-offset
+// addressing form. POL bit 0: nontemporal loads, bit 1: nontemporal stores
+// (streaming data touched once per pass; keeps it from displacing lines
+// that still have a reader in the same pass).
+template <class Conv, int POL> struct Store : Conv {
+    using T = typename Conv::T;
+    __device__ static float rd(const T* base, uint32_t off) {
+        const T* p = (const T*)((const char*)base + off);
+        return Conv::ld((POL & 1) ? __builtin_nontemporal_load(p) : *p);
+    }
+    __device__ static void wr(T* base, uint32_t off, float v) { wr_raw(base, off, Conv::st(v)); }
+    __device__ static T rd_raw(const T* base, uint32_t off) {
+        const T* p = (const T*)((const char*)base + off);
+        return (POL & 1) ? __builtin_nontemporal_load(p) : *p;
+    }
+    __device__ static void wr_raw(T* base, uint32_t off, T t) {
+        T* p = (T*)((char*)base + off);
+        if (POL & 2) __builtin_nontemporal_store(t, p); else *p = t;
+    }
+};
+
+// Access policies per pass (measured on MI355X at 256^3): the even pass and
+// the row-staged odd pass touch whole aligned lines and run fastest fully
+// nontemporal; the direct odd pass writes x-shifted partial lines, which must
+// merge in L2, so it keeps the default policy.
+constexpr int kPolAligned = 3;
+constexpr int kPolShifted = 0;
+using StF32 = Store<ConvF32, kPolAligned>;
+using StF16 = Store<ConvF16, kPolAligned>;
+using StBF16 = Store<ConvBF16, kPolAligned>;
 
 // ---- index helpers ---------------------------------------------------------
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv d) {
@@ -285,7 +319,7 @@ __device__ __forceinline__ void collide(float (&fh)[19], const StepParams& p, in
 }
 
 // ---- the fused step: Kernels.metal:230-355 ------------------------------
-struct Offsets {
+struct Offsets {   // byte offsets to the wrapped neighbour in each axis direction
     int xm, xp, ym, yp, zm, zp;
     template <int CX, int CY, int CZ> __device__ __forceinline__ int at() const {
         return pick<CX>(xm, xp) + pick<CY>(ym, yp) + pick<CZ>(zm, zp);
@@ -304,6 +338,8 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
     T* fs[19];
 #pragma unroll
     for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * S;
+    constexpr int TB = (int)sizeof(T);
+    const uint32_t nb = n * (uint32_t)TB;     // byte offset of cell n within a slot
 
     float eps = 0.0f;
     float fh[19];
@@ -315,11 +351,16 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
         const uint32_t t = fdiv(n, p.divx);
         const int x = (int)(n - t * (uint32_t)p.nx);
 #pragma unroll
-        for (int i = 0; i < 19; i++) fh[i] = St::ld(fs[i][n]);
+        for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nb);
         collide<LES, FORCE, NT>(fh, p, x, eps, ntx, nty, ntz);
-        fs[0][n] = St::st(fh[0]);
+        // A separate copy of the offset keeps the stores on the scalar-base
+        // form too (otherwise the shared base+offset sum is kept as 19
+        // 64-bit VGPR addresses across the collision).
+        uint32_t nbs = nb;
+        asm("" : "+v"(nbs));
+        St::wr(fs[0], nbs, fh[0]);
 #pragma unroll
-        for (int i = 1; i < 19; i++) fs[kOpp[i]][n] = St::st(fh[i]);
+        for (int i = 1; i < 19; i++) St::wr(fs[kOpp[i]], nbs, fh[i]);
         if (NT && p.writeForce != 0 && eps > 0.0f) force[n] = make_float4(ntx, nty, ntz, 0.0f);
         return;
     }
@@ -340,22 +381,22 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int plane = nx * ny;
     Offsets o;
-    o.xm = (x == 0) ? nx - 1 : -1;
-    o.xp = (x == nx - 1) ? 1 - nx : 1;
-    o.ym = (y == 0) ? (ny - 1) * nx : -nx;
-    o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
-    o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
-    o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
+    o.xm = ((x == 0) ? nx - 1 : -1) * TB;
+    o.xp = ((x == nx - 1) ? 1 - nx : 1) * TB;
+    o.ym = ((y == 0) ? (ny - 1) * nx : -nx) * TB;
+    o.yp = ((y == ny - 1) ? -(ny - 1) * nx : nx) * TB;
+    o.zm = (((int)z == 0) ? (nz - 1) * plane : -plane) * TB;
+    o.zp = (((int)z == nz - 1) ? -(nz - 1) * plane : plane) * TB;
     const int ni = (int)n;
 
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     uint32_t deferred = 0u;
-    fh[0] = St::ld(fs[0][n]);
+    fh[0] = St::rd(fs[0], nb);
 #define STRH_PULL(I)                                                                  \
     {                                                                                 \
         constexpr int srcBit = kOpp[I] - 1;                                           \
         if (WALLS && ((sM >> srcBit) & 1u)) {                                         \
-            fh[I] = St::ld(fs[I][n]);                                                 \
+            fh[I] = St::rd(fs[I], nb);                                                \
             if ((lM >> srcBit) & 1u) deferred |= 1u << I;                             \
             if (p.writeForce != 0) {                                                  \
                 ax = acc<-kCx[I]>(ax, fh[I]);                                         \
@@ -363,7 +404,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                 az = acc<-kCz[I]>(az, fh[I]);                                         \
             }                                                                         \
         } else {                                                                      \
-            fh[I] = St::ld(fs[kOpp[I]][(uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>())]); \
+            fh[I] = St::rd(fs[kOpp[I]], nb + (uint32_t)o.at<-kCx[I], -kCy[I], -kCz[I]>()); \
         }                                                                             \
     }
     STRH_PULL(1) STRH_PULL(2) STRH_PULL(3) STRH_PULL(4) STRH_PULL(5) STRH_PULL(6)
@@ -375,7 +416,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
 #define STRH_DEFER(I)                                                                 \
         if ((deferred >> I) & 1u) {                                                   \
             const int sy = y - kCy[I] < 0 ? ny - 1 : (y - kCy[I] >= ny ? 0 : y - kCy[I]); \
-            const uint32_t s = (uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>());    \
+            const uint32_t s = (uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>() / TB); \
             const uint8_t wf = flags[s];                                              \
             if (wf == kOutflow) {                                                     \
                 fh[I] = -fh[I];                                                       \
@@ -404,7 +445,9 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
         ay += nty;
         az += ntz;
     }
-    fs[0][n] = St::st(fh[0]);                                      // 317-350
+    uint32_t nbs = nb;                                             // 317-350
+    asm("" : "+v"(nbs));
+    St::wr(fs[0], nbs, fh[0]);
 #define STRH_PUSH(I)                                                                  \
     {                                                                                 \
         constexpr int dstBit = I - 1;                                                 \
@@ -413,7 +456,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             float sign = 1.0f;                                                        \
             if ((lM >> dstBit) & 1u) {                                                \
                 const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
-                const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>());   \
+                const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>() / TB); \
                 const uint8_t wf = flags[d];                                          \
                 if (wf == kOutflow) {                                                 \
                     sign = -1.0f;                                                     \
@@ -424,7 +467,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                          * ((float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz); \
                 }                                                                     \
             }                                                                         \
-            fs[kOpp[I]][n] = St::st(sign * fh[I] - corr);                             \
+            St::wr(fs[kOpp[I]], nbs, sign * fh[I] - corr);                            \
             if (p.writeForce != 0) {                                                  \
                 const float v = sign * fh[I] + 2.0f * wgt<I>();                       \
                 ax = acc<kCx[I]>(ax, v);                                              \
@@ -432,7 +475,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                 az = acc<kCz[I]>(az, v);                                              \
             }                                                                         \
         } else {                                                                      \
-            fs[I][(uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>())] = St::st(fh[I]);   \
+            St::wr(fs[I], nbs + (uint32_t)o.at<kCx[I], kCy[I], kCz[I]>(), fh[I]);     \
         }                                                                             \
     }
     STRH_PUSH(1) STRH_PUSH(2) STRH_PUSH(3) STRH_PUSH(4) STRH_PUSH(5) STRH_PUSH(6)
@@ -441,6 +484,87 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
 #undef STRH_PUSH
     if (p.writeForce != 0 && (sM != 0u || eps > 0.0f))               // 351-353
         force[n] = make_float4(ax, ay, az, 0.0f);
+}
+
+// ---- odd pass with LDS row staging (wall-free lattices) --------------------
+// A workgroup of BS threads covers BS/nx whole x-rows (nx a power of two
+// dividing BS). The 10 x-shifted directions (c_x != 0) are loaded and stored
+// as aligned row segments and shifted by one cell through LDS, with the
+// periodic x-wrap applied inside the row; the 8 others are aligned already.
+// Every global access is then a whole, aligned run, so the pass streams like
+// the even pass. The slot set a workgroup touches belongs to its own cells
+// (the AA property), so no other workgroup reads or writes it.
+template <int I> __device__ __forceinline__ constexpr int xslot() { return I <= 2 ? I - 1 : I - 5; }
+
+template <class St, int BS, bool NT, bool LES, bool FORCE>
+__global__ void __launch_bounds__(BS)
+step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
+              const float* __restrict__ epsBuf, StepParams p) {
+    using T = typename St::T;
+    __shared__ T row[10][BS];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * BS + t;   // N % BS == 0
+    const size_t S = p.S;
+    T* fs[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * S;
+    constexpr int TB = (int)sizeof(T);
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const uint32_t xmask = (uint32_t)nx - 1u;
+    const uint32_t x = n & xmask;
+    const uint32_t rowbase = t - x;
+    const uint32_t r = fdiv(n, p.divx);
+    const uint32_t z = fdiv(r, p.divy);
+    const int y = (int)(r - z * (uint32_t)ny);
+    const int plane = nx * ny;
+    Offsets o;
+    o.xm = 0; o.xp = 0;
+    o.ym = ((y == 0) ? (ny - 1) * nx : -nx) * TB;
+    o.yp = ((y == ny - 1) ? -(ny - 1) * nx : nx) * TB;
+    o.zm = (((int)z == 0) ? (nz - 1) * plane : -plane) * TB;
+    o.zp = (((int)z == nz - 1) ? -(nz - 1) * plane : plane) * TB;
+    const uint32_t nb = n * (uint32_t)TB;
+
+    float fh[19];
+    fh[0] = St::rd(fs[0], nb);
+#define STRH_RPULL(I)                                                                        \
+    if (kCx[I] == 0) fh[I] = St::rd(fs[kOpp[I]], nb + (uint32_t)o.at<0, -kCy[I], -kCz[I]>()); \
+    else row[xslot<I>()][t] = St::rd_raw(fs[kOpp[I]], nb + (uint32_t)o.at<0, -kCy[I], -kCz[I]>());
+    STRH_RPULL(1) STRH_RPULL(2) STRH_RPULL(3) STRH_RPULL(4) STRH_RPULL(5) STRH_RPULL(6)
+    STRH_RPULL(7) STRH_RPULL(8) STRH_RPULL(9) STRH_RPULL(10) STRH_RPULL(11) STRH_RPULL(12)
+    STRH_RPULL(13) STRH_RPULL(14) STRH_RPULL(15) STRH_RPULL(16) STRH_RPULL(17) STRH_RPULL(18)
+#undef STRH_RPULL
+    float eps = 0.0f;
+    if (NT && p.useEps) eps = epsBuf[n];
+    __syncthreads();
+#define STRH_RSHIFT(I) \
+    if (kCx[I] != 0) fh[I] = St::ld(row[xslot<I>()][rowbase + ((x - (uint32_t)kCx[I]) & xmask)]);
+    STRH_RSHIFT(1) STRH_RSHIFT(2) STRH_RSHIFT(7) STRH_RSHIFT(8) STRH_RSHIFT(9) STRH_RSHIFT(10)
+    STRH_RSHIFT(11) STRH_RSHIFT(12) STRH_RSHIFT(13) STRH_RSHIFT(14)
+#undef STRH_RSHIFT
+
+    float ntx = 0.0f, nty = 0.0f, ntz = 0.0f;
+    collide<LES, FORCE, NT>(fh, p, (int)x, eps, ntx, nty, ntz);
+
+    __syncthreads();   // every shifted read of `row` is done
+    uint32_t nbs = nb;
+    asm("" : "+v"(nbs));
+    St::wr(fs[0], nbs, fh[0]);
+#define STRH_RPUSH(I)                                                                        \
+    if (kCx[I] == 0) St::wr(fs[I], nbs + (uint32_t)o.at<0, kCy[I], kCz[I]>(), fh[I]);       \
+    else row[xslot<I>()][rowbase + ((x + (uint32_t)kCx[I]) & xmask)] = St::st(fh[I]);
+    STRH_RPUSH(1) STRH_RPUSH(2) STRH_RPUSH(3) STRH_RPUSH(4) STRH_RPUSH(5) STRH_RPUSH(6)
+    STRH_RPUSH(7) STRH_RPUSH(8) STRH_RPUSH(9) STRH_RPUSH(10) STRH_RPUSH(11) STRH_RPUSH(12)
+    STRH_RPUSH(13) STRH_RPUSH(14) STRH_RPUSH(15) STRH_RPUSH(16) STRH_RPUSH(17) STRH_RPUSH(18)
+#undef STRH_RPUSH
+    __syncthreads();
+#define STRH_RSTORE(I) \
+    if (kCx[I] != 0) St::wr_raw(fs[I], nbs + (uint32_t)o.at<0, kCy[I], kCz[I]>(), row[xslot<I>()][t]);
+    STRH_RSTORE(1) STRH_RSTORE(2) STRH_RSTORE(7) STRH_RSTORE(8) STRH_RSTORE(9) STRH_RSTORE(10)
+    STRH_RSTORE(11) STRH_RSTORE(12) STRH_RSTORE(13) STRH_RSTORE(14)
+#undef STRH_RSTORE
+    if (NT && p.writeForce != 0 && eps > 0.0f)                     // 351-353 (no walls)
+        force[n] = make_float4(ntx, nty, ntz, 0.0f);
 }
 
 // ---- moments probe: Kernels.metal:359-380 (literal loop order) ----------
@@ -454,7 +578,7 @@ moments_kernel(const typename St::T* __restrict__ f, const uint8_t* __restrict__
     float rhom1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
 #pragma unroll
     for (int i = 0; i < 19; i++) {
-        const float v = St::ld(f[(size_t)i * S + n]);
+        const float v = St::ld(f[(size_t)i * S + n]);   // plain load: probe path
         rhom1 += v;
         px += (float)kCx[i] * v;
         py += (float)kCy[i] * v;
@@ -594,44 +718,69 @@ __global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p
 }
 
 // ---- launchers -------------------------------------------------------------
-template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+template <class Conv, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
 static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
+    using T = typename Conv::T;
+    if (PARITY == 1 && !WALLS && L.rows_bs != 0) {
+        using St = Store<Conv, kPolAligned>;
+        StepParams p = L.p;
+        if (L.rows_bs == 512) {
+            p.blocks = p.N / 512u;
+            hipLaunchKernelGGL((step_odd_rows<St, 512, NT, LES, FORCE>), dim3(p.blocks), dim3(512),
+                               0, s, (T*)L.f, (float4*)L.force, L.eps, p);
+        } else {
+            p.blocks = p.N / 256u;
+            hipLaunchKernelGGL((step_odd_rows<St, 256, NT, LES, FORCE>), dim3(p.blocks), dim3(256),
+                               0, s, (T*)L.f, (float4*)L.force, L.eps, p);
+        }
+        return hipGetLastError();
+    }
+    using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
     hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(L.p.blocks),
-                       dim3(256), 0, s, (typename St::T*)L.f, L.flags, L.link, L.lidm,
+                       dim3(256), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
                        (float4*)L.force, L.eps, L.p);
     return hipGetLastError();
 }
 
-template <class St, int PARITY, bool WALLS, bool NT, bool LES>
+template <class Conv, int PARITY, bool WALLS, bool NT, bool LES>
 static hipError_t pick_force(const StepLaunch& L, hipStream_t s) {
-    return L.has_force ? launch_step_t<St, PARITY, WALLS, NT, LES, true>(L, s)
-                       : launch_step_t<St, PARITY, WALLS, NT, LES, false>(L, s);
+    return L.has_force ? launch_step_t<Conv, PARITY, WALLS, NT, LES, true>(L, s)
+                       : launch_step_t<Conv, PARITY, WALLS, NT, LES, false>(L, s);
 }
-template <class St, int PARITY, bool WALLS, bool NT>
+template <class Conv, int PARITY, bool WALLS, bool NT>
 static hipError_t pick_les(const StepLaunch& L, hipStream_t s) {
-    return L.les ? pick_force<St, PARITY, WALLS, NT, true>(L, s)
-                 : pick_force<St, PARITY, WALLS, NT, false>(L, s);
+    return L.les ? pick_force<Conv, PARITY, WALLS, NT, true>(L, s)
+                 : pick_force<Conv, PARITY, WALLS, NT, false>(L, s);
 }
-template <class St, int PARITY, bool WALLS>
+template <class Conv, int PARITY, bool WALLS>
 static hipError_t pick_nt(const StepLaunch& L, hipStream_t s) {
-    return L.nt ? pick_les<St, PARITY, WALLS, true>(L, s) : pick_les<St, PARITY, WALLS, false>(L, s);
+    return L.nt ? pick_les<Conv, PARITY, WALLS, true>(L, s) : pick_les<Conv, PARITY, WALLS, false>(L, s);
 }
-template <class St, int PARITY>
+template <class Conv, int PARITY>
 static hipError_t pick_walls(const StepLaunch& L, hipStream_t s) {
-    return L.walls ? pick_nt<St, PARITY, true>(L, s) : pick_nt<St, PARITY, false>(L, s);
+    return L.walls ? pick_nt<Conv, PARITY, true>(L, s) : pick_nt<Conv, PARITY, false>(L, s);
 }
-template <class St>
+template <class Conv>
 static hipError_t pick_parity(const StepLaunch& L, hipStream_t s) {
-    return L.p.parity == 0 ? pick_walls<St, 0>(L, s) : pick_walls<St, 1>(L, s);
+    return L.p.parity == 0 ? pick_walls<Conv, 0>(L, s) : pick_walls<Conv, 1>(L, s);
 }
 
 hipError_t launch_step(const StepLaunch& L, hipStream_t s) {
     switch (L.precision) {
-        case kFP32: return pick_parity<StF32>(L, s);
-        case kFP16S: return pick_parity<StF16>(L, s);
-        case kBF16: return pick_parity<StBF16>(L, s);
+        case kFP32: return pick_parity<ConvF32>(L, s);
+        case kFP16S: return pick_parity<ConvF16>(L, s);
+        case kBF16: return pick_parity<ConvBF16>(L, s);
     }
     return hipErrorInvalidValue;
+}
+
+// Row staging applies when nx is a power of two that tiles a 256- or
+// 512-thread workgroup and the lattice is a whole number of workgroups.
+uint32_t rows_block_size(uint32_t nx, uint64_t N) {
+    if (nx < 64 || (nx & (nx - 1)) != 0) return 0;
+    const uint32_t bs = nx <= 256 ? 256u : (nx == 512 ? 512u : 0u);
+    if (bs == 0 || N % bs != 0) return 0;
+    return bs;
 }
 
 static inline uint32_t grid_for(uint64_t n) { return (uint32_t)((n + 255u) / 256u); }
@@ -641,7 +790,7 @@ hipError_t launch_moments(int precision, const void* f, const uint8_t* flags, fl
     const dim3 g(grid_for(N)), b(256);
     switch (precision) {
         case kFP32: hipLaunchKernelGGL(moments_kernel<StF32>, g, b, 0, s, (const float*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
-        case kFP16S: hipLaunchKernelGGL(moments_kernel<StF16>, g, b, 0, s, (const _Float16*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
+        case kFP16S: hipLaunchKernelGGL(moments_kernel<StF16>, g, b, 0, s, (const uint16_t*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
         case kBF16: hipLaunchKernelGGL(moments_kernel<StBF16>, g, b, 0, s, (const uint16_t*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
         default: return hipErrorInvalidValue;
     }
@@ -654,7 +803,7 @@ hipError_t launch_init(int precision, void* f, uint32_t nx, uint32_t ny, uint32_
     const dim3 g(grid_for(N)), b(256);
     switch (precision) {
         case kFP32: hipLaunchKernelGGL(init_kernel<StF32>, g, b, 0, s, (float*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
-        case kFP16S: hipLaunchKernelGGL(init_kernel<StF16>, g, b, 0, s, (_Float16*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
+        case kFP16S: hipLaunchKernelGGL(init_kernel<StF16>, g, b, 0, s, (uint16_t*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
         case kBF16: hipLaunchKernelGGL(init_kernel<StBF16>, g, b, 0, s, (uint16_t*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
         default: return hipErrorInvalidValue;
     }
@@ -677,7 +826,7 @@ hipError_t launch_mass_sum(int precision, const void* f, uint32_t N, uint32_t S,
                            double* partials, uint32_t max_partials, double* out, hipStream_t s) {
     switch (precision) {
         case kFP32: return reduce(MassElem<StF32>{(const float*)f, S}, N, 19, partials, max_partials, out, s);
-        case kFP16S: return reduce(MassElem<StF16>{(const _Float16*)f, S}, N, 19, partials, max_partials, out, s);
+        case kFP16S: return reduce(MassElem<StF16>{(const uint16_t*)f, S}, N, 19, partials, max_partials, out, s);
         case kBF16: return reduce(MassElem<StBF16>{(const uint16_t*)f, S}, N, 19, partials, max_partials, out, s);
     }
     return hipErrorInvalidValue;

@@ -158,6 +158,7 @@ StepLaunch make_launch(const strh_lattice* L, int64_t t, bool writeForce) {
     s.nt = L->useEps;
     s.les = L->cSmago > 0.0f;
     s.has_force = !(L->bodyForce[0] == 0.0f && L->bodyForce[1] == 0.0f && L->bodyForce[2] == 0.0f);
+    s.rows_bs = L->walls ? 0u : rows_block_size((uint32_t)L->nx, L->N);
     s.p = make_params(L, t, writeForce);
     return s;
 }

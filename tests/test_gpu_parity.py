@@ -143,6 +143,41 @@ def test_noble_torczynski_with_forces(gpu, precision):
     assert np.all(np.abs(fg - fr) <= 1e-13 * scale), (fg, fr)
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_row_staged_odd_pass(gpu, precision):
+    """Wall-free lattices with nx a power of two >= 64 run the odd pass with LDS
+    row staging (256- or 512-thread workgroups); same bits as the oracle."""
+    cases = [((64, 8, 4), dict(omega=1.7)),
+             ((128, 6, 3), dict(omega=1.2, omega_minus=1.6, force=(2e-5, 1e-5, -1e-5))),
+             ((256, 2, 2), dict(omega=1.9, omega_minus=1.1, c_smago=0.04)),
+             ((512, 2, 2), dict(omega=1.6))]
+    for (nx, ny, nz), kw in cases:
+        om = kw.pop("omega")
+        omm = kw.pop("omega_minus", None)
+        g, r = make_pair(gpu, precision, nx, ny, nz, om, omm, **kw)
+        _seed_both(g, r, precision, seed=nx + ny)
+        _run_compare(g, r, [1, 2, 3], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_row_staged_noble_torczynski_force(gpu, precision):
+    nx, ny, nz = 64, 12, 4
+    rng = np.random.default_rng(8)
+    eps = np.zeros((nz, ny, nx), dtype=np.float32)
+    eps[:, 3:9, 20:30] = rng.uniform(0.0, 1.0, size=(nz, 6, 10)).astype(np.float32)
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.4, 1.7, force=(1e-5, 0, 0), wants_forces=True)
+    g.set_solid_fractions(eps)
+    r.set_solid_fractions(eps)
+    _seed_both(g, r, precision, seed=31)
+    _run_compare(g, r, [2], precision)
+    fg = g.probe_force((15, 35), (1, 10))
+    fr = r.probe_force((15, 35), (1, 10))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+    assert_bit_exact(g.download_force(), r.force_field().astype(np.float32), "per-cell force")
+    scale = np.abs(r.force_field()[:, :3]).sum() + 1e-300
+    assert np.all(np.abs(fg - fr) <= 1e-13 * scale), (fg, fr)
+
+
 def test_streaming_lone_ddf(gpu):
     """Cases.swift:85-125: omega = 0, each DDF lands at c0 + T c_i, 18 nonzero slots."""
     from strouhal_amd import Simulation
@@ -235,8 +270,13 @@ def test_init_field_matches_oracle(gpu, precision, mode):
     else:
         b = gg.astype(np.float64)
         ulp = 2.0 ** -23
-    tol = 4 * ulp * np.maximum(np.abs(a), 1e-6)
+    # GPU sinf/cosf and glibc differ by an ulp or two of the trig values; the
+    # stored sum feq + fneq carries that at the scale of each slot's largest
+    # value: allow 16 storage ulps of the slot maximum.
+    tol = 16 * ulp * np.abs(a).max(axis=1, keepdims=True)
     assert np.all(np.abs(a - b) <= tol)
+    # and the field is the same one: relative L2 difference at the FP32 level
+    assert np.linalg.norm(a - b) <= 16 * ulp * np.linalg.norm(a)
 
 
 def test_determinism_run_twice(gpu):
@@ -280,8 +320,12 @@ def test_error_paths(gpu):
     with pytest.raises(StrouhalError) as e:
         sim.probe_force((0, 3), (0, 3))
     assert e.value.code == 6
-    with pytest.raises(StrouhalError):
+    with pytest.raises(ValueError):
         sim.upload_ddf(np.zeros((19, 10), dtype=np.float32))
+    import ctypes
+    from strouhal_amd._lib import lib
+    small = np.zeros(10, dtype=np.float32)
+    assert lib().strh_upload_ddf(sim._h, small.ctypes.data_as(ctypes.c_void_p), small.nbytes) == 1
     with pytest.raises(StrouhalError):
         sim.init_field(7, 0.1)
     with pytest.raises(StrouhalError):
