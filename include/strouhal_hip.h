@@ -142,6 +142,37 @@ STRH_API int strh_download_force(strh_lattice* lat, float* out);
 STRH_API int strh_snapshot(strh_lattice* lat);
 STRH_API int strh_restore(strh_lattice* lat);
 
+/* ---- multi-lattice batch (new) -------------------------------------------
+ * Replaces the sequential rung loop of Ladder.credibility / Ladder.measure
+ * (Sources/StrouhalCore/Ladder.swift:54-121): every lattice of a batch is
+ * advanced by one launch per step, its own step count and probe schedule
+ * carried on the device, and chunks of steps are replayed as a HIP graph.
+ * Lattices must share the context and the storage precision; a batched
+ * lattice evolves bit for bit as it does alone. */
+typedef struct strh_batch strh_batch;
+STRH_API int strh_batch_create(strh_ctx* ctx, strh_lattice* const* lattices, int count,
+                               strh_batch** out);
+STRH_API int strh_batch_destroy(strh_batch* batch);
+/* Momentum-exchange samples of lattice `index` on the odd steps
+ * first_step + j*period (j < count; first_step odd, period even), each the
+ * FP64 box sum of probe_force over x in [x0,x1], y in [y0,y1], all z: the
+ * schedule of Ladder.measure (Ladder.swift:56-66) with the probes kept on the
+ * device. Needs wants_forces. */
+STRH_API int strh_batch_set_probe(strh_batch* batch, int index, int64_t first_step,
+                                  int64_t period, int64_t count, int x0, int x1, int y0, int y1);
+/* Advance lattice i by steps[i] steps (all in lock-step from the first). */
+STRH_API int strh_batch_run(strh_batch* batch, const int64_t* steps);
+/* Copy the samples taken so far (3 doubles each, Fx Fy Fz) of lattice `index`. */
+STRH_API int strh_batch_probe_series(strh_batch* batch, int index, double* out,
+                                     int64_t capacity, int64_t* taken);
+/* Workgroup size of the batch step kernel (128, 256 or 512); results do not
+ * depend on it. autotune times each on the batch (state restored after). */
+STRH_API int strh_batch_set_block_size(strh_batch* batch, int block_size);
+STRH_API int strh_batch_autotune(strh_batch* batch, int* block_size);
+/* Steps per captured HIP graph (0: plain launches). Default 64. */
+STRH_API int strh_batch_set_graph_steps(strh_batch* batch, int steps);
+STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);
+
 /* ---- host utilities (no counterpart in the reference, which hashes with
  * CryptoKit): SHA-256 of a host buffer, and a switch that forces the portable
  * block function instead of the x86 SHA extensions (returns 1 when the SHA

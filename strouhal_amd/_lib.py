@@ -28,7 +28,10 @@ EXPORTS = [
     "strh_gpu_seconds", "strh_probe_moments", "strh_probe_force", "strh_state_digest",
     "strh_mass_sum", "strh_kinetic_energy", "strh_download_ddf", "strh_upload_ddf",
     "strh_download_force", "strh_snapshot", "strh_restore", "strh_sha256",
-    "strh_sha256_force_portable",
+    "strh_sha256_force_portable", "strh_batch_create", "strh_batch_destroy",
+    "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
+    "strh_batch_set_block_size", "strh_batch_autotune", "strh_batch_set_graph_steps",
+    "strh_batch_gpu_seconds",
 ]
 
 
@@ -102,6 +105,15 @@ def lib():
         "strh_restore": [vp],
         "strh_sha256": [vp, sz, vp],
         "strh_sha256_force_portable": [i32],
+        "strh_batch_create": [vp, ctypes.POINTER(vp), i32, ctypes.POINTER(vp)],
+        "strh_batch_destroy": [vp],
+        "strh_batch_set_probe": [vp, i32, i64, i64, i64, i32, i32, i32, i32],
+        "strh_batch_run": [vp, ctypes.POINTER(i64)],
+        "strh_batch_probe_series": [vp, i32, vp, i64, ctypes.POINTER(i64)],
+        "strh_batch_set_block_size": [vp, i32],
+        "strh_batch_autotune": [vp, ctypes.POINTER(i32)],
+        "strh_batch_set_graph_steps": [vp, i32],
+        "strh_batch_gpu_seconds": [vp, ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

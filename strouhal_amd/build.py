@@ -26,7 +26,7 @@ DEVICE_FLAGS = [
 ]
 HOST_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden",
-    "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wextra",
+    "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wextra", "-Wno-unused-result",
 ]
 
 

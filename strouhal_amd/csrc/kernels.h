@@ -28,7 +28,40 @@ struct StepLaunch {
     StepParams p;
 };
 
+// One lattice of a multi-lattice batch (device-resident descriptor).
+constexpr uint32_t kBatchMaxChunks = 1024;
+struct BatchLat {
+    StepParams p;            // per-lattice constants; per-step fields set in-kernel
+    void* f;
+    const uint8_t* flags;
+    const uint32_t* link;
+    const uint32_t* lidm;
+    float4* force;
+    const float* eps;
+    uint32_t block0;         // first workgroup of this lattice in the batch grid
+    uint32_t nblocks;
+    int64_t step0;           // step count at the start of the batch run
+    int64_t tEnd;            // absolute step at which this lattice stops
+    float uinTarget;
+    float lid[3];
+    const float* ramp;       // ramp factor for t < rampSteps (host-evaluated)
+    int64_t rampSteps;
+    int32_t hasForce;
+    // momentum-exchange probe schedule: odd steps fT0 + j*fPeriod, j < fCount
+    int64_t fT0, fPeriod, fCount;
+    uint32_t bx0, by0;
+    FastDiv fbw, fbh;        // probe box width / height
+    uint32_t M, Pb, chunk;   // box cells, reduction chunks (as the single probe)
+    double* series;          // 3 doubles per sample
+};
+
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);
+hipError_t launch_batch_step(int precision, int bs, const BatchLat* lats, int L,
+                             uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
+hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, hipStream_t s);
+hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
+hipError_t launch_set_base(int64_t* base, int64_t v, hipStream_t s);
+void box_reduce_plan(uint32_t M, uint32_t max_partials, uint32_t* Pb, uint32_t* chunk);
 // Workgroup size of the LDS row-staged odd pass for this lattice, or 0.
 uint32_t rows_block_size(uint32_t nx, uint64_t N);
 hipError_t launch_moments(int precision, const void* f, const uint8_t* flags, float* out,

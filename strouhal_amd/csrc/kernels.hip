@@ -326,14 +326,21 @@ struct Offsets {   // byte offsets to the wrapped neighbour in each axis directi
     }
 };
 
+// Workgroup size of the one-cell-per-thread kernel (64..512 measured within
+// 2% of each other at 256^3; 256 is the best or tied).
+constexpr uint32_t kStepBS = 256;
+
+// One cell of one AA pass (the body shared by the single-lattice kernel and
+// the multi-lattice batch kernel).
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
-__global__ void __launch_bounds__(256)
-step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
-            const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
-            float4* __restrict__ force, const float* __restrict__ epsBuf, StepParams p) {
+__device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
+                                          const uint8_t* __restrict__ flags,
+                                          const uint32_t* __restrict__ link,
+                                          const uint32_t* __restrict__ lidm,
+                                          float4* __restrict__ force,
+                                          const float* __restrict__ epsBuf,
+                                          const StepParams& p, uint32_t n) {
     using T = typename St::T;
-    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * 256u + threadIdx.x;
-    if (n >= p.N) return;
     const size_t S = p.S;
     T* fs[19];
 #pragma unroll
@@ -484,6 +491,16 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
 #undef STRH_PUSH
     if (p.writeForce != 0 && (sM != 0u || eps > 0.0f))               // 351-353
         force[n] = make_float4(ax, ay, az, 0.0f);
+}
+
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+__global__ void __launch_bounds__(kStepBS)
+step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
+            const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
+            float4* __restrict__ force, const float* __restrict__ epsBuf, StepParams p) {
+    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * kStepBS + threadIdx.x;
+    if (n >= p.N) return;
+    step_cell<St, PARITY, WALLS, NT, LES, FORCE>(f, flags, link, lidm, force, epsBuf, p, n);
 }
 
 // ---- odd pass with LDS row staging (wall-free lattices) --------------------
@@ -717,6 +734,95 @@ __global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p
     if (k < words) p[k] = 0u;
 }
 
+// ---- multi-lattice batch ---------------------------------------------------
+// One launch advances every lattice of a batch by one step. Workgroup b
+// belongs to the lattice whose block range holds it; each lattice keeps its
+// own step index (and so its own AA parity), ramp and probe schedule. The
+// per-cell arithmetic is the single-lattice kernel's (walls, NT and LES
+// enabled are exact no-ops on lattices without them), so a batched lattice
+// evolves bit for bit as it would alone.
+__device__ __forceinline__ int batch_find(const BatchLat* lats, int L, uint32_t b) {
+    int l = 0;
+    while (l + 1 < L && b >= lats[l + 1].block0) l++;
+    return l;
+}
+
+__device__ __forceinline__ bool is_force_step(const BatchLat& B, int64_t t) {
+    if (B.fCount <= 0 || t < B.fT0) return false;
+    const int64_t d = t - B.fT0;
+    return d % B.fPeriod == 0 && d / B.fPeriod < B.fCount;
+}
+
+template <class Conv, int BS>
+__global__ void __launch_bounds__(BS)
+step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
+           const int64_t* __restrict__ base, int k) {
+    const uint32_t b = xcd_remap(blockIdx.x, totalBlocks);
+    const int l = batch_find(lats, L, b);
+    const BatchLat& B = lats[l];
+    const int64_t t = B.step0 + *base + k;
+    if (t >= B.tEnd) return;
+    const uint32_t n = (b - B.block0) * BS + threadIdx.x;
+    if (n >= B.p.N) return;
+    StepParams p = B.p;
+    p.parity = (int)(t & 1);
+    const float r = t < B.rampSteps ? B.ramp[t] : 1.0f;      // Sim.swift:258-266
+    p.uin = B.uinTarget * r;
+    p.ulidX = B.lid[0] * r;
+    p.ulidY = B.lid[1] * r;
+    p.ulidZ = B.lid[2] * r;
+    p.writeForce = is_force_step(B, t) ? 1 : 0;
+    using T = typename Conv::T;
+    T* f = (T*)B.f;
+    if (p.parity == 0) {
+        using St = Store<Conv, kPolAligned>;
+        if (B.hasForce) step_cell<St, 0, true, true, true, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        else step_cell<St, 0, true, true, true, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+    } else {
+        using St = Store<Conv, kPolShifted>;
+        if (B.hasForce) step_cell<St, 1, true, true, true, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        else step_cell<St, 1, true, true, true, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+    }
+}
+
+// After batch step k: for every lattice whose step t was a force step, sum the
+// per-cell momentum exchange over its probe box into its time series. Block
+// (l, c) sums component c of lattice l with exactly the chunking of the
+// single-lattice probe (reduce_stage1 over Pb chunks, then reduce_stage2), so
+// the FP64 values are the same bits.
+__global__ void __launch_bounds__(256)
+probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k) {
+    __shared__ double lds[256];
+    __shared__ double part[kBatchMaxChunks];
+    const int l = blockIdx.x / 3, c = blockIdx.x % 3;
+    if (l >= L) return;
+    const BatchLat& B = lats[l];
+    const int64_t t = B.step0 + *base + k;
+    if (t >= B.tEnd || !is_force_step(B, t)) return;
+    const BoxForceElem e{B.force, (uint32_t)B.p.nx, (uint32_t)B.p.ny, B.bx0, B.by0, B.fbw, B.fbh, c};
+    for (uint32_t pb = 0; pb < B.Pb; pb++) {
+        const uint32_t lo = pb * B.chunk;
+        const uint32_t hi = (B.M - lo) > B.chunk ? lo + B.chunk : B.M;
+        double acc = 0.0;
+        for (uint32_t q = lo + threadIdx.x; q < hi; q += 256u) acc += e(0, q);
+        const double sum = block_tree_sum(acc, lds);
+        if (threadIdx.x == 0) part[pb] = sum;
+        __syncthreads();
+    }
+    double acc = 0.0;
+    for (uint32_t q = threadIdx.x; q < B.Pb; q += 256u) acc += part[q];
+    const double total = block_tree_sum(acc, lds);
+    if (threadIdx.x == 0) B.series[3 * ((t - B.fT0) / B.fPeriod) + c] = total;
+}
+
+__global__ void advance_base(int64_t* base, int64_t by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *base += by;
+}
+
+__global__ void set_base(int64_t* base, int64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *base = v;
+}
+
 // ---- launchers -------------------------------------------------------------
 template <class Conv, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
 static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
@@ -736,9 +842,11 @@ static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
         return hipGetLastError();
     }
     using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
-    hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(L.p.blocks),
-                       dim3(256), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
-                       (float4*)L.force, L.eps, L.p);
+    StepParams p = L.p;
+    p.blocks = (p.N + kStepBS - 1) / kStepBS;
+    hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(p.blocks),
+                       dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
+                       (float4*)L.force, L.eps, p);
     return hipGetLastError();
 }
 
@@ -810,13 +918,13 @@ hipError_t launch_init(int precision, void* f, uint32_t nx, uint32_t ny, uint32_
     return hipGetLastError();
 }
 
+void box_reduce_plan(uint32_t M, uint32_t max_partials, uint32_t* Pb, uint32_t* chunk);
+
 template <class Elem>
 static hipError_t reduce(const Elem& e, uint32_t M, uint32_t segs, double* partials,
                          uint32_t max_partials, double* out, hipStream_t s) {
-    uint32_t Pb = (M + 4095u) / 4096u;               // >= 16 elements per thread
-    if (Pb * segs > max_partials) Pb = max_partials / segs;
-    if (Pb == 0) Pb = 1;
-    const uint32_t chunk = (uint32_t)(((uint64_t)M + Pb - 1) / Pb);
+    uint32_t Pb, chunk;                              // >= 16 elements per thread
+    box_reduce_plan(M, max_partials / segs, &Pb, &chunk);
     hipLaunchKernelGGL(reduce_stage1<Elem>, dim3(Pb, segs), dim3(256), 0, s, e, M, chunk, partials);
     hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(256), 0, s, partials, Pb * segs, out);
     return hipGetLastError();
@@ -857,6 +965,51 @@ hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
     hipLaunchKernelGGL(fill_zero_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, s,
                        (uint32_t*)p, words);
     return hipGetLastError();
+}
+
+template <int BS>
+static hipError_t launch_batch_bs(int precision, const BatchLat* lats, int L, uint32_t totalBlocks,
+                                  const int64_t* base, int k, hipStream_t s) {
+    switch (precision) {
+        case kFP32: hipLaunchKernelGGL((step_batch<ConvF32, BS>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k); break;
+        case kFP16S: hipLaunchKernelGGL((step_batch<ConvF16, BS>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k); break;
+        case kBF16: hipLaunchKernelGGL((step_batch<ConvBF16, BS>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(int precision, int bs, const BatchLat* lats, int L,
+                             uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
+    switch (bs) {
+        case 128: return launch_batch_bs<128>(precision, lats, L, totalBlocks, base, k, s);
+        case 256: return launch_batch_bs<256>(precision, lats, L, totalBlocks, base, k, s);
+        case 512: return launch_batch_bs<512>(precision, lats, L, totalBlocks, base, k, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, hipStream_t s) {
+    hipLaunchKernelGGL(probe_batch, dim3(3 * L), dim3(256), 0, s, lats, L, base, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_base(int64_t* base, int64_t v, hipStream_t s) {
+    hipLaunchKernelGGL(set_base, dim3(1), dim3(64), 0, s, base, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s) {
+    hipLaunchKernelGGL(advance_base, dim3(1), dim3(64), 0, s, base, by);
+    return hipGetLastError();
+}
+
+void box_reduce_plan(uint32_t M, uint32_t max_partials, uint32_t* Pb, uint32_t* chunk) {
+    uint32_t p = (M + 4095u) / 4096u;
+    if (p > max_partials) p = max_partials;
+    if (p == 0) p = 1;
+    *Pb = p;
+    *chunk = (uint32_t)(((uint64_t)M + p - 1) / p);
 }
 
 }  // namespace strh

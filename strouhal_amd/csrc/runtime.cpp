@@ -629,3 +629,349 @@ int strh_restore(strh_lattice* L) {
 }
 
 }  // extern "C"
+
+// ---- multi-lattice batch -----------------------------------------------------
+struct strh_batch {
+    strh_ctx* ctx = nullptr;
+    int precision = 0;
+    std::vector<strh_lattice*> lats;
+    std::vector<BatchLat> host;          // per-lattice descriptors (host)
+    BatchLat* dev = nullptr;             // per-segment compacted copies (device)
+    int64_t* base = nullptr;             // device step offset read by the kernels
+    std::vector<float*> ramp;            // per lattice, device
+    std::vector<int64_t> rampLen;
+    std::vector<double*> series;         // per lattice, device
+    std::vector<int64_t> taken;          // samples taken so far
+    size_t devCap = 0;                   // descriptors `dev` can hold
+    int bs = 256;
+    int graphSteps = 64;
+    double gpuSeconds = 0.0;
+};
+
+namespace {
+
+// Fill the per-run fields of every lattice descriptor (host side).
+int batch_prepare(strh_batch* B, const int64_t* steps) {
+    for (size_t i = 0; i < B->lats.size(); i++) {
+        strh_lattice* L = B->lats[i];
+        BatchLat& d = B->host[i];
+        d.p = make_params(L, L->stepsDone, false);
+        d.f = L->f; d.flags = L->flags; d.link = L->link; d.lidm = L->lidm;
+        d.force = (float4*)L->force; d.eps = L->eps;
+        d.step0 = L->stepsDone;
+        d.tEnd = L->stepsDone + (steps ? steps[i] : 0);
+        d.uinTarget = L->uinTarget;
+        for (int k = 0; k < 3; k++) d.lid[k] = L->lid[k];
+        d.hasForce = !(L->bodyForce[0] == 0.0f && L->bodyForce[1] == 0.0f && L->bodyForce[2] == 0.0f);
+        // ramp factors of Sim.swift:258-266, evaluated by the same host code
+        const int64_t rl = L->rampSteps > 0 ? L->rampSteps : 0;
+        if (rl != B->rampLen[i]) {
+            if (B->ramp[i]) hipFree(B->ramp[i]);
+            B->ramp[i] = nullptr;
+            if (rl > 0) HIP_TRY(hipMalloc(&B->ramp[i], rl * sizeof(float)), "hipMalloc(ramp)");
+            B->rampLen[i] = rl;
+        }
+        if (rl > 0) {
+            std::vector<float> r(rl);
+            for (int64_t t = 0; t < rl; t++) r[t] = ramp(L, t);
+            HIP_TRY(hipMemcpy(B->ramp[i], r.data(), rl * sizeof(float), hipMemcpyHostToDevice), "copy ramp");
+        }
+        d.ramp = B->ramp[i];
+        d.rampSteps = rl;
+    }
+    return STRH_OK;
+}
+
+struct Segment {
+    int64_t k0, k1;          // lock-step batch steps [k0, k1)
+    int L;                   // active lattices
+    uint32_t blocks;         // workgroups of the active lattices
+    BatchLat* dev;           // their descriptors (device)
+};
+
+int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count) {
+    hipStream_t s = B->ctx->stream;
+    for (int k = 0; k < count; k++) {
+        HIP_TRY(launch_batch_step(B->precision, bs, sg.dev, sg.L, sg.blocks, B->base, k, s),
+                "batch step launch");
+        HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, s), "batch probe launch");
+    }
+    return STRH_OK;
+}
+
+int batch_capture(strh_batch* B, const Segment& sg, hipGraphExec_t* out) {
+    hipStream_t s = B->ctx->stream;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    int rc = batch_launch_steps(B, sg, B->bs, B->graphSteps);
+    hipError_t e2 = launch_advance_base(B->base, B->graphSteps, s);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e2 != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(e2, "capture advance"); }
+    if (e != hipSuccess) return hip_fail(e, "end capture");
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) { *out = nullptr; return hip_fail(e, "graph instantiate"); }
+    return STRH_OK;
+}
+
+// Advance the batch by steps[i] per lattice. The run is split where lattices
+// finish: each segment launches only the still-active lattices (compacted
+// descriptors and grid), as whole captured graphs of graphSteps steps plus
+// single launches for the remainder. Kernels read t = step0 + base + k, base
+// being set at each segment start and advanced by every graph replay.
+int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probes, double* seconds) {
+    hipStream_t s = B->ctx->stream;
+    const int n = (int)B->lats.size();
+    std::vector<int64_t> ends;
+    for (int i = 0; i < n; i++) if (steps[i] > 0) ends.push_back(steps[i]);
+    std::sort(ends.begin(), ends.end());
+    ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+    if (ends.empty()) { if (seconds) *seconds = 0.0; return STRH_OK; }
+    // compact descriptor arrays of every segment, uploaded once
+    std::vector<Segment> segs;
+    std::vector<BatchLat> up;
+    int64_t k0 = 0;
+    for (int64_t e : ends) {
+        Segment sg;
+        sg.k0 = k0; sg.k1 = e; sg.L = 0; sg.blocks = 0; sg.dev = nullptr;
+        for (int i = 0; i < n; i++) {
+            if (steps[i] <= k0) continue;
+            BatchLat d = B->host[i];
+            d.block0 = sg.blocks;
+            d.nblocks = (B->lats[i]->N + (uint32_t)B->bs - 1) / (uint32_t)B->bs;
+            if (!probes) d.fCount = 0;
+            sg.blocks += d.nblocks;
+            up.push_back(d);
+            sg.L++;
+        }
+        segs.push_back(sg);
+        k0 = e;
+    }
+    if (up.size() > B->devCap) {
+        if (B->dev) hipFree(B->dev);
+        B->dev = nullptr;
+        B->devCap = 0;
+        HIP_TRY(hipMalloc(&B->dev, up.size() * sizeof(BatchLat)), "hipMalloc(batch descriptors)");
+        B->devCap = up.size();
+    }
+    HIP_TRY(hipStreamSynchronize(s), "batch descriptors");
+    HIP_TRY(hipMemcpy(B->dev, up.data(), up.size() * sizeof(BatchLat), hipMemcpyHostToDevice),
+            "upload batch descriptors");
+    size_t off = 0;
+    for (Segment& sg : segs) { sg.dev = B->dev + off; off += sg.L; }
+
+    HIP_TRY(hipEventRecord(B->ctx->ev0, s), "hipEventRecord");
+    for (const Segment& sg : segs) {
+        HIP_TRY(launch_set_base(B->base, sg.k0, s), "set base");
+        const int64_t K = sg.k1 - sg.k0;
+        int64_t done = 0;
+        if (useGraph && B->graphSteps > 0 && K >= B->graphSteps) {
+            hipGraphExec_t g = nullptr;
+            int rc = batch_capture(B, sg, &g);
+            if (rc) return rc;
+            while (K - done >= B->graphSteps) {
+                hipError_t e = hipGraphLaunch(g, s);
+                if (e != hipSuccess) { hipGraphExecDestroy(g); return hip_fail(e, "graph launch"); }
+                done += B->graphSteps;
+            }
+            // the exec may be destroyed once its launches are queued
+            hipError_t e = hipStreamSynchronize(s);
+            hipGraphExecDestroy(g);
+            if (e != hipSuccess) return hip_fail(e, "graph replays");
+        }
+        if (done < K) {
+            int rc = batch_launch_steps(B, sg, B->bs, (int)(K - done));
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(B->ctx->ev1, s), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(B->ctx->ev1), "batch steps");
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, B->ctx->ev0, B->ctx->ev1), "hipEventElapsedTime");
+    if (seconds) *seconds = ms * 1e-3;
+    return STRH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_batch** out) {
+    if (!c || !lats || !out || count <= 0) return fail(STRH_ERR_INVALID, "bad batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < count; i++) {
+        if (!lats[i] || lats[i]->ctx != c) return fail(STRH_ERR_INVALID, "lattice %d is not on this context", i);
+        if (lats[i]->precision != lats[0]->precision) return fail(STRH_ERR_INVALID, "lattices must share one precision");
+        for (int j = 0; j < i; j++)
+            if (lats[j] == lats[i]) return fail(STRH_ERR_INVALID, "lattice %d appears twice", i);
+    }
+    int rc = set_device(c);
+    if (rc) return rc;
+    strh_batch* B = new strh_batch();
+    B->ctx = c;
+    B->precision = lats[0]->precision;
+    B->lats.assign(lats, lats + count);
+    B->host.assign(count, BatchLat{});
+    B->ramp.assign(count, nullptr);
+    B->rampLen.assign(count, 0);
+    B->series.assign(count, nullptr);
+    B->taken.assign(count, 0);
+    auto bail = [&](hipError_t e, const char* what) {
+        strh_batch_destroy(B);
+        return hip_fail(e, what);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&B->base, sizeof(int64_t))) != hipSuccess) return bail(e, "hipMalloc(base)");
+    // The batch kernel runs the walls path everywhere: wall-free lattices get
+    // all-zero link words (identical arithmetic, every neighbour fluid).
+    for (int i = 0; i < count; i++) {
+        strh_lattice* L = lats[i];
+        if (!L->link) {
+            if ((e = hipMalloc(&L->link, (size_t)L->N * 4)) != hipSuccess) return bail(e, "hipMalloc(link)");
+            if ((e = hipMemset(L->link, 0, (size_t)L->N * 4)) != hipSuccess) return bail(e, "zero link");
+        }
+    }
+    *out = B;
+    return STRH_OK;
+}
+
+int strh_batch_destroy(strh_batch* B) {
+    if (!B) return STRH_OK;
+    hipSetDevice(B->ctx->device);
+    for (float* r : B->ramp) if (r) hipFree(r);
+    for (double* d : B->series) if (d) hipFree(d);
+    if (B->dev) hipFree(B->dev);
+    if (B->base) hipFree(B->base);
+    delete B;
+    return STRH_OK;
+}
+
+int strh_batch_set_probe(strh_batch* B, int index, int64_t first, int64_t period, int64_t count,
+                         int x0, int x1, int y0, int y1) {
+    if (!B || index < 0 || index >= (int)B->lats.size()) return fail(STRH_ERR_INVALID, "bad batch index");
+    int rc = set_device(B->ctx);
+    if (rc) return rc;
+    strh_lattice* L = B->lats[index];
+    if (!L->wantsForces) return fail(STRH_ERR_NO_FORCES, "lattice %d created without wants_forces", index);
+    if (count < 0 || (count > 0 && (period <= 0 || (period & 1) || !(first & 1))))
+        return fail(STRH_ERR_INVALID, "probe steps must be odd with an even period");
+    if (x0 < 0 || x1 >= L->nx || x0 > x1 || y0 < 0 || y1 >= L->ny || y0 > y1)
+        return fail(STRH_ERR_INVALID, "probe box outside the lattice");
+    BatchLat& d = B->host[index];
+    const uint32_t bw = (uint32_t)(x1 - x0 + 1), bh = (uint32_t)(y1 - y0 + 1);
+    d.M = bw * bh * (uint32_t)L->nz;
+    box_reduce_plan(d.M, kMaxPartials, &d.Pb, &d.chunk);
+    if (d.Pb > kBatchMaxChunks) return fail(STRH_ERR_INVALID, "probe box too large for a batched probe");
+    d.bx0 = (uint32_t)x0; d.by0 = (uint32_t)y0;
+    d.fbw = make_fastdiv(bw); d.fbh = make_fastdiv(bh);
+    d.fT0 = first; d.fPeriod = period; d.fCount = count;
+    if (B->series[index]) hipFree(B->series[index]);
+    B->series[index] = nullptr;
+    if (count > 0) HIP_TRY(hipMalloc(&B->series[index], (size_t)count * 3 * sizeof(double)), "hipMalloc(series)");
+    d.series = B->series[index];
+    B->taken[index] = 0;
+    // Cells outside the momentum-exchange set are never written on force
+    // steps; one zero fill here stands for the per-probe clear of Sim.swift:361.
+    HIP_TRY(launch_fill_zero(L->force, (size_t)L->N * 16, B->ctx->stream), "zero force");
+    HIP_TRY(hipStreamSynchronize(B->ctx->stream), "zero force");
+    return STRH_OK;
+}
+
+int strh_batch_run(strh_batch* B, const int64_t* steps) {
+    if (!B || !steps) return fail(STRH_ERR_INVALID, "null argument");
+    int rc = set_device(B->ctx);
+    if (rc) return rc;
+    int64_t K = 0;
+    for (size_t i = 0; i < B->lats.size(); i++) {
+        if (steps[i] < 0) return fail(STRH_ERR_INVALID, "negative step count");
+        K = std::max(K, steps[i]);
+    }
+    if (K == 0) return STRH_OK;
+    if ((rc = batch_prepare(B, steps))) return rc;
+    double sec = 0.0;
+    if ((rc = batch_execute(B, steps, true, true, &sec))) return rc;
+    B->gpuSeconds += sec;
+    for (size_t i = 0; i < B->lats.size(); i++) {
+        strh_lattice* L = B->lats[i];
+        const BatchLat& d = B->host[i];
+        if (d.fCount > 0) {
+            // samples whose step fell inside [step0, tEnd)
+            const int64_t last = d.tEnd - 1;
+            int64_t n = last < d.fT0 ? 0 : (last - d.fT0) / d.fPeriod + 1;
+            B->taken[i] = std::min<int64_t>(std::max<int64_t>(n, 0), d.fCount);
+        }
+        L->stepsDone += steps[i];
+        L->gpuSeconds += sec;
+    }
+    return STRH_OK;
+}
+
+int strh_batch_probe_series(strh_batch* B, int index, double* out, int64_t capacity, int64_t* taken) {
+    if (!B || index < 0 || index >= (int)B->lats.size() || !taken) return fail(STRH_ERR_INVALID, "bad arguments");
+    int rc = set_device(B->ctx);
+    if (rc) return rc;
+    *taken = B->taken[index];
+    const int64_t n = std::min(capacity, B->taken[index]);
+    if (n > 0 && out)
+        HIP_TRY(hipMemcpy(out, B->series[index], (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost), "copy series");
+    return STRH_OK;
+}
+
+int strh_batch_set_block_size(strh_batch* B, int bs) {
+    if (!B || (bs != 128 && bs != 256 && bs != 512)) return fail(STRH_ERR_INVALID, "block size must be 128, 256 or 512");
+    B->bs = bs;
+    return STRH_OK;
+}
+
+int strh_batch_set_graph_steps(strh_batch* B, int steps) {
+    if (!B || steps < 0 || steps > 4096) return fail(STRH_ERR_INVALID, "graph steps must be in [0, 4096]");
+    B->graphSteps = steps;
+    return STRH_OK;
+}
+
+int strh_batch_gpu_seconds(strh_batch* B, double* seconds) {
+    if (!B || !seconds) return fail(STRH_ERR_INVALID, "null argument");
+    *seconds = B->gpuSeconds;
+    return STRH_OK;
+}
+
+// Times 16 steps of the batch at each workgroup size, without probes, and
+// restores every lattice's DDFs afterwards (the state is copied aside on the
+// device). Keeps the fastest; ties go to the smaller size.
+int strh_batch_autotune(strh_batch* B, int* chosen) {
+    if (!B) return fail(STRH_ERR_INVALID, "null batch");
+    int rc = set_device(B->ctx);
+    if (rc) return rc;
+    std::vector<void*> saved(B->lats.size(), nullptr);
+    auto release = [&]() { for (void* p : saved) if (p) hipFree(p); };
+    for (size_t i = 0; i < B->lats.size(); i++) {
+        const size_t bytes = 19 * slot_bytes(B->lats[i]);
+        if (hipMalloc(&saved[i], bytes) != hipSuccess) { release(); return fail(STRH_ERR_ALLOC, "autotune copy"); }
+        if (hipMemcpy(saved[i], B->lats[i]->f, bytes, hipMemcpyDeviceToDevice) != hipSuccess) {
+            release(); return fail(STRH_ERR_DEVICE, "autotune copy");
+        }
+    }
+    const int keep = B->bs;
+    std::vector<int64_t> steps(B->lats.size(), 16);
+    double best = 1e30;
+    int bestBs = keep;
+    for (int bs : {128, 256, 512}) {
+        B->bs = bs;
+        if ((rc = batch_prepare(B, steps.data()))) break;
+        double sec = 0.0;
+        std::vector<int64_t> warm(B->lats.size(), 4);
+        if ((rc = batch_execute(B, warm.data(), false, false, &sec))) break;
+        if ((rc = batch_execute(B, steps.data(), false, false, &sec))) break;
+        if (sec < best * 0.98) { best = sec; bestBs = bs; }
+    }
+    for (size_t i = 0; i < B->lats.size(); i++)
+        hipMemcpy(B->lats[i]->f, saved[i], 19 * slot_bytes(B->lats[i]), hipMemcpyDeviceToDevice);
+    release();
+    if (rc) { B->bs = keep; return rc; }
+    B->bs = bestBs;
+    if (chosen) *chosen = bestBs;
+    return STRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""Resolution ladder of the Karman vortex street, sequential and batched.
+
+Restates the schedule and statistics the reference builds around the hot
+path:
+
+* ``street_variant``  StreetLadder.variant (Sources/StrouhalCore/CaseSetups.swift:177-201)
+* ``probe_schedule``  Ladder.measure's run/probe loop (Sources/StrouhalCore/Ladder.swift:54-71)
+* ``cycle_stats``     DFG 2D-2 statistics: Strouhal number from the linearly
+                      interpolated upward zero crossings of C_L, max C_D / C_L and
+                      mean C_D over complete cycles (Sources/strouhal/Cases.swift:911-947)
+
+``run_sequential`` is the reference's procedure (one lattice after another,
+each probe a host round trip); ``run_batched`` advances all lattices in one
+batch with the probes on the device. Both produce identical DDFs and
+identical force series.
+
+Run as a script for the configuration-3 ladder (D in {16,24,32,48,64} at
+Ma 0.1 plus half-Mach anchors) and print one JSON line:
+
+    python3 strouhal_amd/ladder.py --scale 1.0 --precision fp32
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from strouhal_amd.batch import Batch  # noqa: E402
+from strouhal_amd.cases import vortex_street  # noqa: E402
+
+MA_DEFAULT = 0.1
+
+
+@dataclass
+class Variant:
+    D: int
+    mach_scale: float
+    case: object          # cases.StreetCase
+    transient: int
+    sample: int
+    stride: int
+
+    @property
+    def chunk(self) -> int:
+        return max(2, self.stride - 2) & ~1
+
+    @property
+    def probes(self) -> int:
+        per = self.chunk + 2
+        return -(-self.sample // per)
+
+    @property
+    def total_steps(self) -> int:
+        return self.transient + self.probes * (self.chunk + 2)
+
+
+def street_variant(gpu, precision, D: int, mach_scale: float, base_umax: float,
+                   scale_steps: float = 1.0) -> Variant:
+    """CaseSetups.swift:177-201 with u_max = base_umax * mach_scale and the
+    step schedule scaled by `scale_steps` (1.0 = the reference's)."""
+    u = base_umax * mach_scale
+    case = vortex_street(gpu, precision, D=D, uin_max=u, re=100.0)
+    scale = D / 40.0 / mach_scale
+    transient = int(60_000 * scale * scale_steps) & ~1
+    sample = int(120_000 * scale * scale_steps) & ~1
+    stride = max(8, int(26 * scale)) & ~1
+    return Variant(D, mach_scale, case, transient, sample, stride)
+
+
+def probe_schedule(v: Variant):
+    """(first force step, period, count) of Ladder.measure's probes."""
+    return v.transient + v.chunk + 1, v.chunk + 2, v.probes
+
+
+def cycle_stats(cl, cd, t, D: int, u_mean: float) -> dict:
+    """Cases.swift:911-947 on sampled C_L / C_D series at steps t."""
+    cross = []
+    for k in range(1, len(cl)):
+        if cl[k - 1] < 0 <= cl[k]:
+            frac = -cl[k - 1] / (cl[k] - cl[k - 1])
+            cross.append(t[k - 1] + frac * (t[k] - t[k - 1]))
+    cycles = max(len(cross) - 1, 0)
+    st = 0.0
+    max_cd, max_cl, means = 0.0, -math.inf, []
+    if cycles >= 2:
+        period = (cross[-1] - cross[0]) / cycles
+        st = D / (period * u_mean)
+        for c in range(cycles):
+            lo, hi = cross[c], cross[c + 1]
+            sel = [k for k in range(len(cl)) if lo <= t[k] < hi]
+            if sel:
+                max_cd = max(max_cd, max(cd[k] for k in sel))
+                max_cl = max(max_cl, max(cl[k] for k in sel))
+                means.append(sum(cd[k] for k in sel) / len(sel))
+    return {"strouhal": st, "cycles": cycles, "max_cd": max_cd,
+            "max_cl": max_cl if cycles >= 2 else 0.0,
+            "mean_cd": float(np.mean(means)) if len(means) >= 1 else 0.0}
+
+
+def coefficients(v: Variant, forces: np.ndarray):
+    um = v.case.u_mean
+    cd = 2.0 * forces[:, 0] / (um * um * v.D)
+    cl = 2.0 * forces[:, 1] / (um * um * v.D)
+    first, period, count = probe_schedule(v)
+    t = first + 1 + period * np.arange(len(forces))   # stepsDone after each probe
+    return cd, cl, t
+
+
+def build(gpu, precision, resolutions, scale_steps, base_umax):
+    out = []
+    for D in resolutions:
+        for ms in (1.0, 0.5):
+            out.append(street_variant(gpu, precision, D, ms, base_umax, scale_steps))
+    return out
+
+
+def run_sequential(variants):
+    """Ladder.measure for each lattice in turn (probe = host round trip)."""
+    series = []
+    t0 = time.perf_counter()
+    for v in variants:
+        sim = v.case.sim
+        sim.run(v.transient)
+        f = []
+        for _ in range(v.probes):
+            sim.run(v.chunk)
+            f.append(sim.probe_force(v.case.box_x, v.case.box_y))
+        series.append(np.array(f))
+    return series, time.perf_counter() - t0
+
+
+def run_batched(variants, autotune=True, graph_steps=64):
+    b = Batch([v.case.sim for v in variants])
+    for i, v in enumerate(variants):
+        first, period, count = probe_schedule(v)
+        b.set_probe(i, first, period, count, v.case.box_x, v.case.box_y)
+    bs = b.autotune() if autotune else 256
+    b.set_graph_steps(graph_steps)
+    t0 = time.perf_counter()
+    b.run([v.total_steps for v in variants])
+    wall = time.perf_counter() - t0
+    series = [b.probe_series(i) for i in range(len(variants))]
+    return series, wall, bs, b
+
+
+def main():
+    import argparse
+    from strouhal_amd import default_gpu
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the reference schedule")
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--resolutions", default="16,24,32,48,64")
+    ap.add_argument("--mach", type=float, default=MA_DEFAULT)
+    ap.add_argument("--no-sequential", action="store_true")
+    args = ap.parse_args()
+    res = [int(r) for r in args.resolutions.split(",")]
+    base_umax = 1.5 * args.mach / math.sqrt(3.0)      # Ma of U_mean = umax * 2/3 * sqrt(3)
+    gpu = default_gpu()
+    report = {"workload": f"DFG 2D-2 street ladder D={res} x (Ma {args.mach}, half-Mach), "
+                          f"schedule x{args.scale}", "precision": args.precision}
+    vb = build(gpu, args.precision, res, args.scale, base_umax)
+    cells = sum(v.case.sim.cells for v in vb)
+    updates = sum(v.case.sim.cells * v.total_steps for v in vb)
+    sb, wb, bs, _ = run_batched(vb)
+    report.update({"lattices": len(vb), "cells": cells, "cell_updates": updates,
+                   "batched_s": wb, "batched_mlups": updates / wb / 1e6, "block_size": bs})
+    rungs = []
+    for v, s in zip(vb, sb):
+        cd, cl, t = coefficients(v, s)
+        st = cycle_stats(cl, cd, t, v.D, v.case.u_mean)
+        rungs.append({"D": v.D, "mach_scale": v.mach_scale, "steps": v.total_steps,
+                      "samples": len(s), "finite": bool(np.all(np.isfinite(s))), **st,
+                      "digest": v.case.sim.state_digest[:16]})
+    report["rungs"] = rungs
+    if not args.no_sequential:
+        vs = build(gpu, args.precision, res, args.scale, base_umax)
+        ss, ws = run_sequential(vs)
+        same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(sb, ss)) and all(
+            a.case.sim.state_digest == b.case.sim.state_digest for a, b in zip(vb, vs))
+        report.update({"sequential_s": ws, "speedup": ws / wb, "bitwise_identical": bool(same)})
+    print(json.dumps(report), flush=True)
+
+
+if __name__ == "__main__":
+    main()

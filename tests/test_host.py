@@ -126,3 +126,17 @@ def test_trt_omegas_and_flags():
     assert trt_omegas(0.6, None) == (float(np.float32(1 / 0.6)),) * 2
     fl = flags_array(lambda x, y, z: np.where(y == 0, Cell.solid, Cell.fluid), 5, 4, 3)
     assert fl.shape == (3, 4, 5) and fl[:, 0, :].min() == 1 and fl[:, 1:, :].max() == 0
+
+
+def test_cycle_stats_on_a_sine():
+    from strouhal_amd.ladder import cycle_stats
+    """Strouhal from upward zero crossings of a pure sine equals its frequency."""
+    t = np.arange(0, 4000, 7.0)
+    D, um, f = 20, 0.05, 1.0 / 400.0
+    cl = 0.8 * np.sin(2 * np.pi * f * t + 0.3)
+    cd = 1.5 + 0.1 * np.sin(4 * np.pi * f * t)
+    s = cycle_stats(cl, cd, t, D, um)
+    assert s["cycles"] >= 8
+    assert s["strouhal"] == pytest.approx(f * D / um, rel=1e-3)
+    assert s["max_cl"] == pytest.approx(0.8, abs=2e-3)
+    assert s["mean_cd"] == pytest.approx(1.5, abs=1e-3)
