@@ -45,7 +45,8 @@ enum {
     STRH_ERR_ALLOC = 4,       /* device or host allocation failed (Sim.swift:221) */
     STRH_ERR_ODD_STEPS = 5,   /* probe needs an even step count (Sim.swift:335, 360) */
     STRH_ERR_NO_FORCES = 6,   /* probe_force on a lattice created without forces */
-    STRH_ERR_NOT_FINITE = 7   /* a reduction produced a non-finite value */
+    STRH_ERR_NOT_FINITE = 7,  /* a reduction produced a non-finite value */
+    STRH_ERR_UNSUPPORTED = 8  /* the lattice uses a feature the entry point does not cover */
 };
 
 /* Storage precision. Sim.swift:5-8 has fp32 and fp16s; bf16 is new. */
@@ -100,6 +101,13 @@ STRH_API int strh_set_inflow_uniform(strh_lattice* lat, int enabled);
 STRH_API int strh_set_sponge(strh_lattice* lat, int enabled, float x0, float width, float tau);
 /* Noble-Torczynski solid fractions, nx*ny*nz floats (Sim.swift:413-421). */
 STRH_API int strh_set_solid_fractions(strh_lattice* lat, const float* eps);
+/* Brinkman porosity, nx*ny*nz floats >= 0 (no reference counterpart: the
+ * reference has only the solid fractions above). Each fluid cell feels the
+ * drag -alpha rho u with u = (p + F/2) / (rho (1 + alpha/2)), applied through
+ * the Guo forcing terms; alpha = 0 everywhere gives the plain forced update.
+ * NULL removes the field. Not combinable with solid fractions or LES, and not
+ * accepted by batches (STRH_ERR_INVALID). */
+STRH_API int strh_set_porosity(strh_lattice* lat, const float* alpha);
 
 /* ---- initialisation: Simulation.initField (Sim.swift:424-439) ------------
  * mode 0 rest, 1 2-D Taylor-Green, 2 3-D Taylor-Green vortex; tau = 1/omega. */
@@ -172,6 +180,45 @@ STRH_API int strh_batch_autotune(strh_batch* batch, int* block_size);
 /* Steps per captured HIP graph (0: plain launches). Default 64. */
 STRH_API int strh_batch_set_graph_steps(strh_batch* batch, int steps);
 STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);
+
+/* ---- discrete adjoint (no reference counterpart: the reference has no
+ * gradients) ------------------------------------------------------------------
+ * Objective over the T = spinup + avg steps from the current state, evaluated
+ * on the pre-collision moments of each step k >= spinup and averaged over avg:
+ *   J = (1/avg) sum_k [ w_p (rho_in - rho_out) / 3 + w_d sum_x alpha(x) J_x(x) ]
+ * rho_in / rho_out: mean density of the fluid cells of column x = inlet_x /
+ * outlet_x (-1: no column); J_x = (p_x + F_x/2) / (1 + alpha/2) is the
+ * momentum the Brinkman drag acts on. The lattice may use walls of every
+ * kind, Guo forcing, TRT/SRT and porosity; partial cells, LES and the sponge
+ * give STRH_ERR_UNSUPPORTED. */
+typedef struct strh_objective {
+    int32_t inlet_x, outlet_x;
+    double w_p, w_d;
+    int64_t spinup, avg;
+} strh_objective;
+
+typedef struct strh_vjp_info {
+    int64_t checkpoint_every;   /* steps per recomputed segment */
+    int64_t checkpoints;        /* DDF images kept */
+    double forward_seconds;     /* first forward pass (objective, checkpoints) */
+    double recompute_seconds;   /* segment replays that record the states */
+    double backward_seconds;    /* reverse steps */
+    uint64_t workspace_bytes;   /* device memory of the call beyond the lattice */
+} strh_vjp_info;
+
+/* Advance T steps and return J (FP64, fixed-order reduction). */
+STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, double* J);
+/* Advance T steps, return J and its gradient with respect to the porosity of
+ * every cell (grad_alpha: N doubles, or NULL; zero on non-fluid cells) and to
+ * the inlet velocity target uin (grad_uin, or NULL). The state after the call
+ * is the state after strh_run(lat, T), bit for bit. The backward pass replays
+ * segments of checkpoint_every steps (0: ceil(sqrt(T))) from DDF checkpoints,
+ * recording the pre-collision populations of one segment at a time in FP32;
+ * the adjoint state is FP32 and every accumulation FP64 in a fixed order.
+ * Device memory: ceil(T/K) DDF images + K * 19 * N floats + 2 * 19 * N floats
+ * + 3 * N doubles. info may be NULL. */
+STRH_API int strh_vjp(strh_lattice* lat, const strh_objective* obj, int64_t checkpoint_every,
+                      double* J, double* grad_alpha, double* grad_uin, strh_vjp_info* info);
 
 /* ---- host utilities (no counterpart in the reference, which hashes with
  * CryptoKit): SHA-256 of a host buffer, and a switch that forces the portable

@@ -50,6 +50,15 @@ class HostParams(ctypes.Structure):
     ]
 
 
+class Objective(ctypes.Structure):
+    """The differentiable channel objective (oracle/strouhal_oracle.cpp `Objective`)."""
+    _fields_ = [
+        ("inlet_x", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
+        ("w_p", ctypes.c_double), ("w_d", ctypes.c_double),
+        ("spinup", ctypes.c_int64), ("avg", ctypes.c_int64),
+    ]
+
+
 def build() -> str:
     """Compile the oracle with its Makefile (gcc); returns the library path."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -70,6 +79,11 @@ def lib():
         L.orc_set_host_params.argtypes = [vp, ctypes.POINTER(HostParams)]
         L.orc_set_threads.argtypes = [vp, i32]
         L.orc_set_solid_fractions.argtypes = [vp, ctypes.c_void_p]
+        L.orc_set_porosity.argtypes = [vp, ctypes.c_void_p]
+        L.orc_objective.argtypes = [vp, ctypes.POINTER(Objective)]
+        L.orc_objective.restype = ctypes.c_double
+        L.orc_vjp.argtypes = [vp, ctypes.POINTER(Objective), ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_vjp.restype = ctypes.c_double
         L.orc_init_field.argtypes = [vp, i32, ctypes.c_double]
         L.orc_run.argtypes = [vp, i64]
         L.orc_steps_done.argtypes = [vp]
@@ -191,6 +205,35 @@ class OracleSim:
         if eps.size != self.cells:
             raise ValueError("eps must hold nx*ny*nz values")
         lib().orc_set_solid_fractions(self._h, _ptr(eps))
+
+    def set_porosity(self, alpha):
+        """Brinkman porosity alpha per cell (drag force -alpha rho u)."""
+        a = np.ascontiguousarray(alpha, dtype=np.float32).reshape(-1)
+        if a.size != self.cells:
+            raise ValueError("alpha must hold nx*ny*nz values")
+        lib().orc_set_porosity(self._h, _ptr(a))
+
+    @staticmethod
+    def _objective(inlet_x, outlet_x, w_p, w_d, spinup, avg):
+        o = Objective()
+        o.inlet_x, o.outlet_x, o.w_p, o.w_d = int(inlet_x), int(outlet_x), float(w_p), float(w_d)
+        o.spinup, o.avg = int(spinup), int(avg)
+        return o
+
+    def objective(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0) -> float:
+        """Run spinup + avg steps (FP64 only) and return the time-averaged objective."""
+        self._push()
+        o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg)
+        return float(lib().orc_objective(self._h, ctypes.byref(o)))
+
+    def vjp(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0):
+        """FP64 discrete adjoint: (J, dJ/dalpha per cell, dJ/d uin_target)."""
+        self._push()
+        o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg)
+        ga = np.zeros(self.cells, dtype=np.float64)
+        gu = np.zeros(1, dtype=np.float64)
+        J = float(lib().orc_vjp(self._h, ctypes.byref(o), _ptr(ga), _ptr(gu)))
+        return J, ga, float(gu[0])
 
     def init_field(self, mode: int, amplitude: float):
         self._push()

@@ -42,6 +42,8 @@
 #include <cstring>
 #include <vector>
 #include <memory>
+#include <algorithm>
+#include <type_traits>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -138,6 +140,17 @@ struct HostParams {
     double sponge_x0, sponge_width, sponge_tau;
 };
 
+// Time-averaged objective of the differentiable channel (new; no reference
+// counterpart): J = (1/T) sum over the T steps after `spinup` of
+//   w_p * (mean rho(inlet column) - mean rho(outlet column)) / 3
+// + w_d * sum_x alpha(x) J_x(x)       (the Brinkman drag, J = rho u)
+// with rho, J the pre-collision moments of each step.
+struct Objective {
+    int32_t inlet_x, outlet_x;
+    double w_p, w_d;
+    int64_t spinup, avg;
+};
+
 // Per-dispatch block, Kernels.metal:49-63.
 template <class R> struct Params {
     int nx, ny, nz, parity;
@@ -167,6 +180,7 @@ struct Base {
     bool wantsForces = false;
     int threads = 1;
     virtual void setEps(const float* eps) = 0;
+    virtual void setAlpha(const float* alpha) = 0;
     virtual void initField(int mode, double amplitude) = 0;
     virtual void run(int64_t steps, int64_t forceStep) = 0;
     virtual int moments(double* out) = 0;
@@ -178,6 +192,8 @@ struct Base {
     virtual void upload(const void* in) = 0;
     virtual void collideCell(const double* fin, double eps, int x, double* fout,
                              double* rho, double* ntf) = 0;
+    virtual double vjp(const Objective& o, double* gAlpha, double* gUin) = 0;
+    virtual double objective(const Objective& o) = 0;
 };
 
 template <class S> struct Lattice final : Base {
@@ -187,6 +203,8 @@ template <class S> struct Lattice final : Base {
     std::vector<R> force;   // (x, y, z, 0) per cell, float4 in the reference
     std::vector<R> eps;     // NT solid fraction (Sim.swift:413-421)
     bool useEps = false;
+    std::vector<R> alpha;   // Brinkman porosity (new)
+    bool useAlpha = false;
 
     Lattice(int nx_, int ny_, int nz_, const uint8_t* fl, bool wants) {
         nx = nx_; ny = ny_; nz = nz_;
@@ -220,6 +238,12 @@ template <class S> struct Lattice final : Base {
             }
             smask[idx] = s; lmask[idx] = l;
         }
+    }
+
+    void setAlpha(const float* a) override {
+        alpha.resize(N);
+        for (size_t n = 0; n < N; n++) alpha[n] = (R)a[n];
+        useAlpha = true;
     }
 
     void setEps(const float* e) override {
@@ -269,8 +293,13 @@ template <class S> struct Lattice final : Base {
     }
 
     // Kernels.metal:105-228
-    static void collide(R* fh, const Params<R>& p, int x, R epsv, R& rhoOut,
-                        R& ntx, R& nty, R& ntz) {
+    // alphav / brink: Brinkman porosity (new in this build, no reference
+    // counterpart): the cell feels the drag force -alpha rho u, with u taken
+    // implicitly, u = (p + Fb/2) / (rho (1 + alpha/2)), so the force enters
+    // the Guo terms as F = Fb - alpha rho u. With alpha = 0 every value is
+    // the same bits as without it.
+    static void collide(R* fh, const Params<R>& p, int x, R epsv, R alphav, bool brink,
+                        R& rhoOut, R& ntx, R& nty, R& ntz) {
         R wp = p.omega, wm = p.omegaMinus;
         if ((R)x > p.spongeX0) {                                   // 112-123
             const R frac = std::fmin(((R)x - p.spongeX0) * p.spongeInvW, R(1));
@@ -295,9 +324,16 @@ template <class S> struct Lattice final : Base {
             py += (R)CY[i] * fh[i];
             pz += (R)CZ[i] * fh[i];
         }
-        const R ux = (px + R(0.5) * p.fx) * inv;                   // 135-138
-        const R uy = (py + R(0.5) * p.fy) * inv;
-        const R uz = (pz + R(0.5) * p.fz) * inv;
+        R ux = (px + R(0.5) * p.fx) * inv;                         // 135-138
+        R uy = (py + R(0.5) * p.fy) * inv;
+        R uz = (pz + R(0.5) * p.fz) * inv;
+        R Fx = p.fx, Fy = p.fy, Fz = p.fz;
+        if (brink) {
+            const R g = R(1) + R(0.5) * alphav;
+            ux = ux / g; uy = uy / g; uz = uz / g;
+            const R ar = alphav * rho;
+            Fx = p.fx - ar * ux; Fy = p.fy - ar * uy; Fz = p.fz - ar * uz;
+        }
         const R u2 = ux * ux + uy * uy + uz * uz;
 
         if (p.cSmago > R(0)) {                                     // 140-167
@@ -326,7 +362,7 @@ template <class S> struct Lattice final : Base {
             }
         }
 
-        const R uF = ux * p.fx + uy * p.fy + uz * p.fz;             // 169-171
+        const R uF = ux * Fx + uy * Fy + uz * Fz;                   // 169-171
         const R ap = R(1) - R(0.5) * wp;
         const R am = R(1) - R(0.5) * wm;
 
@@ -339,7 +375,7 @@ template <class S> struct Lattice final : Base {
             for (int i = 1; i < 19; i += 2) {
                 const R W = weight<R>(i);
                 const R cu = R(3) * ((R)CX[i] * ux + (R)CY[i] * uy + (R)CZ[i] * uz);
-                const R cF = (R)CX[i] * p.fx + (R)CY[i] * p.fy + (R)CZ[i] * p.fz;
+                const R cF = (R)CX[i] * Fx + (R)CY[i] * Fy + (R)CZ[i] * Fz;
                 const R feqp = W * (rhom1 + rho * (R(0.5) * cu * cu - R(1.5) * u2));
                 const R feqm = W * rho * cu;
                 const R fp = R(0.5) * (fh[i] + fh[i + 1]);
@@ -365,7 +401,7 @@ template <class S> struct Lattice final : Base {
         for (int i = 1; i < 19; i += 2) {
             const R W = weight<R>(i);
             const R cu = R(3) * ((R)CX[i] * ux + (R)CY[i] * uy + (R)CZ[i] * uz);
-            const R cF = (R)CX[i] * p.fx + (R)CY[i] * p.fy + (R)CZ[i] * p.fz;
+            const R cF = (R)CX[i] * Fx + (R)CY[i] * Fy + (R)CZ[i] * Fz;
             const R feqp = W * (rhom1 + rho * (R(0.5) * cu * cu - R(1.5) * u2));
             const R feqm = W * rho * cu;
             const R fp = R(0.5) * (fh[i] + fh[i + 1]);
@@ -389,6 +425,7 @@ template <class S> struct Lattice final : Base {
     void stepCell(size_t n, const Params<R>& p) {
         if (flags[n] != FLUID) return;
         const R epsv = p.useEps ? eps[n] : R(0);
+        const R alv = useAlpha ? alpha[n] : R(0);
         const int x = (int)(n % (size_t)nx);
         const size_t t = n / (size_t)nx;
         const int y = (int)(t % (size_t)ny);
@@ -398,7 +435,7 @@ template <class S> struct Lattice final : Base {
         R ntx = 0, nty = 0, ntz = 0;
         if (p.parity == 0) {                                       // 252-260
             for (int i = 0; i < 19; i++) fh[i] = S::ld(f[fidx(i, n)]);
-            collide(fh, p, x, epsv, rho, ntx, nty, ntz);
+            collide(fh, p, x, epsv, alv, useAlpha, rho, ntx, nty, ntz);
             f[fidx(0, n)] = S::st(fh[0]);
             for (int i = 1; i < 19; i++) f[fidx(OPP[i], n)] = S::st(fh[i]);
             if (p.writeForce != 0 && epsv > R(0)) {
@@ -452,7 +489,7 @@ template <class S> struct Lattice final : Base {
                 }
             }
         }
-        collide(fh, p, x, epsv, rho, ntx, nty, ntz);                // 315-316
+        collide(fh, p, x, epsv, alv, useAlpha, rho, ntx, nty, ntz);  // 315-316
         ax += ntx; ay += nty; az += ntz;
         f[fidx(0, n)] = S::st(fh[0]);                              // 317-350
         for (int i = 1; i < 19; i++) {
@@ -595,6 +632,235 @@ template <class S> struct Lattice final : Base {
     void download(void* out) override { std::memcpy(out, f.data(), ddfBytes()); }
     void upload(const void* in) override { std::memcpy(f.data(), in, ddfBytes()); }
 
+    // ---- differentiable objective and its discrete adjoint (FP64 only) -----
+    // Pre-collision populations of every fluid cell for the step about to run
+    // (absolute step t): the in-cell slots after an even count, the odd-pass
+    // gather (neighbour pull, bounce-back, moving-wall corrections) otherwise.
+    void gatherState(const Params<R>& p, std::vector<double>& g) const {
+        g.assign(19 * N, 0.0);
+        for (size_t n = 0; n < N; n++) {
+            if (flags[n] != FLUID) continue;
+            const int x = (int)(n % (size_t)nx);
+            const size_t t = n / (size_t)nx;
+            const int y = (int)(t % (size_t)ny), z = (int)(t / (size_t)ny);
+            if (p.parity == 0) {
+                for (int i = 0; i < 19; i++) g[i * N + n] = (double)S::ld(f[fidx(i, n)]);
+                continue;
+            }
+            g[n] = (double)S::ld(f[fidx(0, n)]);
+            for (int i = 1; i < 19; i++) {
+                const int srcBit = OPP[i] - 1;
+                R v;
+                if ((smask[n] >> srcBit) & 1u) {
+                    v = S::ld(f[fidx(i, n)]);
+                    if ((lmask[n] >> srcBit) & 1u) {
+                        const int sy = wrap(y - CY[i], ny);
+                        const size_t s = ((size_t)wrap(z - CZ[i], nz) * ny + sy) * nx + wrap(x - CX[i], nx);
+                        if (flags[s] == OUTFLOW) {
+                            v = -v;
+                        } else {
+                            R uwx, uwy, uwz;
+                            wallVel(flags[s], sy, p, uwx, uwy, uwz);
+                            v += R(6) * weight<R>(i) * ((R)CX[i] * uwx + (R)CY[i] * uwy + (R)CZ[i] * uwz);
+                        }
+                    }
+                } else {
+                    const size_t s = ((size_t)wrap(z - CZ[i], nz) * ny + wrap(y - CY[i], ny)) * nx + wrap(x - CX[i], nx);
+                    v = S::ld(f[fidx(OPP[i], s)]);
+                }
+                g[i * N + n] = (double)v;
+            }
+        }
+    }
+
+    // Column weights of the pressure-drop term: +1/(3 n_in), -1/(3 n_out).
+    void pressureWeights(const Objective& o, std::vector<double>& cp) const {
+        cp.assign(N, 0.0);
+        size_t nin = 0, nout = 0;
+        for (size_t n = 0; n < N; n++) {
+            if (flags[n] != FLUID) continue;
+            const int x = (int)(n % (size_t)nx);
+            if (x == o.inlet_x) nin++;
+            if (x == o.outlet_x) nout++;
+        }
+        for (size_t n = 0; n < N; n++) {
+            if (flags[n] != FLUID) continue;
+            const int x = (int)(n % (size_t)nx);
+            if (x == o.inlet_x && nin) cp[n] += 1.0 / (3.0 * (double)nin);
+            if (x == o.outlet_x && nout) cp[n] -= 1.0 / (3.0 * (double)nout);
+        }
+    }
+
+    double cellObjective(const double* g, double a, const Params<R>& p, double cpn, const Objective& o) const {
+        double rho = 1.0, px = 0.0;
+        for (int i = 0; i < 19; i++) { rho += g[i]; px += CX[i] * g[i]; }
+        const double gam = 1.0 + 0.5 * a;
+        const double Jx = (px + 0.5 * (double)p.fx) / gam;
+        return o.w_p * cpn * rho + o.w_d * a * Jx;
+    }
+
+    bool adjointSupported() const {
+        return !(hp.c_smago > 0.0 || hp.sponge_on || useEps);
+    }
+
+    double objective(const Objective& o) override {
+        if (!std::is_same<R, double>::value || !adjointSupported()) return std::nan("");
+        std::vector<double> g, cp;
+        pressureWeights(o, cp);
+        double J = 0.0;
+        const int64_t total = o.spinup + o.avg;
+        for (int64_t k = 0; k < total; k++) {
+            const Params<R> p = params(stepsDone, false);
+            if (k >= o.spinup) {
+                gatherState(p, g);
+                double sum = 0.0;
+                std::vector<double> gc(19);
+                for (size_t n = 0; n < N; n++) {
+                    if (flags[n] != FLUID) continue;
+                    for (int i = 0; i < 19; i++) gc[i] = g[i * N + n];
+                    sum += cellObjective(gc.data(), useAlpha ? (double)alpha[n] : 0.0, p, cp[n], o);
+                }
+                J += sum / (double)o.avg;
+            }
+            run(1, -1);
+        }
+        return J;
+    }
+
+    // Discrete adjoint of `total = spinup + avg` steps from the current state.
+    // The forward pass keeps every pre-collision state; the backward pass runs
+    //   mu   = P^T lambda^{t+1}            (reverse streaming over walls)
+    //   lambda^t = J_C(g^t)^T mu + dj_t/dg (collision vector-Jacobian product)
+    // and accumulates dJ/dalpha per cell and dJ/d(uin target) from the inflow
+    // links. Collision map: f* = f - w+ f^+ - w- f^- + E(rho, J) with
+    //   E_i = w+ feq+_i + w- feq-_i + S_i(Guo),  J = (p + Fb/2)/(1 + alpha/2),
+    //   F = Fb - alpha J, feq+ = w(rho - 1 + 4.5 (c.J)^2/rho - 1.5 J^2/rho),
+    //   feq- = 3 w c.J, S = w[a+ (9 (c.J)(c.F) - 3 J.F)/rho + 3 a- c.F].
+    double vjp(const Objective& o, double* gAlpha, double* gUin) override {
+        if (!std::is_same<R, double>::value || !adjointSupported()) return std::nan("");
+        const int64_t total = o.spinup + o.avg;
+        const int64_t t0 = stepsDone;
+        std::vector<double> G((size_t)total * 19 * N), g, cp;
+        pressureWeights(o, cp);
+        double J = 0.0;
+        std::vector<double> gc(19);
+        for (int64_t k = 0; k < total; k++) {
+            const Params<R> p = params(stepsDone, false);
+            gatherState(p, g);
+            std::copy(g.begin(), g.end(), G.begin() + (size_t)k * 19 * N);
+            if (k >= o.spinup) {
+                double sum = 0.0;
+                for (size_t n = 0; n < N; n++) {
+                    if (flags[n] != FLUID) continue;
+                    for (int i = 0; i < 19; i++) gc[i] = g[i * N + n];
+                    sum += cellObjective(gc.data(), useAlpha ? (double)alpha[n] : 0.0, p, cp[n], o);
+                }
+                J += sum / (double)o.avg;
+            }
+            run(1, -1);
+        }
+        std::vector<double> lamNext(19 * N, 0.0), lam(19 * N, 0.0);
+        std::vector<double> ga(N, 0.0);
+        double gu = 0.0;
+        for (int64_t k = total - 1; k >= 0; k--) {
+            const int64_t t = t0 + k;
+            const Params<R> p = params(t, false);
+            const int64_t todd = (t & 1) ? t : t + 1;
+            const double rOdd = (double)ramp(todd);
+            const double wobj = k >= o.spinup ? 1.0 / (double)o.avg : 0.0;
+            const double wp = (double)p.omega, wm = (double)p.omegaMinus;
+            const double ap = 1.0 - 0.5 * wp, am = 1.0 - 0.5 * wm;
+            const double Fb[3] = {(double)p.fx, (double)p.fy, (double)p.fz};
+            std::fill(lam.begin(), lam.end(), 0.0);
+            const double* gk = &G[(size_t)k * 19 * N];
+            for (size_t n = 0; n < N; n++) {
+                if (flags[n] != FLUID) continue;
+                const int x = (int)(n % (size_t)nx);
+                const size_t tt = n / (size_t)nx;
+                const int y = (int)(tt % (size_t)ny), z = (int)(tt / (size_t)ny);
+                double mu[19];
+                mu[0] = lamNext[n];
+                for (int i = 1; i < 19; i++) {
+                    const int dx = wrap(x + CX[i], nx), dy = wrap(y + CY[i], ny), dz = wrap(z + CZ[i], nz);
+                    const size_t d = ((size_t)dz * ny + dy) * nx + dx;
+                    if ((smask[n] >> (i - 1)) & 1u) {
+                        const double ln = lamNext[OPP[i] * N + n];
+                        mu[i] = flags[d] == OUTFLOW ? -ln : ln;
+                        if (flags[d] == INFLOW) {
+                            // f^{t+1}_{opp i}(x) = f*_i(x) - 6 w_i c_i.u_w, u_w,x = uin r prof(y_d)
+                            // wallVel clamps the parabolic profile at 0
+                            double prof = 1.0;
+                            if (!hp.inflow_uniform) {
+                                const double H = (double)ny - 2.0, yd = (double)dy - 0.5;
+                                prof = 4.0 * yd * (H - yd) / (H * H);
+                                if (!(hp.uin_target * rOdd * prof > 0.0)) prof = 0.0;
+                            }
+                            gu += ln * (-6.0 * weight<double>(i) * CX[i] * rOdd * prof);
+                        }
+                    } else {
+                        mu[i] = lamNext[i * N + d];
+                    }
+                }
+                const double a = useAlpha ? (double)alpha[n] : 0.0;
+                double rho = 1.0, pv[3] = {0, 0, 0};
+                for (int i = 0; i < 19; i++) {
+                    const double v = gk[i * N + n];
+                    rho += v; pv[0] += CX[i] * v; pv[1] += CY[i] * v; pv[2] += CZ[i] * v;
+                }
+                const double gam = 1.0 + 0.5 * a;
+                double Jv[3], Fv[3];
+                for (int c = 0; c < 3; c++) { Jv[c] = (pv[c] + 0.5 * Fb[c]) / gam; Fv[c] = Fb[c] - a * Jv[c]; }
+                double M0 = 0, Mc[3] = {0, 0, 0}, Mcc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (int i = 0; i < 19; i++) {
+                    const double wm_ = mu[i] * weight<double>(i);
+                    const int cc[3] = {CX[i], CY[i], CZ[i]};
+                    M0 += wm_;
+                    for (int a1 = 0; a1 < 3; a1++) {
+                        Mc[a1] += wm_ * cc[a1];
+                        for (int b1 = 0; b1 < 3; b1++) Mcc[a1][b1] += wm_ * cc[a1] * cc[b1];
+                    }
+                }
+                double MJ[3], MF[3], JMJ = 0, JMF = 0, J2 = 0, JF = 0;
+                for (int a1 = 0; a1 < 3; a1++) {
+                    MJ[a1] = Mcc[a1][0] * Jv[0] + Mcc[a1][1] * Jv[1] + Mcc[a1][2] * Jv[2];
+                    MF[a1] = Mcc[a1][0] * Fv[0] + Mcc[a1][1] * Fv[1] + Mcc[a1][2] * Fv[2];
+                }
+                for (int a1 = 0; a1 < 3; a1++) {
+                    JMJ += Jv[a1] * MJ[a1]; JMF += Jv[a1] * MF[a1];
+                    J2 += Jv[a1] * Jv[a1]; JF += Jv[a1] * Fv[a1];
+                }
+                const double r2 = rho * rho;
+                const double gRho = wp * (M0 - 4.5 * JMJ / r2 + 1.5 * J2 * M0 / r2)
+                                  - ap * (9.0 * JMF - 3.0 * JF * M0) / r2;
+                double gJ[3], gF[3], gP[3];
+                for (int a1 = 0; a1 < 3; a1++) {
+                    gJ[a1] = wp * (9.0 * MJ[a1] - 3.0 * Jv[a1] * M0) / rho + 3.0 * wm * Mc[a1]
+                           + ap * (9.0 * MF[a1] - 3.0 * Fv[a1] * M0) / rho;
+                    gF[a1] = ap * (9.0 * MJ[a1] - 3.0 * Jv[a1] * M0) / rho + 3.0 * am * Mc[a1];
+                    gP[a1] = (gJ[a1] - a * gF[a1]) / gam;
+                }
+                // objective: w_p cp rho + w_d alpha J_x
+                const double dObjRho = wobj * o.w_p * cp[n];
+                const double dObjPx = wobj * o.w_d * a / gam;
+                for (int j = 0; j < 19; j++) {
+                    const double mp = j == 0 ? mu[0] : 0.5 * (mu[j] + mu[OPP[j]]);
+                    const double mm = j == 0 ? 0.0 : 0.5 * (mu[j] - mu[OPP[j]]);
+                    lam[j * N + n] = mu[j] - wp * mp - wm * mm + gRho
+                                   + CX[j] * gP[0] + CY[j] * gP[1] + CZ[j] * gP[2]
+                                   + dObjRho + dObjPx * CX[j];
+                }
+                const double gJdotJ = gJ[0] * Jv[0] + gJ[1] * Jv[1] + gJ[2] * Jv[2];
+                const double gFdotJ = gF[0] * Jv[0] + gF[1] * Jv[1] + gF[2] * Jv[2];
+                ga[n] += -gJdotJ / (2.0 * gam) - gFdotJ * (1.0 - a / (2.0 * gam))
+                       + wobj * o.w_d * Jv[0] * (1.0 - a / (2.0 * gam));
+            }
+            lamNext.swap(lam);
+        }
+        if (gAlpha) for (size_t n = 0; n < N; n++) gAlpha[n] = ga[n];
+        if (gUin) *gUin = gu;
+        return J;
+    }
+
     // Single-cell collide for unit tests: fin are the 19 (widened) DDFs; the
     // step-0 params of this lattice are used with parity ignored.
     void collideCell(const double* fin, double e, int x, double* fout, double* rho, double* ntf) override {
@@ -602,7 +868,7 @@ template <class S> struct Lattice final : Base {
         R fh[19];
         for (int i = 0; i < 19; i++) fh[i] = (R)fin[i];
         R r = 1, a = 0, b = 0, c = 0;
-        collide(fh, p, x, (R)e, r, a, b, c);
+        collide(fh, p, x, (R)e, R(0), false, r, a, b, c);
         for (int i = 0; i < 19; i++) fout[i] = (double)fh[i];
         *rho = (double)r; ntf[0] = (double)a; ntf[1] = (double)b; ntf[2] = (double)c;
     }
@@ -630,6 +896,12 @@ void orc_destroy(void* h) { delete (Base*)h; }
 void orc_set_host_params(void* h, const HostParams* p) { ((Base*)h)->hp = *p; }
 void orc_set_threads(void* h, int t) { ((Base*)h)->threads = t > 0 ? t : 1; }
 void orc_set_solid_fractions(void* h, const float* eps) { ((Base*)h)->setEps(eps); }
+void orc_set_porosity(void* h, const float* alpha) { ((Base*)h)->setAlpha(alpha); }
+// FP64 lattices only (NaN otherwise, or with LES / sponge / NT cells).
+double orc_objective(void* h, const Objective* o) { return ((Base*)h)->objective(*o); }
+double orc_vjp(void* h, const Objective* o, double* grad_alpha, double* grad_uin) {
+    return ((Base*)h)->vjp(*o, grad_alpha, grad_uin);
+}
 void orc_init_field(void* h, int mode, double amplitude) { ((Base*)h)->initField(mode, amplitude); }
 void orc_run(void* h, int64_t steps) { ((Base*)h)->run(steps, -1); }
 int64_t orc_steps_done(void* h) { return ((Base*)h)->stepsDone; }

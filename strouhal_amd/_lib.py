@@ -15,7 +15,7 @@ STRH_OK = 0
 STATUS_NAMES = {
     0: "STRH_OK", 1: "STRH_ERR_INVALID", 2: "STRH_ERR_NO_DEVICE", 3: "STRH_ERR_DEVICE",
     4: "STRH_ERR_ALLOC", 5: "STRH_ERR_ODD_STEPS", 6: "STRH_ERR_NO_FORCES",
-    7: "STRH_ERR_NOT_FINITE",
+    7: "STRH_ERR_NOT_FINITE", 8: "STRH_ERR_UNSUPPORTED",
 }
 
 # Every symbol include/strouhal_hip.h declares.
@@ -24,14 +24,14 @@ EXPORTS = [
     "strh_lattice_create", "strh_lattice_destroy", "strh_lattice_cells", "strh_ddf_bytes",
     "strh_set_omega", "strh_set_lid_velocity", "strh_set_uin", "strh_set_ramp_steps",
     "strh_set_force", "strh_set_c_smago", "strh_set_inflow_uniform", "strh_set_sponge",
-    "strh_set_solid_fractions", "strh_init_field", "strh_run", "strh_steps_done",
+    "strh_set_solid_fractions", "strh_set_porosity", "strh_init_field", "strh_run", "strh_steps_done",
     "strh_gpu_seconds", "strh_probe_moments", "strh_probe_force", "strh_state_digest",
     "strh_mass_sum", "strh_kinetic_energy", "strh_download_ddf", "strh_upload_ddf",
     "strh_download_force", "strh_snapshot", "strh_restore", "strh_sha256",
     "strh_sha256_force_portable", "strh_batch_create", "strh_batch_destroy",
     "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
     "strh_batch_set_block_size", "strh_batch_autotune", "strh_batch_set_graph_steps",
-    "strh_batch_gpu_seconds",
+    "strh_batch_gpu_seconds", "strh_objective_eval", "strh_vjp",
 ]
 
 
@@ -47,6 +47,24 @@ class LatticeDesc(ctypes.Structure):
         ("force", ctypes.c_float * 3),
         ("c_smago", ctypes.c_float),
         ("wants_forces", ctypes.c_int32),
+    ]
+
+
+class Objective(ctypes.Structure):
+    """strh_objective: time-averaged pressure drop plus Brinkman drag."""
+    _fields_ = [
+        ("inlet_x", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
+        ("w_p", ctypes.c_double), ("w_d", ctypes.c_double),
+        ("spinup", ctypes.c_int64), ("avg", ctypes.c_int64),
+    ]
+
+
+class VjpInfo(ctypes.Structure):
+    """strh_vjp_info: checkpoint plan, timings and workspace of one strh_vjp call."""
+    _fields_ = [
+        ("checkpoint_every", ctypes.c_int64), ("checkpoints", ctypes.c_int64),
+        ("forward_seconds", ctypes.c_double), ("recompute_seconds", ctypes.c_double),
+        ("backward_seconds", ctypes.c_double), ("workspace_bytes", ctypes.c_uint64),
     ]
 
 
@@ -89,6 +107,7 @@ def lib():
         "strh_set_inflow_uniform": [vp, i32],
         "strh_set_sponge": [vp, i32, f32, f32, f32],
         "strh_set_solid_fractions": [vp, vp],
+        "strh_set_porosity": [vp, vp],
         "strh_init_field": [vp, u32, f32],
         "strh_run": [vp, i64],
         "strh_steps_done": [vp, ctypes.POINTER(i64)],
@@ -114,6 +133,9 @@ def lib():
         "strh_batch_autotune": [vp, ctypes.POINTER(i32)],
         "strh_batch_set_graph_steps": [vp, i32],
         "strh_batch_gpu_seconds": [vp, ctypes.POINTER(ctypes.c_double)],
+        "strh_objective_eval": [vp, ctypes.POINTER(Objective), ctypes.POINTER(ctypes.c_double)],
+        "strh_vjp": [vp, ctypes.POINTER(Objective), i64, ctypes.POINTER(ctypes.c_double), vp, vp,
+                     ctypes.POINTER(VjpInfo)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -25,6 +25,7 @@ struct StepLaunch {
     bool les;        // cSmago > 0
     bool has_force;  // body force non-zero
     uint32_t rows_bs;  // 256 / 512: odd pass uses LDS row staging; 0: direct
+    const float* alpha;  // Brinkman porosity per cell, or null
     StepParams p;
 };
 
@@ -55,7 +56,37 @@ struct BatchLat {
     double* series;          // 3 doubles per sample
 };
 
+// Hooks of the recording forward step of the adjoint (walls, Guo forcing and
+// Brinkman porosity; no partial cells, LES or sponge).
+struct AdjHooks {
+    float* rec;              // [19][S] pre-collision populations of the step, or null
+    double* jacc;            // per-cell objective accumulator, or null
+    int32_t inletX, outletX; // columns of the pressure-drop term
+    double cin, cout;        // w_p / (3 n_in T), w_p / (3 n_out T)
+    double wd;               // w_d / T
+};
+
+// One reverse step: lambda^t from lambda^{t+1} and the recorded g^t.
+struct AdjStep {
+    uint32_t nx, ny, nz, N, S, blocks;
+    FastDiv divx, divy;
+    double wp, wm, fx, fy, fz;   // collision rates and body force of step t
+    double rOdd;                 // ramp of the odd step that streams after step t
+    double uinTarget;
+    int32_t inflowUniform;
+    int32_t inletX, outletX;     // objective of step t (weights 0 outside the window)
+    double cin, cout, wd;
+};
+
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);
+// Forward step with the adjoint hooks; L.alpha must be set (zeros for a
+// lattice without porosity).
+hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s);
+hipError_t launch_adjoint_step(const AdjStep& q, bool walls, const float* rec, const float* lamNext,
+                               float* lam, const uint8_t* flags, const uint32_t* link,
+                               const float* alpha, double* gAlpha, double* gUin, hipStream_t s);
+hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,
+                              double* out, hipStream_t s);
 hipError_t launch_batch_step(int precision, int bs, const BatchLat* lats, int L,
                              uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
 hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, hipStream_t s);

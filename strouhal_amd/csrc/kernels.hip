@@ -185,8 +185,8 @@ template <bool LES> __device__ __forceinline__ void les_rates(
 // One symmetric/antisymmetric pair (I, I+1) of the eps == 0 path
 // (Kernels.metal:183-197).
 template <int I, bool FORCE> __device__ __forceinline__ void pair_golden(
-    float (&fh)[19], const StepParams& p, float rhom1, float rho, float ux, float uy,
-    float uz, float u2, float wp, float wm, float ap, float am, float uF) {
+    float (&fh)[19], const float* cFv, const float* threeCFv, float rhom1, float rho, float ux,
+    float uy, float uz, float u2, float wp, float wm, float ap, float am, float uF) {
     constexpr float W = wgt<I>();
     const float cu = 3.0f * cdot<I>(ux, uy, uz);
     const float feqp = W * (rhom1 + rho * (0.5f * cu * cu - 1.5f * u2));
@@ -196,8 +196,8 @@ template <int I, bool FORCE> __device__ __forceinline__ void pair_golden(
     const float dp = wp * (feqp - fp);
     const float dm = wm * (feqm - fm);
     if (FORCE) {
-        const float sm = W * am * p.threeCF[I >> 1];
-        const float sp = W * ap * (3.0f * cu * p.cF[I >> 1] - 3.0f * uF);
+        const float sm = W * am * threeCFv[I >> 1];
+        const float sp = W * ap * (3.0f * cu * cFv[I >> 1] - 3.0f * uF);
         fh[I] += dp + dm + sp + sm;
         fh[I + 1] += dp - dm + sp - sm;
     } else {
@@ -208,8 +208,8 @@ template <int I, bool FORCE> __device__ __forceinline__ void pair_golden(
 
 // One pair of the Noble-Torczynski path (Kernels.metal:210-227).
 template <int I, bool FORCE> __device__ __forceinline__ void pair_nt(
-    float (&fh)[19], const StepParams& p, float rhom1, float rho, float ux, float uy,
-    float uz, float u2, float wp, float wm, float ap, float am, float uF, float B,
+    float (&fh)[19], const float* cFv, const float* threeCFv, float rhom1, float rho, float ux,
+    float uy, float uz, float u2, float wp, float wm, float ap, float am, float uF, float B,
     float oneMinusB, float& ntx, float& nty, float& ntz) {
     constexpr float W = wgt<I>();
     const float cu = 3.0f * cdot<I>(ux, uy, uz);
@@ -221,8 +221,8 @@ template <int I, bool FORCE> __device__ __forceinline__ void pair_nt(
     const float dm = wm * (feqm - fm);
     const float bounce = B * (fh[I + 1] - fh[I]);
     if (FORCE) {
-        const float sm = W * am * p.threeCF[I >> 1];
-        const float sp = W * ap * (3.0f * cu * p.cF[I >> 1] - 3.0f * uF);
+        const float sm = W * am * threeCFv[I >> 1];
+        const float sp = W * ap * (3.0f * cu * cFv[I >> 1] - 3.0f * uF);
         fh[I] += oneMinusB * (dp + dm + sp + sm) + bounce;
         fh[I + 1] += oneMinusB * (dp - dm + sp - sm) - bounce;
     } else {
@@ -236,9 +236,15 @@ template <int I, bool FORCE> __device__ __forceinline__ void pair_nt(
     ntz = acc<-kCz[I]>(ntz, tb);
 }
 
-template <bool LES, bool FORCE, bool NT>
+// BRINK: Brinkman porosity alpha (new in this build, no reference
+// counterpart): drag force -alpha rho u with u taken implicitly,
+// u = (p + Fb/2) / (rho (1 + alpha/2)); the Guo terms then see the per-cell
+// force F = Fb - alpha rho u. With alpha = 0 every value is the same bits as
+// the plain forcing path.
+template <bool LES, bool FORCE, bool NT, bool BRINK = false>
 __device__ __forceinline__ void collide(float (&fh)[19], const StepParams& p, int x,
-                                        float eps, float& ntx, float& nty, float& ntz) {
+                                        float eps, float& ntx, float& nty, float& ntz,
+                                        float alpha = 0.0f) {
     float wp = p.omega, wm = p.omegaMinus;
     if ((float)x > p.spongeX0) {                                   // 112-123
         const float frac = fminf(((float)x - p.spongeX0) * p.spongeInvW, 1.0f);
@@ -277,13 +283,35 @@ __device__ __forceinline__ void collide(float (&fh)[19], const StepParams& p, in
         uy = py * inv;
         uz = pz * inv;
     }
+    float Fx = p.fx, Fy = p.fy, Fz = p.fz;
+    float cFl[9], tCFl[9];
+    const float* cFv = p.cF;
+    const float* tCFv = p.threeCF;
+    if (BRINK) {
+        const float g = 1.0f + 0.5f * alpha;
+        ux = ux / g;
+        uy = uy / g;
+        uz = uz / g;
+        const float ar = alpha * rho;
+        Fx = p.fx - ar * ux;
+        Fy = p.fy - ar * uy;
+        Fz = p.fz - ar * uz;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const int i = 2 * k + 1;
+            cFl[k] = (float)kCx[i] * Fx + (float)kCy[i] * Fy + (float)kCz[i] * Fz;
+            tCFl[k] = 3.0f * cFl[k];
+        }
+        cFv = cFl;
+        tCFv = tCFl;
+    }
     const float u2 = ux * ux + uy * uy + uz * uz;
     if (LES) {                                                     // 140-167
         if (p.cSmago > 0.0f) les_rates<LES>(fh, p, rhom1, rho, inv, ux, uy, uz, u2, wp, wm);
     }
     float uF = 0.0f, ap = 0.0f, am = 0.0f;                         // 169-171
     if (FORCE) {
-        uF = ux * p.fx + uy * p.fy + uz * p.fz;
+        uF = ux * Fx + uy * Fy + uz * Fz;
         ap = 1.0f - 0.5f * wp;
         am = 1.0f - 0.5f * wm;
     }
@@ -295,7 +323,7 @@ __device__ __forceinline__ void collide(float (&fh)[19], const StepParams& p, in
         } else {
             fh[0] = fmaf(wp, feq0 - fh[0], fh[0]);
         }
-#define STRH_PAIR(I) pair_golden<I, FORCE>(fh, p, rhom1, rho, ux, uy, uz, u2, wp, wm, ap, am, uF);
+#define STRH_PAIR(I) pair_golden<I, FORCE>(fh, cFv, tCFv, rhom1, rho, ux, uy, uz, u2, wp, wm, ap, am, uF);
         STRH_PAIR(1) STRH_PAIR(3) STRH_PAIR(5) STRH_PAIR(7) STRH_PAIR(9)
         STRH_PAIR(11) STRH_PAIR(13) STRH_PAIR(15) STRH_PAIR(17)
 #undef STRH_PAIR
@@ -311,7 +339,7 @@ __device__ __forceinline__ void collide(float (&fh)[19], const StepParams& p, in
         } else {
             fh[0] += oneMinusB * (wp * (feq0 - fh[0]));
         }
-#define STRH_PAIR(I) pair_nt<I, FORCE>(fh, p, rhom1, rho, ux, uy, uz, u2, wp, wm, ap, am, uF, B, oneMinusB, ntx, nty, ntz);
+#define STRH_PAIR(I) pair_nt<I, FORCE>(fh, cFv, tCFv, rhom1, rho, ux, uy, uz, u2, wp, wm, ap, am, uF, B, oneMinusB, ntx, nty, ntz);
         STRH_PAIR(1) STRH_PAIR(3) STRH_PAIR(5) STRH_PAIR(7) STRH_PAIR(9)
         STRH_PAIR(11) STRH_PAIR(13) STRH_PAIR(15) STRH_PAIR(17)
 #undef STRH_PAIR
@@ -332,14 +360,45 @@ constexpr uint32_t kStepBS = 256;
 
 // One cell of one AA pass (the body shared by the single-lattice kernel and
 // the multi-lattice batch kernel).
-template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+// Adjoint hooks of the forward pass (AdjHooks in kernels.h): record the
+// pre-collision populations the collision sees (natural slot order) and
+// accumulate the cell's objective term in FP64,
+//   cp(x) rho + wd alpha J_x,   J_x = (p_x + F_x / 2) / (1 + alpha / 2),
+// with cp = +cin on the inlet column and -cout on the outlet column.
+__device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[19], uint32_t n,
+                                          uint32_t S, int x, float alpha, float fx) {
+    if (hk.rec) {
+#pragma unroll
+        for (int i = 0; i < 19; i++) __builtin_nontemporal_store(fh[i], hk.rec + (size_t)i * S + n);
+    }
+    if (hk.jacc) {
+        const double cp = (x == hk.inletX ? hk.cin : 0.0) - (x == hk.outletX ? hk.cout : 0.0);
+        if (cp != 0.0 || (hk.wd != 0.0 && alpha != 0.0f)) {
+            double rho = 1.0, px = 0.0;
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                rho += (double)fh[i];
+                if (kCx[i] > 0) px += (double)fh[i];
+                if (kCx[i] < 0) px -= (double)fh[i];
+            }
+            const double a = (double)alpha;
+            const double Jx = (px + 0.5 * (double)fx) / (1.0 + 0.5 * a);
+            hk.jacc[n] += cp * rho + hk.wd * a * Jx;
+        }
+    }
+}
+
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
+          bool HOOK = false>
 __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                                           const uint8_t* __restrict__ flags,
                                           const uint32_t* __restrict__ link,
                                           const uint32_t* __restrict__ lidm,
                                           float4* __restrict__ force,
                                           const float* __restrict__ epsBuf,
-                                          const StepParams& p, uint32_t n) {
+                                          const StepParams& p, uint32_t n,
+                                          const float* __restrict__ alphaBuf = nullptr,
+                                          const AdjHooks* hk = nullptr) {
     using T = typename St::T;
     const size_t S = p.S;
     T* fs[19];
@@ -359,7 +418,9 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         const int x = (int)(n - t * (uint32_t)p.nx);
 #pragma unroll
         for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nb);
-        collide<LES, FORCE, NT>(fh, p, x, eps, ntx, nty, ntz);
+        const float alpha = BRINK ? alphaBuf[n] : 0.0f;
+        if (HOOK) run_hooks(*hk, fh, n, p.S, x, alpha, p.fx);
+        collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);
         // A separate copy of the offset keeps the stores on the scalar-base
         // form too (otherwise the shared base+offset sum is kept as 19
         // 64-bit VGPR addresses across the collision).
@@ -446,7 +507,9 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 #undef STRH_DEFER
     }
 
-    collide<LES, FORCE, NT>(fh, p, x, eps, ntx, nty, ntz);            // 315-316
+    const float alpha = BRINK ? alphaBuf[n] : 0.0f;
+    if (HOOK) run_hooks(*hk, fh, n, p.S, x, alpha, p.fx);
+    collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);  // 315-316
     if (NT) {
         ax += ntx;
         ay += nty;
@@ -493,14 +556,175 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         force[n] = make_float4(ax, ay, az, 0.0f);
 }
 
-template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
-            float4* __restrict__ force, const float* __restrict__ epsBuf, StepParams p) {
+            float4* __restrict__ force, const float* __restrict__ epsBuf, StepParams p,
+            const float* __restrict__ alphaBuf) {
     const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * kStepBS + threadIdx.x;
     if (n >= p.N) return;
-    step_cell<St, PARITY, WALLS, NT, LES, FORCE>(f, flags, link, lidm, force, epsBuf, p, n);
+    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK>(f, flags, link, lidm, force, epsBuf, p, n,
+                                                        alphaBuf);
+}
+
+// Forward step of the differentiable configuration (Guo forcing, Brinkman
+// porosity) with the adjoint hooks. The arithmetic is that of step_kernel:
+// with alpha = 0 and no body force the DDFs are the plain update's bits.
+template <class St, int PARITY, bool WALLS>
+__global__ void __launch_bounds__(kStepBS)
+step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
+                   const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
+                   StepParams p, const float* __restrict__ alphaBuf, AdjHooks hk) {
+    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * kStepBS + threadIdx.x;
+    if (n >= p.N) return;
+    step_cell<St, PARITY, WALLS, false, false, true, true, true>(f, flags, link, lidm, nullptr,
+                                                                 nullptr, p, n, alphaBuf, &hk);
+}
+
+// ---- discrete adjoint: one reverse step ------------------------------------
+// The forward step is g^{t+1} = P(C(g^t)) in natural slot order: C the
+// collision of a fluid cell, P the streaming with halfway bounce-back
+//   g^{t+1}_i(x + c_i)  = C_i(x)                    (fluid neighbour)
+//   g^{t+1}_opp(i)(x)   = C_i(x) - 6 w_i c_i.u_w    (wall at x + c_i; OUTFLOW: -C_i)
+// so mu = P^T lambda^{t+1} gathers lambda from the neighbours (or from the
+// opposite slot of the cell itself across a wall), and
+// lambda^t = J_C(g^t)^T mu + d(objective_t)/dg^t. With M0 = sum w mu,
+// Mc = sum w mu c, Mcc = sum w mu c c^T, J = (p + Fb/2)/(1 + alpha/2),
+// F = Fb - alpha J and the collision written through (rho, J, F):
+//   E_i = w+ feq+_i + w- feq-_i + S_i,
+//   feq+ = w(rho - 1 + 4.5 (c.J)^2/rho - 1.5 J^2/rho), feq- = 3 w c.J,
+//   S = w [a+ (9 (c.J)(c.F) - 3 J.F)/rho + 3 a- c.F],  a+- = 1 - w+-/2,
+// the partial derivatives of mu.E are
+//   dRho = w+ (M0 - 4.5 J.Mcc.J/rho^2 + 1.5 J^2 M0/rho^2) - a+ (9 J.Mcc.F - 3 J.F M0)/rho^2
+//   dJ   = w+ (9 Mcc.J - 3 J M0)/rho + 3 w- Mc + a+ (9 Mcc.F - 3 F M0)/rho
+//   dF   = a+ (9 Mcc.J - 3 J M0)/rho + 3 a- Mc
+// and through J(p, alpha), F(J, alpha): dP = (dJ - alpha dF)/(1 + alpha/2),
+//   dJ/dalpha = -J/(2 gamma), dF/dalpha = -J (1 - alpha/(2 gamma)).
+// All adjoint arithmetic is FP64; lambda is stored as FP32.
+__device__ __forceinline__ double wgt_d(int i) {
+    return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+}
+
+template <bool WALLS>
+__global__ void __launch_bounds__(256)
+adjoint_step_kernel(const float* __restrict__ rec, const float* __restrict__ lamN,
+                    float* __restrict__ lam, const uint8_t* __restrict__ flags,
+                    const uint32_t* __restrict__ link, const float* __restrict__ alphaBuf,
+                    double* __restrict__ gAlpha, double* __restrict__ gUin, AdjStep q) {
+    const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
+    if (n >= q.N) return;
+    if (flags[n] != kFluid) return;
+    const size_t S = q.S;
+    const uint32_t t = fdiv(n, q.divx);
+    const int x = (int)(n - t * q.nx);
+    const uint32_t zz = fdiv(t, q.divy);
+    const int y = (int)(t - zz * q.ny), z = (int)zz;
+    const int nx = (int)q.nx, ny = (int)q.ny, nz = (int)q.nz;
+    const uint32_t sM = WALLS ? (link[n] & 0x3ffffu) : 0u;
+
+    double mu[19];
+    double gu = 0.0;
+    mu[0] = (double)lamN[n];
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        int dx = x + kCx[i], dy = y + kCy[i], dz = z + kCz[i];
+        dx = dx < 0 ? nx - 1 : (dx >= nx ? 0 : dx);
+        dy = dy < 0 ? ny - 1 : (dy >= ny ? 0 : dy);
+        dz = dz < 0 ? nz - 1 : (dz >= nz ? 0 : dz);
+        const uint32_t d = ((uint32_t)dz * q.ny + (uint32_t)dy) * q.nx + (uint32_t)dx;
+        if (WALLS && ((sM >> (i - 1)) & 1u)) {
+            const double ln = (double)lamN[(size_t)kOpp[i] * S + n];
+            const uint8_t wf = flags[d];
+            mu[i] = wf == kOutflow ? -ln : ln;
+            if (wf == kInflow && kCx[i] != 0) {
+                double prof = 1.0;                 // wall_vel: parabolic profile clamped at 0
+                if (!q.inflowUniform) {
+                    const double H = (double)ny - 2.0, yd = (double)dy - 0.5;
+                    prof = 4.0 * yd * (H - yd) / (H * H);
+                    if (!(q.uinTarget * q.rOdd * prof > 0.0)) prof = 0.0;
+                }
+                gu += ln * (-6.0 * wgt_d(i) * (double)kCx[i] * q.rOdd * prof);
+            }
+        } else {
+            mu[i] = (double)lamN[(size_t)i * S + d];
+        }
+    }
+
+    double g[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) g[i] = (double)rec[(size_t)i * S + n];
+    const double a = alphaBuf ? (double)alphaBuf[n] : 0.0;
+    double rho = 1.0, px = 0.0, py = 0.0, pz = 0.0;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        rho += g[i];
+        px += (double)kCx[i] * g[i];
+        py += (double)kCy[i] * g[i];
+        pz += (double)kCz[i] * g[i];
+    }
+    const double gam = 1.0 + 0.5 * a;
+    const double Jx = (px + 0.5 * q.fx) / gam, Jy = (py + 0.5 * q.fy) / gam, Jz = (pz + 0.5 * q.fz) / gam;
+    const double Fx = q.fx - a * Jx, Fy = q.fy - a * Jy, Fz = q.fz - a * Jz;
+    double M0 = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0;
+    double Mxx = 0.0, Myy = 0.0, Mzz = 0.0, Mxy = 0.0, Mxz = 0.0, Myz = 0.0;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const double v = wgt_d(i) * mu[i];
+        M0 += v;
+        if (kCx[i]) Mx += (double)kCx[i] * v;
+        if (kCy[i]) My += (double)kCy[i] * v;
+        if (kCz[i]) Mz += (double)kCz[i] * v;
+        if (kCx[i]) Mxx += v;
+        if (kCy[i]) Myy += v;
+        if (kCz[i]) Mzz += v;
+        if (kCx[i] * kCy[i]) Mxy += (double)(kCx[i] * kCy[i]) * v;
+        if (kCx[i] * kCz[i]) Mxz += (double)(kCx[i] * kCz[i]) * v;
+        if (kCy[i] * kCz[i]) Myz += (double)(kCy[i] * kCz[i]) * v;
+    }
+    const double MJx = Mxx * Jx + Mxy * Jy + Mxz * Jz;
+    const double MJy = Mxy * Jx + Myy * Jy + Myz * Jz;
+    const double MJz = Mxz * Jx + Myz * Jy + Mzz * Jz;
+    const double MFx = Mxx * Fx + Mxy * Fy + Mxz * Fz;
+    const double MFy = Mxy * Fx + Myy * Fy + Myz * Fz;
+    const double MFz = Mxz * Fx + Myz * Fy + Mzz * Fz;
+    const double JMJ = Jx * MJx + Jy * MJy + Jz * MJz;
+    const double JMF = Jx * MFx + Jy * MFy + Jz * MFz;
+    const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+    const double JF = Jx * Fx + Jy * Fy + Jz * Fz;
+    const double wp = q.wp, wm = q.wm;
+    const double ap = 1.0 - 0.5 * wp, am = 1.0 - 0.5 * wm;
+    const double ir = 1.0 / rho, ir2 = ir * ir;
+    const double gRho = wp * (M0 - 4.5 * JMJ * ir2 + 1.5 * J2 * M0 * ir2)
+                      - ap * (9.0 * JMF - 3.0 * JF * M0) * ir2;
+    const double gJx = wp * (9.0 * MJx - 3.0 * Jx * M0) * ir + 3.0 * wm * Mx + ap * (9.0 * MFx - 3.0 * Fx * M0) * ir;
+    const double gJy = wp * (9.0 * MJy - 3.0 * Jy * M0) * ir + 3.0 * wm * My + ap * (9.0 * MFy - 3.0 * Fy * M0) * ir;
+    const double gJz = wp * (9.0 * MJz - 3.0 * Jz * M0) * ir + 3.0 * wm * Mz + ap * (9.0 * MFz - 3.0 * Fz * M0) * ir;
+    const double gFx = ap * (9.0 * MJx - 3.0 * Jx * M0) * ir + 3.0 * am * Mx;
+    const double gFy = ap * (9.0 * MJy - 3.0 * Jy * M0) * ir + 3.0 * am * My;
+    const double gFz = ap * (9.0 * MJz - 3.0 * Jz * M0) * ir + 3.0 * am * Mz;
+    const double gPx = (gJx - a * gFx) / gam, gPy = (gJy - a * gFy) / gam, gPz = (gJz - a * gFz) / gam;
+    // objective of step t: cp rho + wd alpha J_x
+    const double cp = (x == q.inletX ? q.cin : 0.0) - (x == q.outletX ? q.cout : 0.0);
+    const double dPx = gPx + q.wd * a / gam;
+    const double base = gRho + cp;
+    lam[n] = (float)(mu[0] - wp * mu[0] + base);
+#pragma unroll
+    for (int j = 1; j < 19; j++) {
+        const int o = kOpp[j];
+        const double mp = 0.5 * (mu[j] + mu[o]);
+        const double mm = 0.5 * (mu[j] - mu[o]);
+        double v = mu[j] - wp * mp - wm * mm + base;
+        if (kCx[j]) v += (double)kCx[j] * dPx;
+        if (kCy[j]) v += (double)kCy[j] * gPy;
+        if (kCz[j]) v += (double)kCz[j] * gPz;
+        lam[(size_t)j * S + n] = (float)v;
+    }
+    const double gJdotJ = gJx * Jx + gJy * Jy + gJz * Jz;
+    const double gFdotJ = gFx * Jx + gFy * Jy + gFz * Jz;
+    const double k = 1.0 - a / (2.0 * gam);
+    gAlpha[n] += -gJdotJ / (2.0 * gam) - gFdotJ * k + q.wd * Jx * k;
+    if (gu != 0.0) gUin[n] += gu;
 }
 
 // ---- odd pass with LDS row staging (wall-free lattices) --------------------
@@ -827,7 +1051,7 @@ __global__ void set_base(int64_t* base, int64_t v) {
 template <class Conv, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
 static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
     using T = typename Conv::T;
-    if (PARITY == 1 && !WALLS && L.rows_bs != 0) {
+    if (PARITY == 1 && !WALLS && L.rows_bs != 0 && !L.alpha) {
         using St = Store<Conv, kPolAligned>;
         StepParams p = L.p;
         if (L.rows_bs == 512) {
@@ -844,9 +1068,16 @@ static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
     using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
     StepParams p = L.p;
     p.blocks = (p.N + kStepBS - 1) / kStepBS;
+    if (L.alpha) {   // Brinkman lattices: forcing path with per-cell force, no NT / LES
+        if (NT || LES) return hipErrorNotSupported;
+        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, false, false, true, true>), dim3(p.blocks),
+                           dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
+                           (float4*)L.force, L.eps, p, L.alpha);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(p.blocks),
                        dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
-                       (float4*)L.force, L.eps, p);
+                       (float4*)L.force, L.eps, p, (const float*)nullptr);
     return hipGetLastError();
 }
 
@@ -880,6 +1111,48 @@ hipError_t launch_step(const StepLaunch& L, hipStream_t s) {
         case kBF16: return pick_parity<ConvBF16>(L, s);
     }
     return hipErrorInvalidValue;
+}
+
+template <class Conv, int PARITY, bool WALLS>
+static hipError_t launch_hooked_t(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
+    using T = typename Conv::T;
+    using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
+    StepParams p = L.p;
+    p.blocks = (p.N + kStepBS - 1) / kStepBS;
+    hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS>), dim3(p.blocks), dim3(kStepBS), 0, s,
+                       (T*)L.f, L.flags, L.link, L.lidm, p, L.alpha, hk);
+    return hipGetLastError();
+}
+
+template <class Conv>
+static hipError_t launch_hooked_c(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
+    if (L.p.parity == 0)
+        return L.walls ? launch_hooked_t<Conv, 0, true>(L, hk, s) : launch_hooked_t<Conv, 0, false>(L, hk, s);
+    return L.walls ? launch_hooked_t<Conv, 1, true>(L, hk, s) : launch_hooked_t<Conv, 1, false>(L, hk, s);
+}
+
+hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
+    if (!L.alpha || L.nt || L.les) return hipErrorInvalidValue;
+    switch (L.precision) {
+        case kFP32: return launch_hooked_c<ConvF32>(L, hk, s);
+        case kFP16S: return launch_hooked_c<ConvF16>(L, hk, s);
+        case kBF16: return launch_hooked_c<ConvBF16>(L, hk, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float* rec, const float* lamNext,
+                               float* lam, const uint8_t* flags, const uint32_t* link,
+                               const float* alpha, double* gAlpha, double* gUin, hipStream_t s) {
+    AdjStep q = q0;
+    q.blocks = (q.N + 255u) / 256u;
+    if (walls)
+        hipLaunchKernelGGL(adjoint_step_kernel<true>, dim3(q.blocks), dim3(256), 0, s, rec, lamNext,
+                           lam, flags, link, alpha, gAlpha, gUin, q);
+    else
+        hipLaunchKernelGGL(adjoint_step_kernel<false>, dim3(q.blocks), dim3(256), 0, s, rec, lamNext,
+                           lam, flags, link, alpha, gAlpha, gUin, q);
+    return hipGetLastError();
 }
 
 // Row staging applies when nx is a power of two that tiles a 256- or
@@ -952,6 +1225,16 @@ hipError_t launch_box_force_sum(const float* force, uint32_t nx, uint32_t ny, ui
         if (err != hipSuccess) return err;
     }
     return hipSuccess;
+}
+
+struct SumElem {
+    const double* v;
+    __device__ double operator()(uint32_t, uint32_t k) const { return v[k]; }
+};
+
+hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,
+                              double* out, hipStream_t s) {
+    return reduce(SumElem{v}, M, 1, partials, max_partials, out, s);
 }
 
 hipError_t launch_kinetic_sum(const float* moments, uint32_t N, double* partials,

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+
+#include <cmath>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -74,6 +76,7 @@ struct strh_lattice {
     uint32_t* lidm = nullptr;   // N moving-wall masks (walls only)
     float* force = nullptr;     // 4N floats (wants_forces)
     float* eps = nullptr;       // N solid fractions (after set_solid_fractions)
+    float* alpha = nullptr;     // N Brinkman porosities (after set_porosity)
     float* moments = nullptr;   // 4N floats, allocated on first probe
     double* partials = nullptr; // reduction scratch
     double* result = nullptr;   // 3 doubles
@@ -159,6 +162,7 @@ StepLaunch make_launch(const strh_lattice* L, int64_t t, bool writeForce) {
     s.les = L->cSmago > 0.0f;
     s.has_force = !(L->bodyForce[0] == 0.0f && L->bodyForce[1] == 0.0f && L->bodyForce[2] == 0.0f);
     s.rows_bs = L->walls ? 0u : rows_block_size((uint32_t)L->nx, L->N);
+    s.alpha = L->alpha;
     s.p = make_params(L, t, writeForce);
     return s;
 }
@@ -202,7 +206,7 @@ void build_masks(const uint8_t* fl, int nx, int ny, int nz, std::vector<uint32_t
 void free_lattice(strh_lattice* L) {
     if (!L) return;
     if (L->ctx) hipSetDevice(L->ctx->device);
-    void* ptrs[] = {L->f, L->flags, L->link, L->lidm, L->force, L->eps, L->moments,
+    void* ptrs[] = {L->f, L->flags, L->link, L->lidm, L->force, L->eps, L->alpha, L->moments,
                     L->partials, L->result, L->snap};
     for (void* p : ptrs) if (p) hipFree(p);
     delete L;
@@ -218,6 +222,8 @@ int sync(strh_lattice* L, const char* what) {
 int run_steps(strh_lattice* L, int64_t steps, int64_t forceStep) {
     if (steps < 0) return fail(STRH_ERR_INVALID, "negative step count");
     if (steps == 0) return STRH_OK;
+    if (L->alpha && (L->useEps || L->cSmago > 0.0f))
+        return fail(STRH_ERR_INVALID, "porosity cannot be combined with solid fractions or LES");
     strh_ctx* c = L->ctx;
     HIP_TRY(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
     for (int64_t s = 0; s < steps; s++) {
@@ -441,6 +447,23 @@ int strh_set_solid_fractions(strh_lattice* L, const float* eps) {
     return STRH_OK;
 }
 
+int strh_set_porosity(strh_lattice* L, const float* alpha) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!alpha) {   // back to a plain lattice
+        if (L->alpha) HIP_TRY(hipFree(L->alpha), "hipFree(alpha)");
+        L->alpha = nullptr;
+        return STRH_OK;
+    }
+    for (int64_t n = 0; n < L->N; n++)
+        if (!(alpha[n] >= 0.0f) || !std::isfinite(alpha[n]))
+            return fail(STRH_ERR_INVALID, "porosity at cell %lld is not a finite value >= 0",
+                        (long long)n);
+    if (!L->alpha) HIP_TRY(hipMalloc(&L->alpha, (size_t)L->N * 4), "hipMalloc(alpha)");
+    HIP_TRY(hipMemcpy(L->alpha, alpha, (size_t)L->N * 4, hipMemcpyHostToDevice), "copy alpha");
+    return STRH_OK;
+}
+
 int strh_init_field(strh_lattice* L, uint32_t mode, float amplitude) {
     int rc = check_lat(L);
     if (rc) return rc;
@@ -654,6 +677,7 @@ namespace {
 int batch_prepare(strh_batch* B, const int64_t* steps) {
     for (size_t i = 0; i < B->lats.size(); i++) {
         strh_lattice* L = B->lats[i];
+        if (L->alpha) return fail(STRH_ERR_INVALID, "lattice %zu has porosity: batches run plain lattices", i);
         BatchLat& d = B->host[i];
         d.p = make_params(L, L->stepsDone, false);
         d.f = L->f; d.flags = L->flags; d.link = L->link; d.lidm = L->lidm;
@@ -803,6 +827,7 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
     for (int i = 0; i < count; i++) {
         if (!lats[i] || lats[i]->ctx != c) return fail(STRH_ERR_INVALID, "lattice %d is not on this context", i);
         if (lats[i]->precision != lats[0]->precision) return fail(STRH_ERR_INVALID, "lattices must share one precision");
+        if (lats[i]->alpha) return fail(STRH_ERR_INVALID, "lattice %d has porosity: batches run plain lattices", i);
         for (int j = 0; j < i; j++)
             if (lats[j] == lats[i]) return fail(STRH_ERR_INVALID, "lattice %d appears twice", i);
     }
@@ -971,6 +996,240 @@ int strh_batch_autotune(strh_batch* B, int* chosen) {
     if (rc) { B->bs = keep; return rc; }
     B->bs = bestBs;
     if (chosen) *chosen = bestBs;
+    return STRH_OK;
+}
+
+}  // extern "C"
+
+// ---- discrete adjoint ----------------------------------------------------
+namespace {
+
+struct AdjWork {
+    std::vector<void*> bufs;
+    uint64_t bytes = 0;
+    ~AdjWork() { for (void* p : bufs) hipFree(p); }
+    template <class T> int get(T** out, size_t bytes_, const char* what) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes_ ? bytes_ : 1);
+        if (e != hipSuccess) return fail(STRH_ERR_ALLOC, "hipMalloc(%s, %zu bytes): %s", what, bytes_,
+                                         hipGetErrorString(e));
+        bufs.push_back(p);
+        bytes += bytes_;
+        *out = (T*)p;
+        return STRH_OK;
+    }
+};
+
+struct AdjPlan {
+    int64_t T = 0;
+    double cin = 0.0, cout = 0.0, wd = 0.0;   // weights of one window step
+};
+
+int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
+    if (!o) return fail(STRH_ERR_INVALID, "null objective");
+    if (o->spinup < 0 || o->avg <= 0) return fail(STRH_ERR_INVALID, "objective needs spinup >= 0 and avg > 0");
+    if (o->inlet_x < -1 || o->inlet_x >= L->nx || o->outlet_x < -1 || o->outlet_x >= L->nx)
+        return fail(STRH_ERR_INVALID, "objective columns must lie in [-1, nx)");
+    if (!std::isfinite(o->w_p) || !std::isfinite(o->w_d)) return fail(STRH_ERR_INVALID, "objective weights must be finite");
+    if (L->useEps || L->cSmago > 0.0f || L->spongeOn)
+        return fail(STRH_ERR_UNSUPPORTED, "the adjoint covers walls, Guo forcing and Brinkman porosity; "
+                                          "partial cells, LES and the sponge are not differentiated");
+    P.T = o->spinup + o->avg;
+    std::vector<uint8_t> fl(L->N);
+    HIP_TRY(hipMemcpy(fl.data(), L->flags, L->N, hipMemcpyDeviceToHost), "copy flags");
+    int64_t nin = 0, nout = 0;
+    for (uint32_t n = 0; n < L->N; n++) {
+        if (fl[n] != kFluid) continue;
+        const int x = (int)(n % (uint32_t)L->nx);
+        if (x == o->inlet_x) nin++;
+        if (x == o->outlet_x) nout++;
+    }
+    const double inv = 1.0 / (double)o->avg;
+    P.cin = nin ? o->w_p * inv / (3.0 * (double)nin) : 0.0;
+    P.cout = nout ? o->w_p * inv / (3.0 * (double)nout) : 0.0;
+    P.wd = o->w_d * inv;
+    return STRH_OK;
+}
+
+// Hooked forward steps k0 .. k0+count-1 of the objective window on buffer f
+// (absolute steps t0 + k): states recorded into rec (count * 19 * S floats)
+// when given, objective accumulated into jacc when given.
+int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_t k0, int64_t count,
+                const strh_objective* o, const AdjPlan& P, float* rec, double* jacc) {
+    for (int64_t s = 0; s < count; s++) {
+        const int64_t k = k0 + s;
+        StepLaunch sl = make_launch(L, t0 + k, false);
+        sl.f = f;
+        sl.alpha = alpha;
+        AdjHooks hk;
+        hk.rec = rec ? rec + (size_t)s * 19 * L->S : nullptr;
+        const bool win = k >= o->spinup;
+        hk.jacc = (jacc && win) ? jacc : nullptr;
+        hk.inletX = o->inlet_x;
+        hk.outletX = o->outlet_x;
+        hk.cin = win ? P.cin : 0.0;
+        hk.cout = win ? P.cout : 0.0;
+        hk.wd = win ? P.wd : 0.0;
+        HIP_TRY(launch_step_hooked(sl, hk, L->ctx->stream), "hooked step launch");
+    }
+    return STRH_OK;
+}
+
+int reduce_to_host(strh_lattice* L, const double* v, double* out) {
+    HIP_TRY(launch_sum_doubles(v, L->N, L->partials, kMaxPartials, L->result, L->ctx->stream), "sum launch");
+    HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy sum");
+    HIP_TRY(hipStreamSynchronize(L->ctx->stream), "sum");
+    return STRH_OK;
+}
+
+double elapsed(strh_ctx* c) {
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    return (double)ms * 1e-3;
+}
+
+}  // namespace
+
+extern "C" {
+
+int strh_objective_eval(strh_lattice* L, const strh_objective* o, double* J) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!J) return fail(STRH_ERR_INVALID, "null J");
+    AdjPlan P;
+    if ((rc = adj_plan(L, o, P))) return rc;
+    AdjWork W;
+    double* jacc = nullptr;
+    float* zeroAlpha = nullptr;
+    if ((rc = W.get(&jacc, (size_t)L->N * 8, "objective accumulator"))) return rc;
+    HIP_TRY(launch_fill_zero(jacc, (size_t)L->N * 8, L->ctx->stream), "clear");
+    const float* alpha = L->alpha;
+    if (!alpha) {
+        if ((rc = W.get(&zeroAlpha, (size_t)L->N * 4, "zero porosity"))) return rc;
+        HIP_TRY(launch_fill_zero(zeroAlpha, (size_t)L->N * 4, L->ctx->stream), "clear");
+        alpha = zeroAlpha;
+    }
+    strh_ctx* c = L->ctx;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+    if ((rc = adj_forward(L, L->f, alpha, L->stepsDone, 0, P.T, o, P, nullptr, jacc))) return rc;
+    HIP_TRY(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(c->ev1), "objective steps");
+    L->gpuSeconds += elapsed(c);
+    L->stepsDone += P.T;
+    if ((rc = reduce_to_host(L, jacc, J))) return rc;
+    if (!std::isfinite(*J)) return fail(STRH_ERR_NOT_FINITE, "objective is not finite");
+    return STRH_OK;
+}
+
+int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, double* gradAlpha,
+             double* gradUin, strh_vjp_info* info) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!J) return fail(STRH_ERR_INVALID, "null J");
+    AdjPlan P;
+    if ((rc = adj_plan(L, o, P))) return rc;
+    const int64_t T = P.T;
+    if (K < 0) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= 0");
+    if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
+    if (K > T) K = T;
+    const int64_t C = (T + K - 1) / K;
+    const size_t N = L->N, S = L->S;
+    const size_t ddf = 19 * slot_bytes(L);
+    strh_ctx* c = L->ctx;
+    hipStream_t st = c->stream;
+
+    AdjWork W;
+    double *jacc, *gA, *gU;
+    float *rec, *lam0, *lam1, *zeroAlpha = nullptr;
+    uint8_t *ck, *work;
+    if ((rc = W.get(&jacc, N * 8, "objective accumulator"))) return rc;
+    if ((rc = W.get(&gA, N * 8, "porosity gradient"))) return rc;
+    if ((rc = W.get(&gU, N * 8, "inlet gradient"))) return rc;
+    if ((rc = W.get(&lam0, 19 * S * 4, "adjoint state"))) return rc;
+    if ((rc = W.get(&lam1, 19 * S * 4, "adjoint state"))) return rc;
+    if ((rc = W.get(&rec, (size_t)K * 19 * S * 4, "recorded states"))) return rc;
+    if ((rc = W.get(&ck, (size_t)(C > 1 ? C - 1 : 1) * ddf, "checkpoints"))) return rc;
+    if ((rc = W.get(&work, ddf, "replay DDFs"))) return rc;
+    const float* alpha = L->alpha;
+    if (!alpha) {
+        if ((rc = W.get(&zeroAlpha, N * 4, "zero porosity"))) return rc;
+        HIP_TRY(launch_fill_zero(zeroAlpha, N * 4, st), "clear");
+        alpha = zeroAlpha;
+    }
+    for (void* p : {(void*)jacc, (void*)gA, (void*)gU})
+        HIP_TRY(launch_fill_zero(p, N * 8, st), "clear");
+    HIP_TRY(launch_fill_zero(lam0, 19 * S * 4, st), "clear");
+    HIP_TRY(launch_fill_zero(lam1, 19 * S * 4, st), "clear");
+    const int64_t t0 = L->stepsDone;
+
+    // forward pass: checkpoints at the start of segments 0 .. C-2, the last
+    // segment recorded directly
+    HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
+    for (int64_t seg = 0; seg < C; seg++) {
+        const int64_t k0 = seg * K, len = std::min(K, T - k0);
+        const bool last = seg == C - 1;
+        if (!last)
+            HIP_TRY(hipMemcpyAsync(ck + (size_t)seg * ddf, L->f, ddf, hipMemcpyDeviceToDevice, st), "checkpoint");
+        if ((rc = adj_forward(L, L->f, alpha, t0, k0, len, o, P, last ? rec : nullptr, jacc))) return rc;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(c->ev1), "forward pass");
+    double tf = elapsed(c), tr = 0.0, tb = 0.0;
+    L->gpuSeconds += tf;
+    L->stepsDone += T;
+
+    // reverse pass, one segment at a time
+    float* lamN = lam0;
+    float* lamT = lam1;
+    for (int64_t seg = C - 1; seg >= 0; seg--) {
+        const int64_t k0 = seg * K, len = std::min(K, T - k0);
+        if (seg != C - 1) {
+            HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
+            HIP_TRY(hipMemcpyAsync(work, ck + (size_t)seg * ddf, ddf, hipMemcpyDeviceToDevice, st), "restore");
+            if ((rc = adj_forward(L, work, alpha, t0, k0, len, o, P, rec, nullptr))) return rc;
+            HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
+            HIP_TRY(hipEventSynchronize(c->ev1), "replay");
+            tr += elapsed(c);
+        }
+        HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
+        for (int64_t k = k0 + len - 1; k >= k0; k--) {
+            const int64_t t = t0 + k;
+            const StepParams sp = make_params(L, t, false);
+            AdjStep q;
+            memset(&q, 0, sizeof(q));
+            q.nx = (uint32_t)L->nx; q.ny = (uint32_t)L->ny; q.nz = (uint32_t)L->nz;
+            q.N = L->N; q.S = L->S;
+            q.divx = sp.divx; q.divy = sp.divy;
+            q.wp = sp.omega; q.wm = sp.omegaMinus;
+            q.fx = sp.fx; q.fy = sp.fy; q.fz = sp.fz;
+            q.rOdd = (double)ramp(L, (t & 1) ? t : t + 1);
+            q.uinTarget = L->uinTarget;
+            q.inflowUniform = L->inflowUniform ? 1 : 0;
+            const bool win = k >= o->spinup;
+            q.inletX = o->inlet_x; q.outletX = o->outlet_x;
+            q.cin = win ? P.cin : 0.0;
+            q.cout = win ? P.cout : 0.0;
+            q.wd = win ? P.wd : 0.0;
+            HIP_TRY(launch_adjoint_step(q, L->walls, rec + (size_t)(k - k0) * 19 * S, lamN, lamT,
+                                        L->flags, L->link, alpha, gA, gU, st), "adjoint step launch");
+            std::swap(lamN, lamT);
+        }
+        HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
+        HIP_TRY(hipEventSynchronize(c->ev1), "reverse steps");
+        tb += elapsed(c);
+    }
+    if ((rc = reduce_to_host(L, jacc, J))) return rc;
+    if (gradUin && (rc = reduce_to_host(L, gU, gradUin))) return rc;
+    if (gradAlpha) HIP_TRY(hipMemcpy(gradAlpha, gA, N * 8, hipMemcpyDeviceToHost), "copy gradient");
+    if (info) {
+        info->checkpoint_every = K;
+        info->checkpoints = C - 1;
+        info->forward_seconds = tf;
+        info->recompute_seconds = tr;
+        info->backward_seconds = tb;
+        info->workspace_bytes = W.bytes;
+    }
+    if (!std::isfinite(*J)) return fail(STRH_ERR_NOT_FINITE, "objective is not finite");
     return STRH_OK;
 }
 

@@ -262,6 +262,54 @@ class Simulation:
             raise ValueError("eps must hold one value per cell")
         check(lib().strh_set_solid_fractions(self._h, e.ctypes.data_as(ctypes.c_void_p)))
 
+    def set_porosity(self, alpha):
+        """Brinkman porosity per cell (alpha >= 0), or None to remove it.
+
+        New in this implementation; the reference has only solid fractions.
+        """
+        if alpha is None:
+            check(lib().strh_set_porosity(self._h, None))
+            return
+        a = np.ascontiguousarray(alpha, dtype=np.float32).reshape(-1)
+        if a.size != self.cells:
+            raise ValueError("alpha must hold one value per cell")
+        check(lib().strh_set_porosity(self._h, a.ctypes.data_as(ctypes.c_void_p)))
+
+    # ---- differentiation (no reference counterpart) ----------------------
+    @staticmethod
+    def _objective(inlet_x, outlet_x, spinup, avg, w_p, w_d):
+        from ._lib import Objective
+        o = Objective()
+        o.inlet_x, o.outlet_x = int(inlet_x), int(outlet_x)
+        o.w_p, o.w_d = float(w_p), float(w_d)
+        o.spinup, o.avg = int(spinup), int(avg)
+        return o
+
+    def objective(self, inlet_x: int, outlet_x: int, spinup: int, avg: int,
+                  w_p: float = 1.0, w_d: float = 1.0) -> float:
+        """Advance spinup + avg steps; return the time average over the last avg
+        steps of w_p (rho_in - rho_out)/3 + w_d sum alpha J_x (FP64)."""
+        o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d)
+        J = ctypes.c_double()
+        check(lib().strh_objective_eval(self._h, ctypes.byref(o), ctypes.byref(J)))
+        return J.value
+
+    def vjp(self, inlet_x: int, outlet_x: int, spinup: int, avg: int, w_p: float = 1.0,
+            w_d: float = 1.0, checkpoint_every: int = 0, want_alpha: bool = True):
+        """Discrete adjoint of `objective`: returns (J, dJ/dalpha per cell as
+        float64 (nz, ny, nx) or None, dJ/d uin_target, info dict). Leaves the
+        lattice as `run(spinup + avg)` would."""
+        from ._lib import VjpInfo
+        o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d)
+        J, gu, info = ctypes.c_double(), ctypes.c_double(), VjpInfo()
+        ga = np.zeros(self.cells, dtype=np.float64) if want_alpha else None
+        check(lib().strh_vjp(self._h, ctypes.byref(o), int(checkpoint_every), ctypes.byref(J),
+                             ga.ctypes.data_as(ctypes.c_void_p) if want_alpha else None,
+                             ctypes.byref(gu), ctypes.byref(info)))
+        meta = {k: getattr(info, k) for k, _ in VjpInfo._fields_}
+        return (J.value, ga.reshape(self.nz, self.ny, self.nx) if want_alpha else None,
+                gu.value, meta)
+
     def init_field(self, mode: int, amplitude: float):
         """mode 0 rest, 1 2-D Taylor-Green, 2 3-D TGV (Sim.swift:424-439)."""
         check(lib().strh_init_field(self._h, int(mode), float(amplitude)))

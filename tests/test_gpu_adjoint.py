@@ -1,0 +1,153 @@
+"""GPU: Brinkman-porosity objective and its discrete adjoint (strh_vjp).
+
+The FP64 oracle adjoint is pinned to central finite differences on the CPU;
+here the HIP path (FP32 forward, FP32 adjoint state, FP64 accumulation) is
+held to it:
+
+* the recording forward leaves the lattice bit for bit where strh_run does;
+* J within 1e-6 relative and the gradients within 1e-4 of the largest
+  component of the FP64 oracle's (the bar for gradients is 1e-4);
+* the gradient does not depend on the checkpoint interval, bit for bit;
+* lattices with features the adjoint does not cover are refused.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from tests.helpers import STORAGE
+
+pytestmark = pytest.mark.gpu
+
+INFLOW, SOLID, LID, OUTFLOW = 3, 1, 2, 4
+
+
+def _channel(nx, ny, nz=1):
+    fl = np.zeros((nz, ny, nx), dtype=np.uint8)
+    fl[:, :, 0] = INFLOW
+    fl[:, :, nx - 1] = INFLOW
+    fl[:, 0, :] = SOLID
+    fl[:, ny - 1, :] = SOLID
+    return fl
+
+
+def _mixed(nx, ny, nz, seed):
+    rng = np.random.default_rng(seed)
+    fl = rng.choice([0] * 12 + [1, 2, 3, 4], size=(nz, ny, nx)).astype(np.uint8)
+    fl[:, :, 0] = INFLOW
+    return fl
+
+
+def _alpha(shape, seed):
+    nz, ny, nx = shape
+    rng = np.random.default_rng(seed)
+    a = np.zeros(shape, dtype=np.float32)
+    a[:, 2:ny - 2, nx // 3:2 * nx // 3] = rng.uniform(0.0, 0.1, size=(nz, ny - 4, 2 * nx // 3 - nx // 3))
+    return a
+
+
+def _pair(gpu, precision, flags, alpha, uin=0.05, ramp=5, force=(1e-5, 0.0, 0.0), lid=(0.0, 0.0, 0.0)):
+    from strouhal_amd import Simulation, trt_omegas
+    nz, ny, nx = flags.shape
+    wp, wm = trt_omegas(0.7, 3.0 / 16.0)
+    g = Simulation(gpu, precision, nx=nx, ny=ny, nz=nz, omega=wp, omega_minus=wm, uin=uin,
+                   ramp_steps=ramp, force=force, lid=lid, flags=flags)
+    g.init_field(0, 0.0)
+    r = OracleSim(nx, ny, nz, g.omega, g.omega_minus, arith="f64", storage="f64", lid=g.lid_vel,
+                  uin=g.uin_target, ramp_steps=g.ramp_steps, force=g.force, flags=flags)
+    r.init_field(0, 0.0)
+    if alpha is not None:
+        g.set_porosity(alpha)
+        r.set_porosity(alpha)
+    return g, r
+
+
+OBJ = dict(inlet_x=2, outlet_x=15, spinup=6, avg=11, w_p=1.0, w_d=0.5)
+
+
+def _close(a, b, rel):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    scale = np.max(np.abs(b))
+    assert scale > 0
+    err = np.max(np.abs(a - b)) / scale
+    assert err <= rel, err
+
+
+@pytest.mark.parametrize("porous", [True, False])
+def test_vjp_matches_fp64_oracle_channel(gpu, porous):
+    fl = _channel(18, 10)
+    g, r = _pair(gpu, "fp32", fl, _alpha(fl.shape, 1) if porous else None)
+    J, ga, gu, info = g.vjp(**OBJ)
+    Jr, gar, gur = r.vjp(**OBJ)
+    assert J == pytest.approx(Jr, rel=1e-6)
+    _close(ga.reshape(-1), gar, 1e-4)
+    assert gu == pytest.approx(gur, rel=1e-4)
+    assert info["checkpoint_every"] == 5 and info["checkpoints"] == 3
+    assert np.all(ga[:, [0, 9], :] == 0.0)
+
+
+def test_vjp_matches_fp64_oracle_3d_mixed_walls(gpu):
+    """Every wall kind (solid, moving lid, inflow, dormant outflow) in 3D."""
+    fl = _mixed(14, 11, 5, seed=3)
+    g, r = _pair(gpu, "fp32", fl, _alpha(fl.shape, 2), lid=(0.03, 0.0, 0.01),
+                 force=(1e-5, -5e-6, 2e-6))
+    obj = dict(inlet_x=1, outlet_x=12, spinup=3, avg=8, w_p=1.0, w_d=1.0)
+    J, ga, gu, _ = g.vjp(**obj, checkpoint_every=4)
+    Jr, gar, gur = r.vjp(**obj)
+    assert J == pytest.approx(Jr, rel=1e-6)
+    _close(ga.reshape(-1), gar, 1e-4)
+    assert gu == pytest.approx(gur, rel=1e-4)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16s", "bf16"])
+def test_recording_forward_keeps_the_state(gpu, precision):
+    """strh_vjp and strh_objective_eval advance the lattice exactly as strh_run."""
+    fl = _channel(20, 9, 2)
+    a = _alpha(fl.shape, 4)
+    sims = [_pair(gpu, precision, fl, a)[0] for _ in range(3)]
+    sims[0].run(17)
+    sims[1].vjp(**OBJ)
+    sims[2].objective(**OBJ)
+    d = [s.state_digest for s in sims]
+    assert d[0] == d[1] == d[2]
+    assert all(s.steps_done == 17 for s in sims)
+    plain = [_pair(gpu, precision, fl, None)[0] for _ in range(2)]
+    plain[0].run(17)
+    plain[1].vjp(**OBJ)
+    assert plain[0].state_digest == plain[1].state_digest
+
+
+def test_gradient_independent_of_checkpoint_interval(gpu):
+    fl = _channel(18, 10)
+    out = []
+    for k in (1, 3, 17, 0):
+        g, _ = _pair(gpu, "fp32", fl, _alpha(fl.shape, 1))
+        out.append(g.vjp(**OBJ, checkpoint_every=k))
+    for J, ga, gu, _ in out[1:]:
+        assert J == out[0][0] and gu == out[0][2]
+        assert np.array_equal(ga, out[0][1])
+
+
+def test_objective_eval_matches_vjp(gpu):
+    fl = _channel(18, 10)
+    g1, _ = _pair(gpu, "fp32", fl, _alpha(fl.shape, 1))
+    g2, _ = _pair(gpu, "fp32", fl, _alpha(fl.shape, 1))
+    assert g1.objective(**OBJ) == g2.vjp(**OBJ)[0]
+
+
+def test_vjp_refuses_unsupported_lattices(gpu):
+    from strouhal_amd import StrouhalError
+    fl = _channel(18, 10)
+    g, _ = _pair(gpu, "fp32", fl, None)
+    g.c_smago = 0.02
+    with pytest.raises(StrouhalError) as e:
+        g.vjp(**OBJ)
+    assert e.value.code == 8
+    g.c_smago = 0.0
+    with pytest.raises(StrouhalError) as e:
+        g.vjp(**dict(OBJ, inlet_x=18))
+    assert e.value.code == 1
+    with pytest.raises(StrouhalError):
+        g.vjp(**dict(OBJ, avg=0))
+    assert g.steps_done == 0

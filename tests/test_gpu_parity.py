@@ -144,6 +144,54 @@ def test_noble_torczynski_with_forces(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("kind,shape,force", [("channel", (30, 14, 2), (1e-5, 0.0, 0.0)),
+                                              ("periodic", (64, 8, 3), (0.0, 0.0, 0.0)),
+                                              ("mixed", (15, 9, 7), (2e-5, -1e-5, 5e-6))])
+def test_brinkman_porosity(gpu, precision, kind, shape, force):
+    """Brinkman drag through the Guo terms (also with no body force, and on a
+    wall-free lattice that would otherwise take the row-staged odd pass)."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx + nz)
+    alpha = np.zeros((nz, ny, nx), dtype=np.float32)
+    x0, x1 = nx // 4, 3 * nx // 4
+    alpha[:, 2:ny - 2, x0:x1] = rng.uniform(0.0, 0.3, size=(nz, ny - 4, x1 - x0))
+    alpha[:, 3, nx // 2] = 5.0
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.3, 1.6, uin=0.05, ramp_steps=4, force=force,
+                     lid=(0.04, 0.0, 0.0), flags=_flags(kind, nx, ny, nz, seed=5), wants_forces=True)
+    g.set_porosity(alpha)
+    r.set_porosity(alpha)
+    _seed_both(g, r, precision, seed=17)
+    _run_compare(g, r, [1, 2, 3], precision)
+    fg = g.probe_force((1, nx - 2), (1, ny - 2))
+    fr = r.probe_force((1, nx - 2), (1, ny - 2))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+    scale = np.abs(r.force_field()[:, :3]).sum() + 1e-300
+    assert np.all(np.abs(fg - fr) <= 1e-13 * scale), (fg, fr)
+    g.set_porosity(None)           # back to the plain lattice
+    r.set_porosity(np.zeros(g.cells, dtype=np.float32))
+    _run_compare(g, r, [2], precision)
+
+
+def test_brinkman_rejects_les_nt_and_batches(gpu):
+    from strouhal_amd import StrouhalError
+    from strouhal_amd.batch import Batch
+    g, _ = make_pair(gpu, "fp32", 16, 8, 2, 1.3)
+    with pytest.raises(StrouhalError):
+        g.set_porosity(np.full(g.cells, -1.0, dtype=np.float32))
+    g.set_porosity(np.full(g.cells, 0.1, dtype=np.float32))
+    g.c_smago = 0.02
+    with pytest.raises(StrouhalError):
+        g.run(2)
+    g.c_smago = 0.0
+    g.set_solid_fractions(np.zeros(g.cells, dtype=np.float32))
+    with pytest.raises(StrouhalError):
+        g.run(2)
+    with pytest.raises(StrouhalError):
+        Batch([g])
+    assert g.steps_done == 0
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 def test_row_staged_odd_pass(gpu, precision):
     """Wall-free lattices with nx a power of two >= 64 run the odd pass with LDS
     row staging (256- or 512-thread workgroups); same bits as the oracle."""

@@ -1,0 +1,164 @@
+"""CPU: Brinkman porosity and the discrete adjoint of the FP64 oracle.
+
+Neither has a counterpart in the reference (it has only Noble-Torczynski
+solid fractions, Kernels.metal:97-104, and no gradients), so nothing there
+pins them. They are pinned here by mathematics:
+
+* a uniform-porosity channel driven by a body force relaxes to the analytic
+  Brinkman profile u(y) = F/(rho alpha) (1 - cosh(k y) / cosh(k h)),
+  k = sqrt(alpha / nu);
+* alpha = 0 everywhere is the plain forced update, bit for bit;
+* the adjoint's dJ/dalpha and dJ/d(uin) equal central finite differences of
+  the FP64 objective to 1e-6 relative (the bar for the product is 1e-4), with
+  the perturbation measured after the float32 rounding the porosity field
+  goes through.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+
+INFLOW, SOLID = 3, 1
+
+
+def _channel(nx, ny):
+    fl = np.zeros((1, ny, nx), dtype=np.uint8)
+    fl[:, :, 0] = INFLOW
+    fl[:, :, nx - 1] = INFLOW
+    fl[:, 0, :] = SOLID
+    fl[:, ny - 1, :] = SOLID
+    return fl
+
+
+def _walls(nx, ny):
+    fl = np.zeros((1, ny, nx), dtype=np.uint8)
+    fl[:, 0, :] = SOLID
+    fl[:, ny - 1, :] = SOLID
+    return fl
+
+
+@pytest.mark.parametrize("lam", [None, 3.0 / 16.0])
+def test_brinkman_channel_matches_cosh_profile(lam):
+    ny, tau, alpha, fb = 34, 0.8, 1e-3, 1e-6
+    wp = 1.0 / tau
+    wm = wp if lam is None else 1.0 / (0.5 + lam / (tau - 0.5))
+    sim = OracleSim(2, ny, 1, wp, wm, arith="f64", storage="f64", force=(fb, 0.0, 0.0),
+                    flags=_walls(2, ny))
+    sim.set_porosity(np.full(2 * ny, alpha, dtype=np.float32))
+    sim.init_field(0, 0.0)
+    sim.run(20000)
+    m = sim.probe_moments().reshape(ny, 2, 4)
+    u = m[1:ny - 1, 0, 0]            # moments are (ux, uy, uz, rho)
+    a = float(np.float32(alpha))
+    nu = (tau - 0.5) / 3.0
+    k = math.sqrt(a / nu)
+    h = (ny - 2) / 2.0                        # halfway walls at y = 0.5 and ny - 1.5
+    y = np.arange(1, ny - 1) - (ny - 1) / 2.0
+    ua = fb / a * (1.0 - np.cosh(k * y) / math.cosh(k * h))
+    err = np.max(np.abs(u - ua)) / np.max(ua)
+    assert err < 5e-3, err
+    # total porous drag = the integral of alpha u over the gap, F (2h - 2 tanh(kh) / k);
+    # the rest of the body force goes into wall shear
+    rho = m[1:ny - 1, 0, 3]
+    drag = fb * (2.0 * h - 2.0 * math.tanh(k * h) / k)
+    assert float(np.sum(a * rho * u)) == pytest.approx(drag, rel=2e-3)
+
+
+@pytest.mark.parametrize("arith,storage", [("f64", "f64"), ("f32", "f32"), ("f32", "f16")])
+def test_zero_porosity_is_the_plain_forced_update(arith, storage):
+    rng = np.random.default_rng(3)
+    fl = _channel(12, 9)
+
+    def make():
+        s = OracleSim(12, 9, 2, 1.3, 1.1, arith=arith, storage=storage, force=(2e-5, -1e-5, 0.0),
+                      uin=0.04, ramp_steps=6, flags=np.repeat(fl, 2, axis=0))
+        s.init_field(0, 0.0)
+        return s
+    a, b = make(), make()
+    b.set_porosity(np.zeros(a.cells, dtype=np.float32))
+    raw = a.download_ddf()
+    if raw.dtype == np.float64:
+        raw = raw + 1e-3 * rng.standard_normal(raw.shape)
+    a.upload_ddf(raw)
+    b.upload_ddf(raw)
+    a.run(9)
+    b.run(9)
+    assert a.state_digest == b.state_digest
+
+
+def _adjoint_case(uin=0.05, ramp=5, seed=0):
+    nx, ny = 18, 10
+    rng = np.random.default_rng(seed)
+    fl = _channel(nx, ny)
+    alpha = np.zeros((ny, nx), dtype=np.float32)
+    alpha[2:8, 6:11] = rng.uniform(0.005, 0.08, size=(6, 5)).astype(np.float32)
+
+    def make(al=alpha, u=uin):
+        tau = 0.7
+        wp = 1.0 / tau
+        wm = 1.0 / (0.5 + 0.1875 / (tau - 0.5))
+        s = OracleSim(nx, ny, 1, wp, wm, arith="f64", storage="f64", uin=u, ramp_steps=ramp,
+                      force=(1e-5, 0.0, 0.0), flags=fl)
+        s.set_porosity(al)
+        s.init_field(0, 0.0)
+        return s
+    return make, alpha
+
+
+OBJ = dict(inlet_x=2, outlet_x=15, spinup=6, avg=11, w_p=1.0, w_d=0.5)
+
+
+def test_adjoint_alpha_gradient_matches_central_differences():
+    make, alpha = _adjoint_case()
+    J, ga, gu = make().vjp(**OBJ)
+    assert J == make().objective(**OBJ)
+    ny, nx = alpha.shape
+    cells = [(3, 7), (5, 9), (7, 10), (4, 4), (2, 13), (1, 8)]   # porous, plain fluid, wall-adjacent
+    h = 1e-4
+    for (y, x) in cells:
+        ap, am = alpha.copy(), alpha.copy()
+        ap[y, x] = np.float32(alpha[y, x] + h)
+        am[y, x] = np.float32(alpha[y, x] - h)      # the map is smooth through alpha = 0
+        d = float(ap[y, x]) - float(am[y, x])
+        fd = (make(ap).objective(**OBJ) - make(am).objective(**OBJ)) / d
+        g = ga[y * nx + x]
+        assert abs(g - fd) <= 1e-6 * abs(fd), ((y, x), g, fd)
+    assert np.all(ga.reshape(ny, nx)[[0, ny - 1], :] == 0.0)   # walls carry no gradient
+
+
+@pytest.mark.parametrize("ramp", [0, 5])
+def test_adjoint_uin_gradient_matches_central_differences(ramp):
+    make, alpha = _adjoint_case(ramp=ramp)
+    u0, h = 0.05, 1e-5
+    J, ga, gu = make(u=u0).vjp(**OBJ)
+    fd = (make(u=u0 + h).objective(**OBJ) - make(u=u0 - h).objective(**OBJ)) / (2 * h)
+    assert gu == pytest.approx(fd, rel=1e-6)
+
+
+def test_adjoint_objective_terms_separately():
+    """Each objective term alone: pressure drop only, Brinkman drag only."""
+    make, alpha = _adjoint_case(seed=1)
+    nx = alpha.shape[1]
+    for wts in [(1.0, 0.0), (0.0, 1.0)]:
+        o = dict(OBJ, w_p=wts[0], w_d=wts[1])
+        J, ga, gu = make().vjp(**o)
+        y, x = 4, 8
+        h = 1e-4
+        ap, am = alpha.copy(), alpha.copy()
+        ap[y, x] += np.float32(h)
+        am[y, x] -= np.float32(h)
+        d = float(ap[y, x]) - float(am[y, x])
+        fd = (make(ap).objective(**o) - make(am).objective(**o)) / d
+        assert ga[y * nx + x] == pytest.approx(fd, rel=1e-6)
+
+
+def test_adjoint_rejects_unsupported_features():
+    make, _ = _adjoint_case()
+    s = make()
+    s.c_smago = 0.01
+    J, ga, gu = s.vjp(**OBJ)
+    assert math.isnan(J)
