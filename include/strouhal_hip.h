@@ -211,12 +211,17 @@ STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, d
 /* Advance T steps, return J and its gradient with respect to the porosity of
  * every cell (grad_alpha: N doubles, or NULL; zero on non-fluid cells) and to
  * the inlet velocity target uin (grad_uin, or NULL). The state after the call
- * is the state after strh_run(lat, T), bit for bit. The backward pass replays
- * segments of checkpoint_every steps (0: ceil(sqrt(T))) from DDF checkpoints,
- * recording the pre-collision populations of one segment at a time in FP32;
- * the adjoint state is FP32 and every accumulation FP64 in a fixed order.
- * Device memory: ceil(T/K) DDF images + K * 19 * N floats + 2 * 19 * N floats
- * + 3 * N doubles. info may be NULL. */
+ * is the state after strh_run(lat, T), bit for bit. The forward pass keeps a
+ * DDF checkpoint at the start of every segment of K = checkpoint_every steps;
+ * the reverse pass replays one segment at a time, recording the four
+ * pre-collision moments (rho - 1, p) per cell and step in FP32 (the
+ * collision's Jacobian depends on the state only through them), and runs the
+ * reverse steps over it. The last segment is recorded in the forward pass, so
+ * K >= T replays nothing. K = 0: ceil(sqrt(T)); K = -1: the largest K whose
+ * recording fits half the free device memory. The adjoint state is FP32;
+ * every accumulation is FP64 in a fixed order, so the result does not depend
+ * on K. Device memory: (ceil(T/K) - 1) DDF images + K * N * 16 bytes +
+ * 2 * 19 * N floats + 3 * N doubles + one DDF image. info may be NULL. */
 STRH_API int strh_vjp(strh_lattice* lat, const strh_objective* obj, int64_t checkpoint_every,
                       double* J, double* grad_alpha, double* grad_uin, strh_vjp_info* info);
 

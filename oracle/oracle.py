@@ -82,7 +82,7 @@ def lib():
         L.orc_set_porosity.argtypes = [vp, ctypes.c_void_p]
         L.orc_objective.argtypes = [vp, ctypes.POINTER(Objective)]
         L.orc_objective.restype = ctypes.c_double
-        L.orc_vjp.argtypes = [vp, ctypes.POINTER(Objective), ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_vjp.argtypes = [vp, ctypes.POINTER(Objective), ctypes.c_void_p, ctypes.c_void_p, i64]
         L.orc_vjp.restype = ctypes.c_double
         L.orc_init_field.argtypes = [vp, i32, ctypes.c_double]
         L.orc_run.argtypes = [vp, i64]
@@ -226,13 +226,16 @@ class OracleSim:
         o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg)
         return float(lib().orc_objective(self._h, ctypes.byref(o)))
 
-    def vjp(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0):
-        """FP64 discrete adjoint: (J, dJ/dalpha per cell, dJ/d uin_target)."""
+    def vjp(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0, checkpoint_every=0):
+        """FP64 discrete adjoint: (J, dJ/dalpha per cell, dJ/d uin_target).
+
+        Replays segments of checkpoint_every steps (0: ceil(sqrt(T))) from DDF
+        checkpoints; the result does not depend on the interval."""
         self._push()
         o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg)
         ga = np.zeros(self.cells, dtype=np.float64)
         gu = np.zeros(1, dtype=np.float64)
-        J = float(lib().orc_vjp(self._h, ctypes.byref(o), _ptr(ga), _ptr(gu)))
+        J = float(lib().orc_vjp(self._h, ctypes.byref(o), _ptr(ga), _ptr(gu), int(checkpoint_every)))
         return J, ga, float(gu[0])
 
     def init_field(self, mode: int, amplitude: float):

@@ -192,7 +192,7 @@ struct Base {
     virtual void upload(const void* in) = 0;
     virtual void collideCell(const double* fin, double eps, int x, double* fout,
                              double* rho, double* ntf) = 0;
-    virtual double vjp(const Objective& o, double* gAlpha, double* gUin) = 0;
+    virtual double vjp(const Objective& o, double* gAlpha, double* gUin, int64_t checkpointEvery) = 0;
     virtual double objective(const Objective& o) = 0;
 };
 
@@ -638,7 +638,11 @@ template <class S> struct Lattice final : Base {
     // gather (neighbour pull, bounce-back, moving-wall corrections) otherwise.
     void gatherState(const Params<R>& p, std::vector<double>& g) const {
         g.assign(19 * N, 0.0);
-        for (size_t n = 0; n < N; n++) {
+#ifdef _OPENMP
+        #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+#endif
+        for (long long nn = 0; nn < (long long)N; nn++) {
+            const size_t n = (size_t)nn;
             if (flags[n] != FLUID) continue;
             const int x = (int)(n % (size_t)nx);
             const size_t t = n / (size_t)nx;
@@ -703,6 +707,20 @@ template <class S> struct Lattice final : Base {
         return !(hp.c_smago > 0.0 || hp.sponge_on || useEps);
     }
 
+    // Objective terms of step k's pre-collision state g, summed over the
+    // fluid cells in index order.
+    double windowSum(const std::vector<double>& g, const Params<R>& p, const std::vector<double>& cp,
+                     const Objective& o) const {
+        double sum = 0.0;
+        double gc[19];
+        for (size_t n = 0; n < N; n++) {
+            if (flags[n] != FLUID) continue;
+            for (int i = 0; i < 19; i++) gc[i] = g[i * N + n];
+            sum += cellObjective(gc, useAlpha ? (double)alpha[n] : 0.0, p, cp[n], o);
+        }
+        return sum;
+    }
+
     double objective(const Objective& o) override {
         if (!std::is_same<R, double>::value || !adjointSupported()) return std::nan("");
         std::vector<double> g, cp;
@@ -713,14 +731,7 @@ template <class S> struct Lattice final : Base {
             const Params<R> p = params(stepsDone, false);
             if (k >= o.spinup) {
                 gatherState(p, g);
-                double sum = 0.0;
-                std::vector<double> gc(19);
-                for (size_t n = 0; n < N; n++) {
-                    if (flags[n] != FLUID) continue;
-                    for (int i = 0; i < 19; i++) gc[i] = g[i * N + n];
-                    sum += cellObjective(gc.data(), useAlpha ? (double)alpha[n] : 0.0, p, cp[n], o);
-                }
-                J += sum / (double)o.avg;
+                J += windowSum(g, p, cp, o) / (double)o.avg;
             }
             run(1, -1);
         }
@@ -736,33 +747,43 @@ template <class S> struct Lattice final : Base {
     //   E_i = w+ feq+_i + w- feq-_i + S_i(Guo),  J = (p + Fb/2)/(1 + alpha/2),
     //   F = Fb - alpha J, feq+ = w(rho - 1 + 4.5 (c.J)^2/rho - 1.5 J^2/rho),
     //   feq- = 3 w c.J, S = w[a+ (9 (c.J)(c.F) - 3 J.F)/rho + 3 a- c.F].
-    double vjp(const Objective& o, double* gAlpha, double* gUin) override {
+    double vjp(const Objective& o, double* gAlpha, double* gUin, int64_t K) override {
         if (!std::is_same<R, double>::value || !adjointSupported()) return std::nan("");
         const int64_t total = o.spinup + o.avg;
+        if (K <= 0) K = (int64_t)std::ceil(std::sqrt((double)total));
+        if (K > total) K = total;
+        const int64_t C = (total + K - 1) / K;
         const int64_t t0 = stepsDone;
-        std::vector<double> G((size_t)total * 19 * N), g, cp;
+        std::vector<double> g, cp;
         pressureWeights(o, cp);
+        // forward pass: objective and a DDF checkpoint every K steps
+        std::vector<std::vector<T>> ck((size_t)C);
         double J = 0.0;
-        std::vector<double> gc(19);
         for (int64_t k = 0; k < total; k++) {
+            if (k % K == 0) ck[(size_t)(k / K)] = f;
             const Params<R> p = params(stepsDone, false);
-            gatherState(p, g);
-            std::copy(g.begin(), g.end(), G.begin() + (size_t)k * 19 * N);
             if (k >= o.spinup) {
-                double sum = 0.0;
-                for (size_t n = 0; n < N; n++) {
-                    if (flags[n] != FLUID) continue;
-                    for (int i = 0; i < 19; i++) gc[i] = g[i * N + n];
-                    sum += cellObjective(gc.data(), useAlpha ? (double)alpha[n] : 0.0, p, cp[n], o);
-                }
-                J += sum / (double)o.avg;
+                gatherState(p, g);
+                J += windowSum(g, p, cp, o) / (double)o.avg;
             }
             run(1, -1);
         }
+        std::vector<T> fEnd = f;
+        const int64_t tEnd = stepsDone;
+        std::vector<double> G((size_t)K * 19 * N);
         std::vector<double> lamNext(19 * N, 0.0), lam(19 * N, 0.0);
-        std::vector<double> ga(N, 0.0);
-        double gu = 0.0;
-        for (int64_t k = total - 1; k >= 0; k--) {
+        std::vector<double> ga(N, 0.0), guc(N, 0.0);
+        for (int64_t seg = C - 1; seg >= 0; seg--) {
+          // replay the segment from its checkpoint, keeping every state
+          const int64_t k0 = seg * K, len = std::min(K, total - k0);
+          f = ck[(size_t)seg];
+          stepsDone = t0 + k0;
+          for (int64_t s2 = 0; s2 < len; s2++) {
+              gatherState(params(stepsDone, false), g);
+              std::copy(g.begin(), g.end(), G.begin() + (size_t)s2 * 19 * N);
+              run(1, -1);
+          }
+          for (int64_t k = k0 + len - 1; k >= k0; k--) {
             const int64_t t = t0 + k;
             const Params<R> p = params(t, false);
             const int64_t todd = (t & 1) ? t : t + 1;
@@ -772,8 +793,12 @@ template <class S> struct Lattice final : Base {
             const double ap = 1.0 - 0.5 * wp, am = 1.0 - 0.5 * wm;
             const double Fb[3] = {(double)p.fx, (double)p.fy, (double)p.fz};
             std::fill(lam.begin(), lam.end(), 0.0);
-            const double* gk = &G[(size_t)k * 19 * N];
-            for (size_t n = 0; n < N; n++) {
+            const double* gk = &G[(size_t)(k - k0) * 19 * N];
+#ifdef _OPENMP
+            #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+#endif
+            for (long long nn = 0; nn < (long long)N; nn++) {
+                const size_t n = (size_t)nn;
                 if (flags[n] != FLUID) continue;
                 const int x = (int)(n % (size_t)nx);
                 const size_t tt = n / (size_t)nx;
@@ -795,7 +820,7 @@ template <class S> struct Lattice final : Base {
                                 prof = 4.0 * yd * (H - yd) / (H * H);
                                 if (!(hp.uin_target * rOdd * prof > 0.0)) prof = 0.0;
                             }
-                            gu += ln * (-6.0 * weight<double>(i) * CX[i] * rOdd * prof);
+                            guc[n] += ln * (-6.0 * weight<double>(i) * CX[i] * rOdd * prof);
                         }
                     } else {
                         mu[i] = lamNext[i * N + d];
@@ -855,7 +880,12 @@ template <class S> struct Lattice final : Base {
                        + wobj * o.w_d * Jv[0] * (1.0 - a / (2.0 * gam));
             }
             lamNext.swap(lam);
+          }
         }
+        f.swap(fEnd);
+        stepsDone = tEnd;
+        double gu = 0.0;
+        for (size_t n = 0; n < N; n++) gu += guc[n];
         if (gAlpha) for (size_t n = 0; n < N; n++) gAlpha[n] = ga[n];
         if (gUin) *gUin = gu;
         return J;
@@ -899,8 +929,9 @@ void orc_set_solid_fractions(void* h, const float* eps) { ((Base*)h)->setEps(eps
 void orc_set_porosity(void* h, const float* alpha) { ((Base*)h)->setAlpha(alpha); }
 // FP64 lattices only (NaN otherwise, or with LES / sponge / NT cells).
 double orc_objective(void* h, const Objective* o) { return ((Base*)h)->objective(*o); }
-double orc_vjp(void* h, const Objective* o, double* grad_alpha, double* grad_uin) {
-    return ((Base*)h)->vjp(*o, grad_alpha, grad_uin);
+double orc_vjp(void* h, const Objective* o, double* grad_alpha, double* grad_uin,
+               int64_t checkpoint_every) {
+    return ((Base*)h)->vjp(*o, grad_alpha, grad_uin, checkpoint_every);
 }
 void orc_init_field(void* h, int mode, double amplitude) { ((Base*)h)->initField(mode, amplitude); }
 void orc_run(void* h, int64_t steps) { ((Base*)h)->run(steps, -1); }

@@ -59,14 +59,14 @@ struct BatchLat {
 // Hooks of the recording forward step of the adjoint (walls, Guo forcing and
 // Brinkman porosity; no partial cells, LES or sponge).
 struct AdjHooks {
-    float* rec;              // [19][S] pre-collision populations of the step, or null
+    float4* rec;             // [N] pre-collision moments (rho - 1, p_x, p_y, p_z), or null
     double* jacc;            // per-cell objective accumulator, or null
     int32_t inletX, outletX; // columns of the pressure-drop term
     double cin, cout;        // w_p / (3 n_in T), w_p / (3 n_out T)
     double wd;               // w_d / T
 };
 
-// One reverse step: lambda^t from lambda^{t+1} and the recorded g^t.
+// One reverse step: lambda^t from lambda^{t+1} and the recorded moments of g^t.
 struct AdjStep {
     uint32_t nx, ny, nz, N, S, blocks;
     FastDiv divx, divy;
@@ -82,7 +82,7 @@ hipError_t launch_step(const StepLaunch& L, hipStream_t s);
 // Forward step with the adjoint hooks; L.alpha must be set (zeros for a
 // lattice without porosity).
 hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s);
-hipError_t launch_adjoint_step(const AdjStep& q, bool walls, const float* rec, const float* lamNext,
+hipError_t launch_adjoint_step(const AdjStep& q, bool walls, const float4* rec, const float* lamNext,
                                float* lam, const uint8_t* flags, const uint32_t* link,
                                const float* alpha, double* gAlpha, double* gUin, hipStream_t s);
 hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,

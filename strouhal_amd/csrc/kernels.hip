@@ -360,30 +360,36 @@ constexpr uint32_t kStepBS = 256;
 
 // One cell of one AA pass (the body shared by the single-lattice kernel and
 // the multi-lattice batch kernel).
-// Adjoint hooks of the forward pass (AdjHooks in kernels.h): record the
-// pre-collision populations the collision sees (natural slot order) and
-// accumulate the cell's objective term in FP64,
+// Adjoint hooks of the forward pass (AdjHooks in kernels.h). The collision's
+// Jacobian depends on the state only through rho and p, so the reverse step
+// needs just these four moments of the pre-collision populations (summed in
+// FP64, stored as FP32: rho - 1 keeps the density's small part exact to the
+// float's 24 bits). The hook also accumulates the cell's objective term,
 //   cp(x) rho + wd alpha J_x,   J_x = (p_x + F_x / 2) / (1 + alpha / 2),
 // with cp = +cin on the inlet column and -cout on the outlet column.
 __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[19], uint32_t n,
-                                          uint32_t S, int x, float alpha, float fx) {
-    if (hk.rec) {
+                                          int x, float alpha, float fx) {
+    if (!hk.rec && !hk.jacc) return;
+    double rm1 = 0.0, px = 0.0, py = 0.0, pz = 0.0;
 #pragma unroll
-        for (int i = 0; i < 19; i++) __builtin_nontemporal_store(fh[i], hk.rec + (size_t)i * S + n);
+    for (int i = 0; i < 19; i++) {
+        const double v = (double)fh[i];
+        rm1 += v;
+        if (kCx[i]) px += (double)kCx[i] * v;
+        if (kCy[i]) py += (double)kCy[i] * v;
+        if (kCz[i]) pz += (double)kCz[i] * v;
+    }
+    if (hk.rec) {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        const f4v m = {(float)rm1, (float)px, (float)py, (float)pz};
+        __builtin_nontemporal_store(m, reinterpret_cast<f4v*>(hk.rec) + n);
     }
     if (hk.jacc) {
         const double cp = (x == hk.inletX ? hk.cin : 0.0) - (x == hk.outletX ? hk.cout : 0.0);
         if (cp != 0.0 || (hk.wd != 0.0 && alpha != 0.0f)) {
-            double rho = 1.0, px = 0.0;
-#pragma unroll
-            for (int i = 0; i < 19; i++) {
-                rho += (double)fh[i];
-                if (kCx[i] > 0) px += (double)fh[i];
-                if (kCx[i] < 0) px -= (double)fh[i];
-            }
             const double a = (double)alpha;
             const double Jx = (px + 0.5 * (double)fx) / (1.0 + 0.5 * a);
-            hk.jacc[n] += cp * rho + hk.wd * a * Jx;
+            hk.jacc[n] += cp * (1.0 + rm1) + hk.wd * a * Jx;
         }
     }
 }
@@ -419,7 +425,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 #pragma unroll
         for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nb);
         const float alpha = BRINK ? alphaBuf[n] : 0.0f;
-        if (HOOK) run_hooks(*hk, fh, n, p.S, x, alpha, p.fx);
+        if (HOOK) run_hooks(*hk, fh, n, x, alpha, p.fx);
         collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);
         // A separate copy of the offset keeps the stores on the scalar-base
         // form too (otherwise the shared base+offset sum is kept as 19
@@ -508,7 +514,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     }
 
     const float alpha = BRINK ? alphaBuf[n] : 0.0f;
-    if (HOOK) run_hooks(*hk, fh, n, p.S, x, alpha, p.fx);
+    if (HOOK) run_hooks(*hk, fh, n, x, alpha, p.fx);
     collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);  // 315-316
     if (NT) {
         ax += ntx;
@@ -608,7 +614,7 @@ __device__ __forceinline__ double wgt_d(int i) {
 
 template <bool WALLS>
 __global__ void __launch_bounds__(256)
-adjoint_step_kernel(const float* __restrict__ rec, const float* __restrict__ lamN,
+adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ lamN,
                     float* __restrict__ lam, const uint8_t* __restrict__ flags,
                     const uint32_t* __restrict__ link, const float* __restrict__ alphaBuf,
                     double* __restrict__ gAlpha, double* __restrict__ gUin, AdjStep q) {
@@ -651,18 +657,9 @@ adjoint_step_kernel(const float* __restrict__ rec, const float* __restrict__ lam
         }
     }
 
-    double g[19];
-#pragma unroll
-    for (int i = 0; i < 19; i++) g[i] = (double)rec[(size_t)i * S + n];
+    const float4 m = rec[n];
     const double a = alphaBuf ? (double)alphaBuf[n] : 0.0;
-    double rho = 1.0, px = 0.0, py = 0.0, pz = 0.0;
-#pragma unroll
-    for (int i = 0; i < 19; i++) {
-        rho += g[i];
-        px += (double)kCx[i] * g[i];
-        py += (double)kCy[i] * g[i];
-        pz += (double)kCz[i] * g[i];
-    }
+    const double rho = 1.0 + (double)m.x, px = (double)m.y, py = (double)m.z, pz = (double)m.w;
     const double gam = 1.0 + 0.5 * a;
     const double Jx = (px + 0.5 * q.fx) / gam, Jy = (py + 0.5 * q.fy) / gam, Jz = (pz + 0.5 * q.fz) / gam;
     const double Fx = q.fx - a * Jx, Fy = q.fy - a * Jy, Fz = q.fz - a * Jz;
@@ -1141,7 +1138,7 @@ hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream
     }
 }
 
-hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float* rec, const float* lamNext,
+hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float4* rec, const float* lamNext,
                                float* lam, const uint8_t* flags, const uint32_t* link,
                                const float* alpha, double* gAlpha, double* gUin, hipStream_t s) {
     AdjStep q = q0;

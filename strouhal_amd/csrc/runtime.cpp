@@ -1052,17 +1052,17 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
 }
 
 // Hooked forward steps k0 .. k0+count-1 of the objective window on buffer f
-// (absolute steps t0 + k): states recorded into rec (count * 19 * S floats)
-// when given, objective accumulated into jacc when given.
+// (absolute steps t0 + k): pre-collision moments recorded into rec (count * N
+// float4) when given, objective accumulated into jacc when given.
 int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_t k0, int64_t count,
-                const strh_objective* o, const AdjPlan& P, float* rec, double* jacc) {
+                const strh_objective* o, const AdjPlan& P, float4* rec, double* jacc) {
     for (int64_t s = 0; s < count; s++) {
         const int64_t k = k0 + s;
         StepLaunch sl = make_launch(L, t0 + k, false);
         sl.f = f;
         sl.alpha = alpha;
         AdjHooks hk;
-        hk.rec = rec ? rec + (size_t)s * 19 * L->S : nullptr;
+        hk.rec = rec ? rec + (size_t)s * L->N : nullptr;
         const bool win = k >= o->spinup;
         hk.jacc = (jacc && win) ? jacc : nullptr;
         hk.inletX = o->inlet_x;
@@ -1129,25 +1129,36 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     AdjPlan P;
     if ((rc = adj_plan(L, o, P))) return rc;
     const int64_t T = P.T;
-    if (K < 0) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= 0");
-    if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
-    if (K > T) K = T;
-    const int64_t C = (T + K - 1) / K;
     const size_t N = L->N, S = L->S;
     const size_t ddf = 19 * slot_bytes(L);
+    if (K < -1) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= -1");
+    if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
+    if (K == -1) {
+        // fewest replays: the longest segment whose recording fits half the
+        // free device memory left after the fixed buffers
+        size_t freeB = 0, totalB = 0;
+        HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+        const double fixed = (double)N * (3 * 8 + 4) + 2.0 * 19 * S * 4 + 2.0 * ddf;
+        const double budget = 0.5 * (double)freeB - fixed;
+        const double perStep = (double)N * 16 + (double)ddf / 64.0;   // recording + its share of checkpoints
+        K = budget > perStep ? (int64_t)(budget / perStep) : 1;
+    }
+    if (K > T) K = T;
+    const int64_t C = (T + K - 1) / K;
     strh_ctx* c = L->ctx;
     hipStream_t st = c->stream;
 
     AdjWork W;
     double *jacc, *gA, *gU;
-    float *rec, *lam0, *lam1, *zeroAlpha = nullptr;
+    float4* rec;
+    float *lam0, *lam1, *zeroAlpha = nullptr;
     uint8_t *ck, *work;
     if ((rc = W.get(&jacc, N * 8, "objective accumulator"))) return rc;
     if ((rc = W.get(&gA, N * 8, "porosity gradient"))) return rc;
     if ((rc = W.get(&gU, N * 8, "inlet gradient"))) return rc;
     if ((rc = W.get(&lam0, 19 * S * 4, "adjoint state"))) return rc;
     if ((rc = W.get(&lam1, 19 * S * 4, "adjoint state"))) return rc;
-    if ((rc = W.get(&rec, (size_t)K * 19 * S * 4, "recorded states"))) return rc;
+    if ((rc = W.get(&rec, (size_t)K * N * 16, "recorded moments"))) return rc;
     if ((rc = W.get(&ck, (size_t)(C > 1 ? C - 1 : 1) * ddf, "checkpoints"))) return rc;
     if ((rc = W.get(&work, ddf, "replay DDFs"))) return rc;
     const float* alpha = L->alpha;
@@ -1210,7 +1221,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
             q.cin = win ? P.cin : 0.0;
             q.cout = win ? P.cout : 0.0;
             q.wd = win ? P.wd : 0.0;
-            HIP_TRY(launch_adjoint_step(q, L->walls, rec + (size_t)(k - k0) * 19 * S, lamN, lamT,
+            HIP_TRY(launch_adjoint_step(q, L->walls, rec + (size_t)(k - k0) * N, lamN, lamT,
                                         L->flags, L->link, alpha, gA, gU, st), "adjoint step launch");
             std::swap(lamN, lamT);
         }

@@ -295,10 +295,11 @@ class Simulation:
         return J.value
 
     def vjp(self, inlet_x: int, outlet_x: int, spinup: int, avg: int, w_p: float = 1.0,
-            w_d: float = 1.0, checkpoint_every: int = 0, want_alpha: bool = True):
+            w_d: float = 1.0, checkpoint_every: int = -1, want_alpha: bool = True):
         """Discrete adjoint of `objective`: returns (J, dJ/dalpha per cell as
         float64 (nz, ny, nx) or None, dJ/d uin_target, info dict). Leaves the
-        lattice as `run(spinup + avg)` would."""
+        lattice as `run(spinup + avg)` would. checkpoint_every: steps per
+        replayed segment; 0 = ceil(sqrt(T)), -1 = as long as memory allows."""
         from ._lib import VjpInfo
         o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d)
         J, gu, info = ctypes.c_double(), ctypes.c_double(), VjpInfo()

@@ -156,6 +156,23 @@ def test_adjoint_objective_terms_separately():
         assert ga[y * nx + x] == pytest.approx(fd, rel=1e-6)
 
 
+def test_adjoint_independent_of_checkpoint_interval():
+    """Replaying segments from checkpoints gives the same bits as keeping every
+    state, also with several threads, and leaves the lattice where run() does."""
+    make, _ = _adjoint_case(seed=2)
+    ref = make()
+    J0, g0, u0 = ref.vjp(**OBJ, checkpoint_every=17)
+    for k, threads in [(1, 1), (4, 3), (0, 4), (6, 2)]:
+        s = make()
+        s.set_threads(threads)
+        J, ga, gu = s.vjp(**OBJ, checkpoint_every=k)
+        assert J == J0 and gu == u0 and np.array_equal(ga, g0)
+        assert s.state_digest == ref.state_digest
+    plain = make()
+    plain.run(OBJ["spinup"] + OBJ["avg"])
+    assert plain.state_digest == ref.state_digest
+
+
 def test_adjoint_rejects_unsupported_features():
     make, _ = _adjoint_case()
     s = make()
