@@ -138,6 +138,12 @@ STRH_API int strh_mass_sum(strh_lattice* lat, double* out);
  * per-cell term in FP32 as Cases.swift:1242), reduced on the GPU in a fixed
  * order. Requires an even step count. */
 STRH_API int strh_kinetic_energy(strh_lattice* lat, double* out);
+/* Viscous dissipation rate sum_fluid 2 nu S:S (FP64, fixed order), nu =
+ * (1/omega - 1/2)/3, with the strain rate from the non-equilibrium momentum
+ * flux, S = -3 Pi_neq / (2 rho tau). No reference counterpart: the reference
+ * differentiates E(t) in time (Cases.swift:1254-1257). Even step count;
+ * molecular viscosity only (no eddy viscosity, no body-force correction). */
+STRH_API int strh_dissipation(strh_lattice* lat, double* out);
 
 /* ---- raw state (the self-test writes fBuf directly, Cases.swift:90-99) --- */
 STRH_API int strh_download_ddf(strh_lattice* lat, void* out, size_t bytes);

@@ -105,6 +105,9 @@ hipError_t launch_box_force_sum(const float* force, uint32_t nx, uint32_t ny, ui
                                 uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
                                 double* partials, uint32_t max_partials, double* out3,
                                 hipStream_t s);
+hipError_t launch_dissipation_sum(int precision, const void* f, const uint8_t* flags, uint32_t N,
+                                  uint32_t S, double tauP, double* partials, uint32_t max_partials,
+                                  double* out, hipStream_t s);
 hipError_t launch_kinetic_sum(const float* moments, uint32_t N, double* partials,
                               uint32_t max_partials, double* out, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);

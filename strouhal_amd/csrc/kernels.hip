@@ -950,6 +950,43 @@ struct KineticElem {
     }
 };
 
+// Viscous dissipation of one cell, 2 nu S:S, with the strain rate from the
+// non-equilibrium momentum flux of the natural-order state (Chapman-Enskog:
+// Pi_neq = -2 rho c_s^2 tau+ S). With shifted DDFs sum c_a c_b f~ = Pi - delta/3,
+// so Pi_neq = sum c_a c_b f~ - rho u_a u_b - (rho - 1)/3 delta_ab. FP64
+// throughout; non-fluid cells give 0. No body-force correction (the
+// dissipation observable is for force-free flows).
+template <class St> struct DissElem {
+    const typename St::T* f;
+    const uint8_t* flags;     // null: all fluid
+    uint32_t S;
+    double tauP, twoNu;
+    __device__ double operator()(uint32_t, uint32_t k) const {
+        if (flags && flags[k] != kFluid) return 0.0;
+        double rm1 = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+        double xx = 0.0, yy = 0.0, zz = 0.0, xy = 0.0, xz = 0.0, yz = 0.0;
+#pragma unroll
+        for (int i = 0; i < 19; i++) {
+            const double v = (double)St::ld(f[(size_t)i * S + k]);
+            rm1 += v;
+            if (kCx[i]) { jx += (double)kCx[i] * v; xx += v; }
+            if (kCy[i]) { jy += (double)kCy[i] * v; yy += v; }
+            if (kCz[i]) { jz += (double)kCz[i] * v; zz += v; }
+            if (kCx[i] * kCy[i]) xy += (double)(kCx[i] * kCy[i]) * v;
+            if (kCx[i] * kCz[i]) xz += (double)(kCx[i] * kCz[i]) * v;
+            if (kCy[i] * kCz[i]) yz += (double)(kCy[i] * kCz[i]) * v;
+        }
+        const double rho = 1.0 + rm1, ir = 1.0 / rho;
+        const double third = rm1 / 3.0;
+        const double pxx = xx - jx * jx * ir - third, pyy = yy - jy * jy * ir - third;
+        const double pzz = zz - jz * jz * ir - third;
+        const double pxy = xy - jx * jy * ir, pxz = xz - jx * jz * ir, pyz = yz - jy * jz * ir;
+        const double c = -1.5 / (rho * tauP);          // S = c Pi_neq
+        const double ss = pxx * pxx + pyy * pyy + pzz * pzz + 2.0 * (pxy * pxy + pxz * pxz + pyz * pyz);
+        return twoNu * c * c * ss;
+    }
+};
+
 __global__ void __launch_bounds__(256) fill_zero_kernel(uint32_t* __restrict__ p, uint64_t words) {
     const uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (k < words) p[k] = 0u;
@@ -1232,6 +1269,18 @@ struct SumElem {
 hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,
                               double* out, hipStream_t s) {
     return reduce(SumElem{v}, M, 1, partials, max_partials, out, s);
+}
+
+hipError_t launch_dissipation_sum(int precision, const void* f, const uint8_t* flags, uint32_t N,
+                                  uint32_t S, double tauP, double* partials, uint32_t max_partials,
+                                  double* out, hipStream_t s) {
+    const double twoNu = 2.0 * (tauP - 0.5) / 3.0;
+    switch (precision) {
+        case kFP32: return reduce(DissElem<StF32>{(const float*)f, flags, S, tauP, twoNu}, N, 1, partials, max_partials, out, s);
+        case kFP16S: return reduce(DissElem<StF16>{(const uint16_t*)f, flags, S, tauP, twoNu}, N, 1, partials, max_partials, out, s);
+        case kBF16: return reduce(DissElem<StBF16>{(const uint16_t*)f, flags, S, tauP, twoNu}, N, 1, partials, max_partials, out, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_kinetic_sum(const float* moments, uint32_t N, double* partials,

@@ -542,6 +542,19 @@ int strh_kinetic_energy(strh_lattice* L, double* out) {
     return sync(L, "kinetic energy");
 }
 
+int strh_dissipation(strh_lattice* L, double* out) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    if (L->stepsDone % 2 != 0) return fail(STRH_ERR_ODD_STEPS, "dissipation requires even step count");
+    const double tauP = 1.0 / (double)L->omega;
+    HIP_TRY(launch_dissipation_sum(L->precision, L->f, L->walls ? L->flags : nullptr, L->N, L->S, tauP,
+                                   L->partials, kMaxPartials, L->result, L->ctx->stream),
+            "dissipation reduction launch");
+    HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy dissipation");
+    return sync(L, "dissipation");
+}
+
 // Canonical image: 19 slots of N values, slot-major (Sim.swift:441-444 hashes
 // exactly these bytes). Device slots are padded to S; copy with a pitch.
 int strh_download_ddf(strh_lattice* L, void* out, size_t bytes) {

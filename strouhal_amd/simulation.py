@@ -363,6 +363,13 @@ class Simulation:
         check(lib().strh_kinetic_energy(self._h, ctypes.byref(v)))
         return v.value
 
+    def dissipation(self) -> float:
+        """FP64 sum over fluid cells of 2 nu S:S, S from the non-equilibrium
+        momentum flux (GPU fixed-order reduction; even step count)."""
+        v = ctypes.c_double()
+        check(lib().strh_dissipation(self._h, ctypes.byref(v)))
+        return v.value
+
     # ---- raw state -------------------------------------------------------------
     def download_ddf(self) -> np.ndarray:
         """Canonical DDF image, shape (19, cells), slot-major (i*N + n)."""

@@ -65,3 +65,26 @@ def make_pair(gpu, precision, nx, ny, nz, omega, omega_minus=None, flags=None, *
                   lid=g.lid_vel, uin=g.uin_target, ramp_steps=g.ramp_steps, force=g.force,
                   c_smago=g.c_smago, wants_forces=g.wants_forces, flags=g.flags, threads=4)
     return g, r
+
+
+def dissipation_sum(f: np.ndarray, omega: float, fluid=None) -> float:
+    """sum over fluid cells of 2 nu S:S from a natural-order shifted state
+    f (19, N), S = -3 Pi_neq / (2 rho tau), nu = (tau - 1/2)/3 (FP64)."""
+    f = np.asarray(f, dtype=np.float64)
+    tau = 1.0 / omega
+    rm1 = f.sum(axis=0)
+    rho = 1.0 + rm1
+    j = np.stack([(c[:, None] * f).sum(axis=0) for c in (CX, CY, CZ)])
+    cc = (CX, CY, CZ)
+    ss = np.zeros_like(rho)
+    for a in range(3):
+        for b in range(3):
+            p = (cc[a][:, None] * cc[b][:, None] * f).sum(axis=0) - j[a] * j[b] / rho
+            if a == b:
+                p = p - rm1 / 3.0
+            ss += p * p
+    s2 = ss * (1.5 / (rho * tau)) ** 2
+    e = 2.0 * (tau - 0.5) / 3.0 * s2
+    if fluid is not None:
+        e = e[np.asarray(fluid).reshape(-1)]
+    return float(e.sum())

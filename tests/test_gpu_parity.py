@@ -290,10 +290,17 @@ def test_moments_and_mass_and_energy(gpu, precision):
     ke_ref = float(((mg[:, 0] * mg[:, 0] + mg[:, 1] * mg[:, 1]) + mg[:, 2] * mg[:, 2])
                    .astype(np.float64).sum())
     assert abs(g.kinetic_energy() - ke_ref) <= 1e-13 * ke_ref
+    # viscous dissipation: GPU FP64 fixed-order sum vs the numpy FP64 sum of the
+    # same (bit-identical) state; walls excluded
+    from tests.helpers import dissipation_sum
+    d_ref = dissipation_sum(ddf, g.omega, fluid=(g.flags.reshape(-1) == 0))
+    assert d_ref > 0 and abs(g.dissipation() - d_ref) <= 1e-12 * d_ref
     g.run(1)
     from strouhal_amd import StrouhalError
     with pytest.raises(StrouhalError):
         g.probe_moments()
+    with pytest.raises(StrouhalError):
+        g.dissipation()
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)

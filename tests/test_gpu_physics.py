@@ -224,3 +224,37 @@ def test_tgv_256_energy_decay(gpu):
     assert abs(e32[0] - 0.125) < 1e-3
     assert np.all(np.diff(e32) < 0)
     assert np.max(np.abs(e16 - e32)) / e32[0] < 5e-3
+
+
+@pytest.mark.parametrize("precision,lam", [("fp32", 3.0 / 16.0), ("fp16s", None), ("bf16", None)])
+def test_tgv64_energy_and_dissipation_vs_fp64_oracle(gpu, precision, lam):
+    """Config 2's observables at 64^3 over one convective time, every 0.1 t*:
+    E* and the strain-rate eps* (both FP64 GPU reductions) against the FP64
+    oracle, with the error per precision stated in the assertion bounds."""
+    from oracle.oracle import OracleSim
+    from strouhal_amd import Simulation, trt_omegas
+    from strouhal_amd.tgv import tgv_setup
+    from tests.helpers import dissipation_sum
+    n, u0 = 64, 0.05
+    L, tau, tc, stride = tgv_setup(n, u0)
+    wp, wm = trt_omegas(tau, lam)
+    g = Simulation(gpu, precision, nx=n, ny=n, nz=n, omega=wp, omega_minus=wm)
+    r = OracleSim(n, n, n, g.omega, g.omega_minus, arith="f64", storage="f64", threads=8)
+    g.init_field(2, u0)
+    r.init_field(2, u0)
+    N = n ** 3
+    ne, nd = 1.0 / (2 * N * u0 * u0), L / (N * u0 ** 3)
+    err_e, err_d = [], []
+    for k in range(11):
+        if k:
+            g.run(stride)
+            r.run(stride)
+        m = r.probe_moments()
+        e_ref = float((m[:, :3] ** 2).sum()) * ne
+        d_ref = dissipation_sum(r.download_ddf().reshape(19, -1), g.omega) * nd
+        err_e.append(abs(g.kinetic_energy() * ne - e_ref) / e_ref)
+        err_d.append(abs(g.dissipation() * nd - d_ref) / d_ref)
+    print(f"TGV 64^3 {precision}: max rel err E {max(err_e):.2e}, eps {max(err_d):.2e}")
+    # measured on MI355X: fp32 1e-7 / 1.3e-7, fp16s 8e-6 / 6e-4, bf16 1.2e-3 / 3.4e-2
+    bound = {"fp32": (1e-6, 1e-5), "fp16s": (1e-4, 5e-3), "bf16": (1e-2, 0.1)}[precision]
+    assert max(err_e) < bound[0] and max(err_d) < bound[1]

@@ -140,3 +140,40 @@ def test_cycle_stats_on_a_sine():
     assert s["strouhal"] == pytest.approx(f * D / um, rel=1e-3)
     assert s["max_cl"] == pytest.approx(0.8, abs=2e-3)
     assert s["mean_cd"] == pytest.approx(1.5, abs=1e-3)
+
+
+def test_channel_design_field_fixture():
+    """The porous-channel design field is reproducible: its float32 bytes hash
+    to the committed SHA-256, 30% of the interior is porous, and the smoothing
+    kernel has unit mass."""
+    import json
+    from strouhal_amd.channel_adjoint import _gauss_kernel, design_field
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "channel_design_field.json")))
+    a = design_field(fx["nx"], fx["ny"], sigma=fx["sigma"], solid=fx["solid"], seed=fx["seed"],
+                     margin=fx["margin"])
+    assert hashlib.sha256(a.tobytes()).hexdigest() == fx["sha256_float32"]
+    assert int(a.sum()) == fx["porous_cells"]
+    inner = a[0, 1:fx["ny"] - 1, fx["margin"]:fx["nx"] - fx["margin"]]
+    assert abs(inner.mean() - fx["solid"]) < 1e-3
+    assert a[0, :, :fx["margin"]].max() == 0.0 and a[0, [0, -1], :].max() == 0.0
+    assert abs(_gauss_kernel(8.0).sum() - 1.0) < 1e-15
+
+
+def test_tgv_post_processing():
+    """Central differences of a quadratic are exact; the parabolic peak
+    refinement recovers the vertex of a sampled parabola; the DNS table's RMS
+    distance to itself is zero."""
+    import json
+    from strouhal_amd.tgv import central_eps, peak, rms_vs, tgv_setup
+    t = np.arange(0, 3.0001, 0.1)
+    e = 1.0 - 0.3 * t + 0.05 * t * t
+    tc, ec = central_eps(t, e)
+    assert np.allclose(ec, 0.3 - 0.1 * tc, atol=1e-12)
+    ts = np.arange(0, 12, 0.1)
+    pt, pe = peak(ts, 0.0128 - 0.002 * (ts - 8.97) ** 2, 0.1)
+    assert pt == pytest.approx(8.97, abs=1e-9) and pe == pytest.approx(0.0128, abs=1e-12)
+    dns = json.load(open(os.path.join(ROOT, "tests", "golden", "tgv_dns_re1600.json")))
+    assert rms_vs(dns["t_star"], dns["eps_star"], dns["t_star"], dns["eps_star"], 14.0) == 0.0
+    assert dns["eps_star"][0] == pytest.approx(3.0 / (4.0 * 1600.0), rel=1e-3)
+    L, tau, tc_, stride = tgv_setup(256)
+    assert stride == 80 and tau == pytest.approx(0.503820, abs=1e-6)

@@ -415,3 +415,19 @@ def test_cavity_re400_against_ghia():
     s.run(30000)
     m = s.probe_moments()
     assert ghia_rms(m, n, nx, float(np.float32(ulid)), "re400", u_only=True) <= 0.02
+
+
+def test_dissipation_of_the_taylor_green_initial_field():
+    """2 nu <S:S> of the 3-D TGV initial field is 3 nu U0^2 / (4 L^2), i.e.
+    eps* = 3 / (4 Re) = 4.6875e-4 at Re 1600 (the first entry of the DNS
+    table, Cases.swift:1184). initField writes f_neq from the analytic strain
+    rate, so the Chapman-Enskog estimate recovers it to O(A^2) (density)."""
+    from tests.helpers import dissipation_sum
+    n, u0, re = 32, 0.05, 1600.0
+    L = n / (2 * np.pi)
+    tau = 3 * u0 * L / re + 0.5
+    sim = OracleSim(n, n, n, 1.0 / tau, arith="f64", storage="f64", threads=4)
+    sim.init_field(2, u0)
+    f = sim.download_ddf().reshape(19, -1)
+    eps_star = dissipation_sum(f, 1.0 / tau) / n ** 3 * L / u0 ** 3
+    assert eps_star == pytest.approx(3.0 / (4.0 * re), rel=2e-3)
