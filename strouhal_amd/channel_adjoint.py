@@ -66,15 +66,16 @@ def design_field(nx: int, ny: int, sigma: float = 8.0, solid: float = 0.30, seed
 
 def channel_case(gpu=None, precision: str = "fp32", nx: int = 512, ny: int = 256, re: float = 50.0,
                  umax: float = 0.05, ramp_steps: int = 500, seed: int = 0, margin: int = 16,
-                 lam: float = 3.0 / 16.0):
+                 lam: float = 3.0 / 16.0, nz: int = 1):
+    """The channel lattice with its design field (extruded along z when nz > 1)."""
     from strouhal_amd import Simulation, trt_omegas
     u_mean = 2.0 / 3.0 * umax
     nu = u_mean * (ny - 2) / re
     tau = 3.0 * nu + 0.5
     wp, wm = trt_omegas(tau, lam)
-    sim = Simulation(gpu, precision, nx=nx, ny=ny, nz=1, omega=wp, omega_minus=wm, uin=umax,
-                     ramp_steps=ramp_steps, flags=channel_flags(nx, ny, 1))
-    alpha = design_field(nx, ny, seed=seed, margin=margin)
+    sim = Simulation(gpu, precision, nx=nx, ny=ny, nz=nz, omega=wp, omega_minus=wm, uin=umax,
+                     ramp_steps=ramp_steps, flags=channel_flags(nx, ny, nz))
+    alpha = np.repeat(design_field(nx, ny, seed=seed, margin=margin), nz, axis=0)
     sim.set_porosity(alpha)
     sim.init_field(0, 0.0)
     return sim, alpha
@@ -101,6 +102,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=1)
     ap.add_argument("--spinup", type=int, default=2000)
     ap.add_argument("--avg", type=int, default=4000)
     ap.add_argument("--precision", default="fp32")
@@ -117,7 +119,8 @@ def main():
     obj = dict(inlet_x=margin // 2, outlet_x=args.nx - 1 - margin // 2, spinup=args.spinup,
                avg=args.avg, w_p=args.w_p, w_d=args.w_d)
 
-    fwd, alpha = channel_case(gpu, args.precision, args.nx, args.ny, seed=args.seed, margin=margin)
+    fwd, alpha = channel_case(gpu, args.precision, args.nx, args.ny, seed=args.seed, margin=margin,
+                              nz=args.nz)
     fwd.run(2)                                     # warm the kernels
     f0 = fwd.gpu_seconds
     t0 = time.perf_counter()
@@ -125,12 +128,14 @@ def main():
     wall_f = time.perf_counter() - t0
     gpu_f = fwd.gpu_seconds - f0
 
-    ev, _ = channel_case(gpu, args.precision, args.nx, args.ny, seed=args.seed, margin=margin)
+    ev, _ = channel_case(gpu, args.precision, args.nx, args.ny, seed=args.seed, margin=margin,
+                              nz=args.nz)
     t0 = time.perf_counter()
     J_eval = ev.objective(**obj)
     wall_o = time.perf_counter() - t0
 
-    adj, _ = channel_case(gpu, args.precision, args.nx, args.ny, seed=args.seed, margin=margin)
+    adj, _ = channel_case(gpu, args.precision, args.nx, args.ny, seed=args.seed, margin=margin,
+                              nz=args.nz)
     t0 = time.perf_counter()
     J, ga, gu, info = adj.vjp(**obj, checkpoint_every=args.checkpoint_every)
     wall_v = time.perf_counter() - t0
@@ -144,7 +149,7 @@ def main():
     step_ms = gpu_f / (T - 2) * 1e3
     adj_s = info["forward_seconds"] + info["recompute_seconds"] + info["backward_seconds"]
     report = {
-        "workload": f"porous channel {args.nx}x{args.ny}x1 Re 50, Brinkman design field, "
+        "workload": f"porous channel {args.nx}x{args.ny}x{args.nz} Re 50, Brinkman design field, "
                     f"spin-up {args.spinup} + window {args.avg}",
         "precision": args.precision, "cells": n, "steps": T,
         "forward_ms_per_step": step_ms, "forward_mlups": n / (step_ms * 1e-3) / 1e6,

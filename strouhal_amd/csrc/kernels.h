@@ -70,12 +70,12 @@ struct AdjHooks {
 struct AdjStep {
     uint32_t nx, ny, nz, N, S, blocks;
     FastDiv divx, divy;
-    double wp, wm, fx, fy, fz;   // collision rates and body force of step t
+    float wp, wm, fx, fy, fz;    // collision rates and body force of step t
     double rOdd;                 // ramp of the odd step that streams after step t
     double uinTarget;
     int32_t inflowUniform;
     int32_t inletX, outletX;     // objective of step t (weights 0 outside the window)
-    double cin, cout, wd;
+    float cin, cout, wd;
 };
 
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);

@@ -358,8 +358,6 @@ struct Offsets {   // byte offsets to the wrapped neighbour in each axis directi
 // 2% of each other at 256^3; 256 is the best or tied).
 constexpr uint32_t kStepBS = 256;
 
-// One cell of one AA pass (the body shared by the single-lattice kernel and
-// the multi-lattice batch kernel).
 // Adjoint hooks of the forward pass (AdjHooks in kernels.h). The collision's
 // Jacobian depends on the state only through rho and p, so the reverse step
 // needs just these four moments of the pre-collision populations (summed in
@@ -394,6 +392,8 @@ __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[
     }
 }
 
+// One cell of one AA pass (the body shared by the single-lattice kernel and
+// the multi-lattice batch kernel).
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
           bool HOOK = false>
 __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
@@ -607,9 +607,10 @@ step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ f
 //   dF   = a+ (9 Mcc.J - 3 J M0)/rho + 3 a- Mc
 // and through J(p, alpha), F(J, alpha): dP = (dJ - alpha dF)/(1 + alpha/2),
 //   dJ/dalpha = -J/(2 gamma), dF/dalpha = -J (1 - alpha/(2 gamma)).
-// All adjoint arithmetic is FP64; lambda is stored as FP32.
-__device__ __forceinline__ double wgt_d(int i) {
-    return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+// The adjoint state and the collision's vector-Jacobian product are FP32 (the
+// precision of the recorded state); the gradients accumulate in FP64.
+template <int I> __device__ __forceinline__ float wgt_f() {
+    return I == 0 ? 1.0f / 3.0f : (I <= 6 ? 1.0f / 18.0f : 1.0f / 36.0f);
 }
 
 template <bool WALLS>
@@ -621,106 +622,121 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
     if (n >= q.N) return;
     if (flags[n] != kFluid) return;
-    const size_t S = q.S;
+    const uint32_t S = q.S;
     const uint32_t t = fdiv(n, q.divx);
     const int x = (int)(n - t * q.nx);
-    const uint32_t zz = fdiv(t, q.divy);
-    const int y = (int)(t - zz * q.ny), z = (int)zz;
+    const uint32_t z = fdiv(t, q.divy);
+    const int y = (int)(t - z * q.ny);
     const int nx = (int)q.nx, ny = (int)q.ny, nz = (int)q.nz;
+    const int plane = nx * ny;
+    Offsets o;   // element offsets to the wrapped neighbours
+    o.xm = (x == 0) ? nx - 1 : -1;
+    o.xp = (x == nx - 1) ? 1 - nx : 1;
+    o.ym = (y == 0) ? (ny - 1) * nx : -nx;
+    o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
+    o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
+    o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
     const uint32_t sM = WALLS ? (link[n] & 0x3ffffu) : 0u;
 
-    double mu[19];
+    // mu = P^T lambda^{t+1}
+    float mu[19];
     double gu = 0.0;
-    mu[0] = (double)lamN[n];
-#pragma unroll
-    for (int i = 1; i < 19; i++) {
-        int dx = x + kCx[i], dy = y + kCy[i], dz = z + kCz[i];
-        dx = dx < 0 ? nx - 1 : (dx >= nx ? 0 : dx);
-        dy = dy < 0 ? ny - 1 : (dy >= ny ? 0 : dy);
-        dz = dz < 0 ? nz - 1 : (dz >= nz ? 0 : dz);
-        const uint32_t d = ((uint32_t)dz * q.ny + (uint32_t)dy) * q.nx + (uint32_t)dx;
-        if (WALLS && ((sM >> (i - 1)) & 1u)) {
-            const double ln = (double)lamN[(size_t)kOpp[i] * S + n];
-            const uint8_t wf = flags[d];
-            mu[i] = wf == kOutflow ? -ln : ln;
-            if (wf == kInflow && kCx[i] != 0) {
-                double prof = 1.0;                 // wall_vel: parabolic profile clamped at 0
-                if (!q.inflowUniform) {
-                    const double H = (double)ny - 2.0, yd = (double)dy - 0.5;
-                    prof = 4.0 * yd * (H - yd) / (H * H);
-                    if (!(q.uinTarget * q.rOdd * prof > 0.0)) prof = 0.0;
-                }
-                gu += ln * (-6.0 * wgt_d(i) * (double)kCx[i] * q.rOdd * prof);
-            }
-        } else {
-            mu[i] = (double)lamN[(size_t)i * S + d];
-        }
+    mu[0] = lamN[n];
+#define STRH_GATHER(I)                                                                \
+    {                                                                                 \
+        const uint32_t d = (uint32_t)((int)n + o.at<kCx[I], kCy[I], kCz[I]>());       \
+        if (WALLS && ((sM >> (I - 1)) & 1u)) {                                        \
+            const float ln = lamN[kOpp[I] * S + n];                                   \
+            const uint8_t wf = flags[d];                                              \
+            mu[I] = wf == kOutflow ? -ln : ln;                                        \
+            if (kCx[I] != 0 && wf == kInflow) {                                       \
+                const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
+                double prof = 1.0;                                                    \
+                if (!q.inflowUniform) {                                               \
+                    const double H = (double)ny - 2.0, yd = (double)dy - 0.5;         \
+                    prof = 4.0 * yd * (H - yd) / (H * H);                             \
+                    if (!(q.uinTarget * q.rOdd * prof > 0.0)) prof = 0.0;             \
+                }                                                                     \
+                gu += (double)ln * (-6.0 * (double)wgt_f<I>() * (double)kCx[I] * q.rOdd * prof); \
+            }                                                                         \
+        } else {                                                                      \
+            mu[I] = lamN[I * S + d];                                                  \
+        }                                                                             \
     }
+    STRH_GATHER(1) STRH_GATHER(2) STRH_GATHER(3) STRH_GATHER(4) STRH_GATHER(5) STRH_GATHER(6)
+    STRH_GATHER(7) STRH_GATHER(8) STRH_GATHER(9) STRH_GATHER(10) STRH_GATHER(11) STRH_GATHER(12)
+    STRH_GATHER(13) STRH_GATHER(14) STRH_GATHER(15) STRH_GATHER(16) STRH_GATHER(17) STRH_GATHER(18)
+#undef STRH_GATHER
 
     const float4 m = rec[n];
-    const double a = alphaBuf ? (double)alphaBuf[n] : 0.0;
-    const double rho = 1.0 + (double)m.x, px = (double)m.y, py = (double)m.z, pz = (double)m.w;
-    const double gam = 1.0 + 0.5 * a;
-    const double Jx = (px + 0.5 * q.fx) / gam, Jy = (py + 0.5 * q.fy) / gam, Jz = (pz + 0.5 * q.fz) / gam;
-    const double Fx = q.fx - a * Jx, Fy = q.fy - a * Jy, Fz = q.fz - a * Jz;
-    double M0 = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0;
-    double Mxx = 0.0, Myy = 0.0, Mzz = 0.0, Mxy = 0.0, Mxz = 0.0, Myz = 0.0;
-#pragma unroll
-    for (int i = 0; i < 19; i++) {
-        const double v = wgt_d(i) * mu[i];
-        M0 += v;
-        if (kCx[i]) Mx += (double)kCx[i] * v;
-        if (kCy[i]) My += (double)kCy[i] * v;
-        if (kCz[i]) Mz += (double)kCz[i] * v;
-        if (kCx[i]) Mxx += v;
-        if (kCy[i]) Myy += v;
-        if (kCz[i]) Mzz += v;
-        if (kCx[i] * kCy[i]) Mxy += (double)(kCx[i] * kCy[i]) * v;
-        if (kCx[i] * kCz[i]) Mxz += (double)(kCx[i] * kCz[i]) * v;
-        if (kCy[i] * kCz[i]) Myz += (double)(kCy[i] * kCz[i]) * v;
+    const float a = alphaBuf ? alphaBuf[n] : 0.0f;
+    const float rho = 1.0f + m.x, ir = 1.0f / rho, ir2 = ir * ir;
+    const float gam = 1.0f + 0.5f * a, ig = 1.0f / gam;
+    const float fx = q.fx, fy = q.fy, fz = q.fz;
+    const float Jx = (m.y + 0.5f * fx) * ig, Jy = (m.z + 0.5f * fy) * ig, Jz = (m.w + 0.5f * fz) * ig;
+    const float Fx = fx - a * Jx, Fy = fy - a * Jy, Fz = fz - a * Jz;
+    float M0 = 0.0f, Mx = 0.0f, My = 0.0f, Mz = 0.0f;
+    float Mxx = 0.0f, Myy = 0.0f, Mzz = 0.0f, Mxy = 0.0f, Mxz = 0.0f, Myz = 0.0f;
+#define STRH_MSUM(I)                                                                  \
+    {                                                                                 \
+        const float v = wgt_f<I>() * mu[I];                                           \
+        M0 += v;                                                                      \
+        if (kCx[I]) { Mx += (float)kCx[I] * v; Mxx += v; }                            \
+        if (kCy[I]) { My += (float)kCy[I] * v; Myy += v; }                            \
+        if (kCz[I]) { Mz += (float)kCz[I] * v; Mzz += v; }                            \
+        if (kCx[I] * kCy[I]) Mxy += (float)(kCx[I] * kCy[I]) * v;                     \
+        if (kCx[I] * kCz[I]) Mxz += (float)(kCx[I] * kCz[I]) * v;                     \
+        if (kCy[I] * kCz[I]) Myz += (float)(kCy[I] * kCz[I]) * v;                     \
     }
-    const double MJx = Mxx * Jx + Mxy * Jy + Mxz * Jz;
-    const double MJy = Mxy * Jx + Myy * Jy + Myz * Jz;
-    const double MJz = Mxz * Jx + Myz * Jy + Mzz * Jz;
-    const double MFx = Mxx * Fx + Mxy * Fy + Mxz * Fz;
-    const double MFy = Mxy * Fx + Myy * Fy + Myz * Fz;
-    const double MFz = Mxz * Fx + Myz * Fy + Mzz * Fz;
-    const double JMJ = Jx * MJx + Jy * MJy + Jz * MJz;
-    const double JMF = Jx * MFx + Jy * MFy + Jz * MFz;
-    const double J2 = Jx * Jx + Jy * Jy + Jz * Jz;
-    const double JF = Jx * Fx + Jy * Fy + Jz * Fz;
-    const double wp = q.wp, wm = q.wm;
-    const double ap = 1.0 - 0.5 * wp, am = 1.0 - 0.5 * wm;
-    const double ir = 1.0 / rho, ir2 = ir * ir;
-    const double gRho = wp * (M0 - 4.5 * JMJ * ir2 + 1.5 * J2 * M0 * ir2)
-                      - ap * (9.0 * JMF - 3.0 * JF * M0) * ir2;
-    const double gJx = wp * (9.0 * MJx - 3.0 * Jx * M0) * ir + 3.0 * wm * Mx + ap * (9.0 * MFx - 3.0 * Fx * M0) * ir;
-    const double gJy = wp * (9.0 * MJy - 3.0 * Jy * M0) * ir + 3.0 * wm * My + ap * (9.0 * MFy - 3.0 * Fy * M0) * ir;
-    const double gJz = wp * (9.0 * MJz - 3.0 * Jz * M0) * ir + 3.0 * wm * Mz + ap * (9.0 * MFz - 3.0 * Fz * M0) * ir;
-    const double gFx = ap * (9.0 * MJx - 3.0 * Jx * M0) * ir + 3.0 * am * Mx;
-    const double gFy = ap * (9.0 * MJy - 3.0 * Jy * M0) * ir + 3.0 * am * My;
-    const double gFz = ap * (9.0 * MJz - 3.0 * Jz * M0) * ir + 3.0 * am * Mz;
-    const double gPx = (gJx - a * gFx) / gam, gPy = (gJy - a * gFy) / gam, gPz = (gJz - a * gFz) / gam;
+    STRH_MSUM(0) STRH_MSUM(1) STRH_MSUM(2) STRH_MSUM(3) STRH_MSUM(4) STRH_MSUM(5) STRH_MSUM(6)
+    STRH_MSUM(7) STRH_MSUM(8) STRH_MSUM(9) STRH_MSUM(10) STRH_MSUM(11) STRH_MSUM(12)
+    STRH_MSUM(13) STRH_MSUM(14) STRH_MSUM(15) STRH_MSUM(16) STRH_MSUM(17) STRH_MSUM(18)
+#undef STRH_MSUM
+    const float MJx = Mxx * Jx + Mxy * Jy + Mxz * Jz;
+    const float MJy = Mxy * Jx + Myy * Jy + Myz * Jz;
+    const float MJz = Mxz * Jx + Myz * Jy + Mzz * Jz;
+    const float MFx = Mxx * Fx + Mxy * Fy + Mxz * Fz;
+    const float MFy = Mxy * Fx + Myy * Fy + Myz * Fz;
+    const float MFz = Mxz * Fx + Myz * Fy + Mzz * Fz;
+    const float JMJ = Jx * MJx + Jy * MJy + Jz * MJz;
+    const float JMF = Jx * MFx + Jy * MFy + Jz * MFz;
+    const float J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+    const float JF = Jx * Fx + Jy * Fy + Jz * Fz;
+    const float wp = q.wp, wm = q.wm;
+    const float ap = 1.0f - 0.5f * wp, am = 1.0f - 0.5f * wm;
+    const float gRho = wp * (M0 - 4.5f * JMJ * ir2 + 1.5f * J2 * M0 * ir2)
+                     - ap * (9.0f * JMF - 3.0f * JF * M0) * ir2;
+    const float gJx = wp * (9.0f * MJx - 3.0f * Jx * M0) * ir + 3.0f * wm * Mx + ap * (9.0f * MFx - 3.0f * Fx * M0) * ir;
+    const float gJy = wp * (9.0f * MJy - 3.0f * Jy * M0) * ir + 3.0f * wm * My + ap * (9.0f * MFy - 3.0f * Fy * M0) * ir;
+    const float gJz = wp * (9.0f * MJz - 3.0f * Jz * M0) * ir + 3.0f * wm * Mz + ap * (9.0f * MFz - 3.0f * Fz * M0) * ir;
+    const float gFx = ap * (9.0f * MJx - 3.0f * Jx * M0) * ir + 3.0f * am * Mx;
+    const float gFy = ap * (9.0f * MJy - 3.0f * Jy * M0) * ir + 3.0f * am * My;
+    const float gFz = ap * (9.0f * MJz - 3.0f * Jz * M0) * ir + 3.0f * am * Mz;
+    const float gPx = (gJx - a * gFx) * ig, gPy = (gJy - a * gFy) * ig, gPz = (gJz - a * gFz) * ig;
     // objective of step t: cp rho + wd alpha J_x
-    const double cp = (x == q.inletX ? q.cin : 0.0) - (x == q.outletX ? q.cout : 0.0);
-    const double dPx = gPx + q.wd * a / gam;
-    const double base = gRho + cp;
-    lam[n] = (float)(mu[0] - wp * mu[0] + base);
-#pragma unroll
-    for (int j = 1; j < 19; j++) {
-        const int o = kOpp[j];
-        const double mp = 0.5 * (mu[j] + mu[o]);
-        const double mm = 0.5 * (mu[j] - mu[o]);
-        double v = mu[j] - wp * mp - wm * mm + base;
-        if (kCx[j]) v += (double)kCx[j] * dPx;
-        if (kCy[j]) v += (double)kCy[j] * gPy;
-        if (kCz[j]) v += (double)kCz[j] * gPz;
-        lam[(size_t)j * S + n] = (float)v;
+    const float cp = (x == q.inletX ? q.cin : 0.0f) - (x == q.outletX ? q.cout : 0.0f);
+    const float wd = q.wd;
+    const float dPx = gPx + wd * a * ig;
+    const float base = gRho + cp;
+    lam[n] = mu[0] - wp * mu[0] + base;
+#define STRH_LAM(I)                                                                   \
+    {                                                                                 \
+        const float mp = 0.5f * (mu[I] + mu[kOpp[I]]);                                \
+        const float mm = 0.5f * (mu[I] - mu[kOpp[I]]);                                \
+        float v = mu[I] - wp * mp - wm * mm + base;                                   \
+        if (kCx[I]) v += (float)kCx[I] * dPx;                                         \
+        if (kCy[I]) v += (float)kCy[I] * gPy;                                         \
+        if (kCz[I]) v += (float)kCz[I] * gPz;                                         \
+        lam[I * S + n] = v;                                                           \
     }
-    const double gJdotJ = gJx * Jx + gJy * Jy + gJz * Jz;
-    const double gFdotJ = gFx * Jx + gFy * Jy + gFz * Jz;
-    const double k = 1.0 - a / (2.0 * gam);
-    gAlpha[n] += -gJdotJ / (2.0 * gam) - gFdotJ * k + q.wd * Jx * k;
+    STRH_LAM(1) STRH_LAM(2) STRH_LAM(3) STRH_LAM(4) STRH_LAM(5) STRH_LAM(6)
+    STRH_LAM(7) STRH_LAM(8) STRH_LAM(9) STRH_LAM(10) STRH_LAM(11) STRH_LAM(12)
+    STRH_LAM(13) STRH_LAM(14) STRH_LAM(15) STRH_LAM(16) STRH_LAM(17) STRH_LAM(18)
+#undef STRH_LAM
+    const float gJdotJ = gJx * Jx + gJy * Jy + gJz * Jz;
+    const float gFdotJ = gFx * Jx + gFy * Jy + gFz * Jz;
+    const float k = 1.0f - 0.5f * a * ig;
+    gAlpha[n] += (double)(-0.5f * gJdotJ * ig - gFdotJ * k + wd * Jx * k);
     if (gu != 0.0) gUin[n] += gu;
 }
 

@@ -1231,9 +1231,9 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
             q.inflowUniform = L->inflowUniform ? 1 : 0;
             const bool win = k >= o->spinup;
             q.inletX = o->inlet_x; q.outletX = o->outlet_x;
-            q.cin = win ? P.cin : 0.0;
-            q.cout = win ? P.cout : 0.0;
-            q.wd = win ? P.wd : 0.0;
+            q.cin = win ? (float)P.cin : 0.0f;
+            q.cout = win ? (float)P.cout : 0.0f;
+            q.wd = win ? (float)P.wd : 0.0f;
             HIP_TRY(launch_adjoint_step(q, L->walls, rec + (size_t)(k - k0) * N, lamN, lamT,
                                         L->flags, L->link, alpha, gA, gU, st), "adjoint step launch");
             std::swap(lamN, lamT);
