@@ -96,17 +96,17 @@ void box_reduce_plan(uint32_t M, uint32_t max_partials, uint32_t* Pb, uint32_t* 
 // Workgroup size of the LDS row-staged odd pass for this lattice, or 0.
 uint32_t rows_block_size(uint32_t nx, uint64_t N);
 hipError_t launch_moments(int precision, const void* f, const uint8_t* flags, float* out,
-                          uint32_t N, uint32_t S, float fx, float fy, float fz, hipStream_t s);
+                          uint32_t N, DdfLayout lay, float fx, float fy, float fz, hipStream_t s);
 hipError_t launch_init(int precision, void* f, uint32_t nx, uint32_t ny, uint32_t nz,
-                       uint32_t S, uint32_t mode, float amplitude, float tau, hipStream_t s);
-hipError_t launch_mass_sum(int precision, const void* f, uint32_t N, uint32_t S,
+                       DdfLayout lay, uint32_t mode, float amplitude, float tau, hipStream_t s);
+hipError_t launch_mass_sum(int precision, const void* f, uint32_t N, DdfLayout lay,
                            double* partials, uint32_t max_partials, double* out, hipStream_t s);
 hipError_t launch_box_force_sum(const float* force, uint32_t nx, uint32_t ny, uint32_t nz,
                                 uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
                                 double* partials, uint32_t max_partials, double* out3,
                                 hipStream_t s);
 hipError_t launch_dissipation_sum(int precision, const void* f, const uint8_t* flags, uint32_t N,
-                                  uint32_t S, double tauP, double* partials, uint32_t max_partials,
+                                  DdfLayout lay, double tauP, double* partials, uint32_t max_partials,
                                   double* out, hipStream_t s);
 hipError_t launch_kinetic_sum(const float* moments, uint32_t N, double* partials,
                               uint32_t max_partials, double* out, hipStream_t s);

@@ -95,6 +95,11 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv d) {
     return (__umulhi(n, d.m) + n) >> d.s;
 }
 
+// Element offset of slot 0 of cell n (DdfLayout in lbm.h); slot i adds i * W.
+__device__ __forceinline__ uint32_t cell_elem(const DdfLayout& l, uint32_t n) {
+    return (n >> l.lw) * l.bstride + (n & l.mask);
+}
+
 // Workgroup b of G runs on XCD b % 8. Give each XCD a contiguous range of
 // logical blocks so neighbouring cell ranges share an L2.
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
@@ -347,7 +352,7 @@ __device__ __forceinline__ void collide(float (&fh)[19], const StepParams& p, in
 }
 
 // ---- the fused step: Kernels.metal:230-355 ------------------------------
-struct Offsets {   // byte offsets to the wrapped neighbour in each axis direction
+struct Offsets {   // cell-index offsets to the wrapped neighbour in each axis direction
     int xm, xp, ym, yp, zm, zp;
     template <int CX, int CY, int CZ> __device__ __forceinline__ int at() const {
         return pick<CX>(xm, xp) + pick<CY>(ym, yp) + pick<CZ>(zm, zp);
@@ -406,12 +411,11 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                                           const float* __restrict__ alphaBuf = nullptr,
                                           const AdjHooks* hk = nullptr) {
     using T = typename St::T;
-    const size_t S = p.S;
     T* fs[19];
 #pragma unroll
-    for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * S;
+    for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * p.lay.W;
     constexpr int TB = (int)sizeof(T);
-    const uint32_t nb = n * (uint32_t)TB;     // byte offset of cell n within a slot
+    const uint32_t nb = cell_elem(p.lay, n) * (uint32_t)TB;   // byte offset of cell n from the slot bases
 
     float eps = 0.0f;
     float fh[19];
@@ -455,13 +459,15 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int plane = nx * ny;
     Offsets o;
-    o.xm = ((x == 0) ? nx - 1 : -1) * TB;
-    o.xp = ((x == nx - 1) ? 1 - nx : 1) * TB;
-    o.ym = ((y == 0) ? (ny - 1) * nx : -nx) * TB;
-    o.yp = ((y == ny - 1) ? -(ny - 1) * nx : nx) * TB;
-    o.zm = (((int)z == 0) ? (nz - 1) * plane : -plane) * TB;
-    o.zp = (((int)z == nz - 1) ? -(nz - 1) * plane : plane) * TB;
+    o.xm = (x == 0) ? nx - 1 : -1;
+    o.xp = (x == nx - 1) ? 1 - nx : 1;
+    o.ym = (y == 0) ? (ny - 1) * nx : -nx;
+    o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
+    o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
+    o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
     const int ni = (int)n;
+    // byte offset of the neighbour n + d from the slot bases
+    auto nbo = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(ni + d)) * (uint32_t)TB; };
 
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     uint32_t deferred = 0u;
@@ -478,7 +484,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                 az = acc<-kCz[I]>(az, fh[I]);                                         \
             }                                                                         \
         } else {                                                                      \
-            fh[I] = St::rd(fs[kOpp[I]], nb + (uint32_t)o.at<-kCx[I], -kCy[I], -kCz[I]>()); \
+            fh[I] = St::rd(fs[kOpp[I]], nbo(o.at<-kCx[I], -kCy[I], -kCz[I]>()));        \
         }                                                                             \
     }
     STRH_PULL(1) STRH_PULL(2) STRH_PULL(3) STRH_PULL(4) STRH_PULL(5) STRH_PULL(6)
@@ -490,7 +496,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 #define STRH_DEFER(I)                                                                 \
         if ((deferred >> I) & 1u) {                                                   \
             const int sy = y - kCy[I] < 0 ? ny - 1 : (y - kCy[I] >= ny ? 0 : y - kCy[I]); \
-            const uint32_t s = (uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>() / TB); \
+            const uint32_t s = (uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>());      \
             const uint8_t wf = flags[s];                                              \
             if (wf == kOutflow) {                                                     \
                 fh[I] = -fh[I];                                                       \
@@ -523,6 +529,11 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     }
     uint32_t nbs = nb;                                             // 317-350
     asm("" : "+v"(nbs));
+    // The push recomputes the neighbour offsets from a pinned copy of n
+    // rather than keeping the pull's 18 offsets live across the collision.
+    int nip = ni;
+    asm("" : "+v"(nip));
+    auto nbp = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB; };
     St::wr(fs[0], nbs, fh[0]);
 #define STRH_PUSH(I)                                                                  \
     {                                                                                 \
@@ -532,7 +543,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
             float sign = 1.0f;                                                        \
             if ((lM >> dstBit) & 1u) {                                                \
                 const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
-                const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>() / TB); \
+                const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>());     \
                 const uint8_t wf = flags[d];                                          \
                 if (wf == kOutflow) {                                                 \
                     sign = -1.0f;                                                     \
@@ -551,7 +562,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                 az = acc<kCz[I]>(az, v);                                              \
             }                                                                         \
         } else {                                                                      \
-            St::wr(fs[I], nbs + (uint32_t)o.at<kCx[I], kCy[I], kCz[I]>(), fh[I]);     \
+            St::wr(fs[I], nbp(o.at<kCx[I], kCy[I], kCz[I]>()), fh[I]);                \
         }                                                                             \
     }
     STRH_PUSH(1) STRH_PUSH(2) STRH_PUSH(3) STRH_PUSH(4) STRH_PUSH(5) STRH_PUSH(6)
@@ -758,10 +769,9 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
     __shared__ T row[10][BS];
     const uint32_t t = threadIdx.x;
     const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * BS + t;   // N % BS == 0
-    const size_t S = p.S;
     T* fs[19];
 #pragma unroll
-    for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * S;
+    for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * p.lay.W;
     constexpr int TB = (int)sizeof(T);
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t xmask = (uint32_t)nx - 1u;
@@ -773,17 +783,19 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
     const int plane = nx * ny;
     Offsets o;
     o.xm = 0; o.xp = 0;
-    o.ym = ((y == 0) ? (ny - 1) * nx : -nx) * TB;
-    o.yp = ((y == ny - 1) ? -(ny - 1) * nx : nx) * TB;
-    o.zm = (((int)z == 0) ? (nz - 1) * plane : -plane) * TB;
-    o.zp = (((int)z == nz - 1) ? -(nz - 1) * plane : plane) * TB;
-    const uint32_t nb = n * (uint32_t)TB;
+    o.ym = (y == 0) ? (ny - 1) * nx : -nx;
+    o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
+    o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
+    o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
+    const uint32_t nb = cell_elem(p.lay, n) * (uint32_t)TB;
+    // byte offset of the cell at n + d (same x, neighbouring row) from the slot bases
+    auto nbo = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)((int)n + d)) * (uint32_t)TB; };
 
     float fh[19];
     fh[0] = St::rd(fs[0], nb);
 #define STRH_RPULL(I)                                                                        \
-    if (kCx[I] == 0) fh[I] = St::rd(fs[kOpp[I]], nb + (uint32_t)o.at<0, -kCy[I], -kCz[I]>()); \
-    else row[xslot<I>()][t] = St::rd_raw(fs[kOpp[I]], nb + (uint32_t)o.at<0, -kCy[I], -kCz[I]>());
+    if (kCx[I] == 0) fh[I] = St::rd(fs[kOpp[I]], nbo(o.at<0, -kCy[I], -kCz[I]>()));          \
+    else row[xslot<I>()][t] = St::rd_raw(fs[kOpp[I]], nbo(o.at<0, -kCy[I], -kCz[I]>()));
     STRH_RPULL(1) STRH_RPULL(2) STRH_RPULL(3) STRH_RPULL(4) STRH_RPULL(5) STRH_RPULL(6)
     STRH_RPULL(7) STRH_RPULL(8) STRH_RPULL(9) STRH_RPULL(10) STRH_RPULL(11) STRH_RPULL(12)
     STRH_RPULL(13) STRH_RPULL(14) STRH_RPULL(15) STRH_RPULL(16) STRH_RPULL(17) STRH_RPULL(18)
@@ -803,9 +815,12 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
     __syncthreads();   // every shifted read of `row` is done
     uint32_t nbs = nb;
     asm("" : "+v"(nbs));
+    int nip = (int)n;   // pinned: recompute the row offsets instead of keeping them live
+    asm("" : "+v"(nip));
+    auto nbp = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB; };
     St::wr(fs[0], nbs, fh[0]);
 #define STRH_RPUSH(I)                                                                        \
-    if (kCx[I] == 0) St::wr(fs[I], nbs + (uint32_t)o.at<0, kCy[I], kCz[I]>(), fh[I]);       \
+    if (kCx[I] == 0) St::wr(fs[I], nbp(o.at<0, kCy[I], kCz[I]>()), fh[I]);                  \
     else row[xslot<I>()][rowbase + ((x + (uint32_t)kCx[I]) & xmask)] = St::st(fh[I]);
     STRH_RPUSH(1) STRH_RPUSH(2) STRH_RPUSH(3) STRH_RPUSH(4) STRH_RPUSH(5) STRH_RPUSH(6)
     STRH_RPUSH(7) STRH_RPUSH(8) STRH_RPUSH(9) STRH_RPUSH(10) STRH_RPUSH(11) STRH_RPUSH(12)
@@ -813,7 +828,7 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
 #undef STRH_RPUSH
     __syncthreads();
 #define STRH_RSTORE(I) \
-    if (kCx[I] != 0) St::wr_raw(fs[I], nbs + (uint32_t)o.at<0, kCy[I], kCz[I]>(), row[xslot<I>()][t]);
+    if (kCx[I] != 0) St::wr_raw(fs[I], nbp(o.at<0, kCy[I], kCz[I]>()), row[xslot<I>()][t]);
     STRH_RSTORE(1) STRH_RSTORE(2) STRH_RSTORE(7) STRH_RSTORE(8) STRH_RSTORE(9) STRH_RSTORE(10)
     STRH_RSTORE(11) STRH_RSTORE(12) STRH_RSTORE(13) STRH_RSTORE(14)
 #undef STRH_RSTORE
@@ -825,14 +840,15 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
 template <class St>
 __global__ void __launch_bounds__(256)
 moments_kernel(const typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
-               float4* __restrict__ out, uint32_t N, uint32_t S, float fx, float fy, float fz) {
+               float4* __restrict__ out, uint32_t N, DdfLayout lay, float fx, float fy, float fz) {
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= N) return;
     if (flags[n] != kFluid) { out[n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); return; }
+    const size_t e = cell_elem(lay, n);
     float rhom1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
 #pragma unroll
     for (int i = 0; i < 19; i++) {
-        const float v = St::ld(f[(size_t)i * S + n]);   // plain load: probe path
+        const float v = St::ld(f[e + (size_t)i * lay.W]);   // plain load: probe path
         rhom1 += v;
         px += (float)kCx[i] * v;
         py += (float)kCy[i] * v;
@@ -847,9 +863,10 @@ moments_kernel(const typename St::T* __restrict__ f, const uint8_t* __restrict__
 template <class St>
 __global__ void __launch_bounds__(256)
 init_kernel(typename St::T* __restrict__ f, uint32_t nx, uint32_t ny, uint32_t nz,
-            uint32_t N, uint32_t S, uint32_t mode, float amplitude, float tau) {
+            uint32_t N, DdfLayout lay, uint32_t mode, float amplitude, float tau) {
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= N) return;
+    const size_t e = cell_elem(lay, n);
     const float PI = 3.14159265358979323846f;
     float ux = 0.0f, uy = 0.0f, uz = 0.0f;
     float sxx = 0.0f, rhom1 = 0.0f, sxz = 0.0f, syz = 0.0f;
@@ -890,7 +907,7 @@ init_kernel(typename St::T* __restrict__ f, uint32_t nx, uint32_t ny, uint32_t n
         const float scc = sxx * cc + 2.0f * sxz * (float)(kCx[i] * kCz[i])
                         + 2.0f * syz * (float)(kCy[i] * kCz[i]);
         const float fneq = -3.0f * W * tau * scc;
-        f[(size_t)i * S + n] = St::st(feq + fneq);
+        f[e + (size_t)i * lay.W] = St::st(feq + fneq);
     }
 }
 
@@ -937,9 +954,9 @@ reduce_stage2(const double* __restrict__ partials, uint32_t P, double* __restric
 
 template <class St> struct MassElem {
     const typename St::T* f;
-    uint32_t S;
+    DdfLayout lay;
     __device__ double operator()(uint32_t slot, uint32_t n) const {
-        return (double)St::ld(f[(size_t)slot * S + n]);
+        return (double)St::ld(f[cell_elem(lay, n) + (size_t)slot * lay.W]);
     }
 };
 
@@ -975,15 +992,16 @@ struct KineticElem {
 template <class St> struct DissElem {
     const typename St::T* f;
     const uint8_t* flags;     // null: all fluid
-    uint32_t S;
+    DdfLayout lay;
     double tauP, twoNu;
     __device__ double operator()(uint32_t, uint32_t k) const {
         if (flags && flags[k] != kFluid) return 0.0;
+        const size_t e = cell_elem(lay, k);
         double rm1 = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
         double xx = 0.0, yy = 0.0, zz = 0.0, xy = 0.0, xz = 0.0, yz = 0.0;
 #pragma unroll
         for (int i = 0; i < 19; i++) {
-            const double v = (double)St::ld(f[(size_t)i * S + k]);
+            const double v = (double)St::ld(f[e + (size_t)i * lay.W]);
             rm1 += v;
             if (kCx[i]) { jx += (double)kCx[i] * v; xx += v; }
             if (kCy[i]) { jy += (double)kCy[i] * v; yy += v; }
@@ -1217,25 +1235,25 @@ uint32_t rows_block_size(uint32_t nx, uint64_t N) {
 static inline uint32_t grid_for(uint64_t n) { return (uint32_t)((n + 255u) / 256u); }
 
 hipError_t launch_moments(int precision, const void* f, const uint8_t* flags, float* out,
-                          uint32_t N, uint32_t S, float fx, float fy, float fz, hipStream_t s) {
+                          uint32_t N, DdfLayout lay, float fx, float fy, float fz, hipStream_t s) {
     const dim3 g(grid_for(N)), b(256);
     switch (precision) {
-        case kFP32: hipLaunchKernelGGL(moments_kernel<StF32>, g, b, 0, s, (const float*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
-        case kFP16S: hipLaunchKernelGGL(moments_kernel<StF16>, g, b, 0, s, (const uint16_t*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
-        case kBF16: hipLaunchKernelGGL(moments_kernel<StBF16>, g, b, 0, s, (const uint16_t*)f, flags, (float4*)out, N, S, fx, fy, fz); break;
+        case kFP32: hipLaunchKernelGGL(moments_kernel<StF32>, g, b, 0, s, (const float*)f, flags, (float4*)out, N, lay, fx, fy, fz); break;
+        case kFP16S: hipLaunchKernelGGL(moments_kernel<StF16>, g, b, 0, s, (const uint16_t*)f, flags, (float4*)out, N, lay, fx, fy, fz); break;
+        case kBF16: hipLaunchKernelGGL(moments_kernel<StBF16>, g, b, 0, s, (const uint16_t*)f, flags, (float4*)out, N, lay, fx, fy, fz); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_init(int precision, void* f, uint32_t nx, uint32_t ny, uint32_t nz,
-                       uint32_t S, uint32_t mode, float amplitude, float tau, hipStream_t s) {
+                       DdfLayout lay, uint32_t mode, float amplitude, float tau, hipStream_t s) {
     const uint32_t N = nx * ny * nz;
     const dim3 g(grid_for(N)), b(256);
     switch (precision) {
-        case kFP32: hipLaunchKernelGGL(init_kernel<StF32>, g, b, 0, s, (float*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
-        case kFP16S: hipLaunchKernelGGL(init_kernel<StF16>, g, b, 0, s, (uint16_t*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
-        case kBF16: hipLaunchKernelGGL(init_kernel<StBF16>, g, b, 0, s, (uint16_t*)f, nx, ny, nz, N, S, mode, amplitude, tau); break;
+        case kFP32: hipLaunchKernelGGL(init_kernel<StF32>, g, b, 0, s, (float*)f, nx, ny, nz, N, lay, mode, amplitude, tau); break;
+        case kFP16S: hipLaunchKernelGGL(init_kernel<StF16>, g, b, 0, s, (uint16_t*)f, nx, ny, nz, N, lay, mode, amplitude, tau); break;
+        case kBF16: hipLaunchKernelGGL(init_kernel<StBF16>, g, b, 0, s, (uint16_t*)f, nx, ny, nz, N, lay, mode, amplitude, tau); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1253,12 +1271,12 @@ static hipError_t reduce(const Elem& e, uint32_t M, uint32_t segs, double* parti
     return hipGetLastError();
 }
 
-hipError_t launch_mass_sum(int precision, const void* f, uint32_t N, uint32_t S,
+hipError_t launch_mass_sum(int precision, const void* f, uint32_t N, DdfLayout lay,
                            double* partials, uint32_t max_partials, double* out, hipStream_t s) {
     switch (precision) {
-        case kFP32: return reduce(MassElem<StF32>{(const float*)f, S}, N, 19, partials, max_partials, out, s);
-        case kFP16S: return reduce(MassElem<StF16>{(const uint16_t*)f, S}, N, 19, partials, max_partials, out, s);
-        case kBF16: return reduce(MassElem<StBF16>{(const uint16_t*)f, S}, N, 19, partials, max_partials, out, s);
+        case kFP32: return reduce(MassElem<StF32>{(const float*)f, lay}, N, 19, partials, max_partials, out, s);
+        case kFP16S: return reduce(MassElem<StF16>{(const uint16_t*)f, lay}, N, 19, partials, max_partials, out, s);
+        case kBF16: return reduce(MassElem<StBF16>{(const uint16_t*)f, lay}, N, 19, partials, max_partials, out, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1288,13 +1306,13 @@ hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uin
 }
 
 hipError_t launch_dissipation_sum(int precision, const void* f, const uint8_t* flags, uint32_t N,
-                                  uint32_t S, double tauP, double* partials, uint32_t max_partials,
+                                  DdfLayout lay, double tauP, double* partials, uint32_t max_partials,
                                   double* out, hipStream_t s) {
     const double twoNu = 2.0 * (tauP - 0.5) / 3.0;
     switch (precision) {
-        case kFP32: return reduce(DissElem<StF32>{(const float*)f, flags, S, tauP, twoNu}, N, 1, partials, max_partials, out, s);
-        case kFP16S: return reduce(DissElem<StF16>{(const uint16_t*)f, flags, S, tauP, twoNu}, N, 1, partials, max_partials, out, s);
-        case kBF16: return reduce(DissElem<StBF16>{(const uint16_t*)f, flags, S, tauP, twoNu}, N, 1, partials, max_partials, out, s);
+        case kFP32: return reduce(DissElem<StF32>{(const float*)f, flags, lay, tauP, twoNu}, N, 1, partials, max_partials, out, s);
+        case kFP16S: return reduce(DissElem<StF16>{(const uint16_t*)f, flags, lay, tauP, twoNu}, N, 1, partials, max_partials, out, s);
+        case kBF16: return reduce(DissElem<StBF16>{(const uint16_t*)f, flags, lay, tauP, twoNu}, N, 1, partials, max_partials, out, s);
     }
     return hipErrorInvalidValue;
 }

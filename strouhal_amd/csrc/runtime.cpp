@@ -68,9 +68,10 @@ struct strh_lattice {
     strh_ctx* ctx = nullptr;
     int precision = kFP32;
     int nx = 0, ny = 0, nz = 0;
-    uint32_t N = 0, S = 0;
+    uint32_t N = 0, S = 0;      // cells; N rounded up to 64
     size_t B = 4;               // bytes per stored DDF
-    void* f = nullptr;          // 19 slots of S elements
+    DdfLayout lay;              // block layout of f (lbm.h)
+    void* f = nullptr;          // the DDFs, ddf_elements(lay) elements
     uint8_t* flags = nullptr;   // N raw Cell values
     uint32_t* link = nullptr;   // N link words (walls only)
     uint32_t* lidm = nullptr;   // N moving-wall masks (walls only)
@@ -99,7 +100,44 @@ struct strh_lattice {
 
 namespace {
 
-size_t slot_bytes(const strh_lattice* L) { return (size_t)L->S * L->B; }
+size_t ddf_bytes(const strh_lattice* L) { return (size_t)ddf_elements(L->lay) * L->B; }
+
+// Copy cells [c0, c0 + count) of DDF slot `slot` between the device image
+// and the contiguous buffer `lin` (count elements), in at most three copies:
+// the rest of a partly covered first block, one pitched copy of the whole
+// blocks, and the tail.
+hipError_t slot_copy(const strh_lattice* L, uint32_t slot, uint64_t c0, uint64_t count, void* lin,
+                     bool toDevice, hipStream_t s) {
+    const DdfLayout& l = L->lay;
+    const uint64_t B = L->B, W = l.W;
+    char* h = (char*)lin;
+    auto dptr = [&](uint64_t c) {
+        return (char*)L->f + ((c >> l.lw) * (uint64_t)l.bstride + slot * W + (c & l.mask)) * B;
+    };
+    const hipMemcpyKind kind = toDevice ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    auto flat = [&](uint64_t c, uint64_t n) {
+        return toDevice ? hipMemcpyAsync(dptr(c), h, n * B, kind, s) : hipMemcpyAsync(h, dptr(c), n * B, kind, s);
+    };
+    hipError_t e = hipSuccess;
+    if (l.lw == 31u) return count ? flat(c0, count) : hipSuccess;   // one block: plain slot-major
+    uint64_t c = c0, left = count;
+    const uint64_t w = c & (W - 1);
+    if (w != 0 && left != 0) {
+        const uint64_t take = std::min(left, W - w);
+        if ((e = flat(c, take)) != hipSuccess) return e;
+        c += take; h += take * B; left -= take;
+    }
+    const uint64_t full = left / W;
+    if (full != 0) {
+        const size_t pitch = (size_t)l.bstride * B, row = (size_t)(W * B);
+        e = toDevice ? hipMemcpy2DAsync(dptr(c), pitch, h, row, row, full, kind, s)
+                     : hipMemcpy2DAsync(h, row, dptr(c), pitch, row, full, kind, s);
+        if (e != hipSuccess) return e;
+        c += full * W; h += full * W * B; left -= full * W;
+    }
+    if (left != 0) e = flat(c, left);
+    return e;
+}
 
 int set_device(strh_ctx* c) {
     HIP_TRY(hipSetDevice(c->device), "hipSetDevice");
@@ -121,6 +159,7 @@ StepParams make_params(const strh_lattice* L, int64_t t, bool writeForce) {
     p.nx = L->nx; p.ny = L->ny; p.nz = L->nz;
     p.parity = (int)(t & 1);
     p.N = L->N; p.S = L->S;
+    p.lay = L->lay;
     p.omega = L->omega; p.omegaMinus = L->omegaMinus;
     p.uin = L->uinTarget * r;
     p.cSmago = L->cSmago;
@@ -249,7 +288,7 @@ int ensure_moments(strh_lattice* L) {
 int compute_moments(strh_lattice* L) {
     int rc = ensure_moments(L);
     if (rc) return rc;
-    HIP_TRY(launch_moments(L->precision, L->f, L->flags, L->moments, L->N, L->S,
+    HIP_TRY(launch_moments(L->precision, L->f, L->flags, L->moments, L->N, L->lay,
                            L->bodyForce[0], L->bodyForce[1], L->bodyForce[2], L->ctx->stream),
             "moments kernel launch");
     return STRH_OK;
@@ -326,6 +365,7 @@ int strh_lattice_create(strh_ctx* c, const strh_lattice_desc* d, const uint8_t* 
     L->N = (uint32_t)N;
     L->S = (uint32_t)((N + 63u) & ~63ull);
     L->B = d->precision == STRH_FP32 ? 4 : 2;
+    L->lay = make_ddf_layout(N, (uint32_t)L->B);
     L->omega = d->omega;
     L->omegaMinus = d->omega_minus < 0.0f ? d->omega : d->omega_minus;
     for (int k = 0; k < 3; k++) { L->lid[k] = d->lid[k]; L->bodyForce[k] = d->force[k]; }
@@ -341,7 +381,7 @@ int strh_lattice_create(strh_ctx* c, const strh_lattice_desc* d, const uint8_t* 
                    : hip_fail(e, what);
     };
     hipError_t e;
-    const size_t fbytes = 19 * slot_bytes(L);
+    const size_t fbytes = ddf_bytes(L);
     if ((e = hipMalloc(&L->f, fbytes)) != hipSuccess) return bail(e, "hipMalloc(f)");
     // Shifted rest state is exactly 0 in every precision (Sim.swift:225).
     if ((e = hipMemsetAsync(L->f, 0, fbytes, c->stream)) != hipSuccess) return bail(e, "hipMemset(f)");
@@ -470,7 +510,7 @@ int strh_init_field(strh_lattice* L, uint32_t mode, float amplitude) {
     if (mode > 2) return fail(STRH_ERR_INVALID, "unknown init mode %u", mode);
     const float tau = 1.0f / L->omega;
     HIP_TRY(launch_init(L->precision, L->f, (uint32_t)L->nx, (uint32_t)L->ny, (uint32_t)L->nz,
-                        L->S, mode, amplitude, tau, L->ctx->stream), "init kernel launch");
+                        L->lay, mode, amplitude, tau, L->ctx->stream), "init kernel launch");
     return sync(L, "init kernel");
 }
 
@@ -524,7 +564,7 @@ int strh_mass_sum(strh_lattice* L, double* out) {
     int rc = check_lat(L);
     if (rc) return rc;
     if (!out) return fail(STRH_ERR_INVALID, "null output");
-    HIP_TRY(launch_mass_sum(L->precision, L->f, L->N, L->S, L->partials, kMaxPartials, L->result,
+    HIP_TRY(launch_mass_sum(L->precision, L->f, L->N, L->lay, L->partials, kMaxPartials, L->result,
                             L->ctx->stream), "mass reduction launch");
     HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy mass");
     return sync(L, "mass sum");
@@ -548,7 +588,7 @@ int strh_dissipation(strh_lattice* L, double* out) {
     if (!out) return fail(STRH_ERR_INVALID, "null output");
     if (L->stepsDone % 2 != 0) return fail(STRH_ERR_ODD_STEPS, "dissipation requires even step count");
     const double tauP = 1.0 / (double)L->omega;
-    HIP_TRY(launch_dissipation_sum(L->precision, L->f, L->walls ? L->flags : nullptr, L->N, L->S, tauP,
+    HIP_TRY(launch_dissipation_sum(L->precision, L->f, L->walls ? L->flags : nullptr, L->N, L->lay, tauP,
                                    L->partials, kMaxPartials, L->result, L->ctx->stream),
             "dissipation reduction launch");
     HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy dissipation");
@@ -556,13 +596,14 @@ int strh_dissipation(strh_lattice* L, double* out) {
 }
 
 // Canonical image: 19 slots of N values, slot-major (Sim.swift:441-444 hashes
-// exactly these bytes). Device slots are padded to S; copy with a pitch.
+// exactly these bytes), gathered from / scattered to the block layout.
 int strh_download_ddf(strh_lattice* L, void* out, size_t bytes) {
     int rc = check_lat(L);
     if (rc) return rc;
     const size_t row = (size_t)L->N * L->B;
     if (!out || bytes != 19 * row) return fail(STRH_ERR_INVALID, "download needs exactly %zu bytes", 19 * row);
-    HIP_TRY(hipMemcpy2DAsync(out, row, L->f, slot_bytes(L), row, 19, hipMemcpyDeviceToHost, L->ctx->stream), "download ddf");
+    for (uint32_t i = 0; i < 19; i++)
+        HIP_TRY(slot_copy(L, i, 0, L->N, (char*)out + i * row, false, L->ctx->stream), "download ddf");
     return sync(L, "download ddf");
 }
 
@@ -571,7 +612,8 @@ int strh_upload_ddf(strh_lattice* L, const void* in, size_t bytes) {
     if (rc) return rc;
     const size_t row = (size_t)L->N * L->B;
     if (!in || bytes != 19 * row) return fail(STRH_ERR_INVALID, "upload needs exactly %zu bytes", 19 * row);
-    HIP_TRY(hipMemcpy2DAsync(L->f, slot_bytes(L), in, row, row, 19, hipMemcpyHostToDevice, L->ctx->stream), "upload ddf");
+    for (uint32_t i = 0; i < 19; i++)
+        HIP_TRY(slot_copy(L, i, 0, L->N, (char*)in + i * row, true, L->ctx->stream), "upload ddf");
     return sync(L, "upload ddf");
 }
 
@@ -615,9 +657,8 @@ int strh_state_digest(strh_lattice* L, uint8_t out[32]) {
             const size_t o = off + done_bytes;
             const size_t slot = o / row, in = o % row;
             const size_t take = std::min(len - done_bytes, row - in);
-            hipError_t e = hipMemcpyAsync((char*)stage[k] + done_bytes,
-                                          (const char*)L->f + slot * slot_bytes(L) + in, take,
-                                          hipMemcpyDeviceToHost, L->ctx->stream);
+            hipError_t e = slot_copy(L, (uint32_t)slot, in / L->B, take / L->B,
+                                     (char*)stage[k] + done_bytes, false, L->ctx->stream);
             if (e != hipSuccess) return e;
             done_bytes += take;
         }
@@ -648,7 +689,7 @@ int strh_state_digest(strh_lattice* L, uint8_t out[32]) {
 int strh_snapshot(strh_lattice* L) {
     int rc = check_lat(L);
     if (rc) return rc;
-    const size_t fbytes = 19 * slot_bytes(L);
+    const size_t fbytes = ddf_bytes(L);
     if (!L->snap) HIP_TRY(hipMalloc(&L->snap, fbytes), "hipMalloc(snapshot)");
     HIP_TRY(hipMemcpyAsync(L->snap, L->f, fbytes, hipMemcpyDeviceToDevice, L->ctx->stream), "snapshot");
     L->snapSteps = L->stepsDone;
@@ -659,7 +700,7 @@ int strh_restore(strh_lattice* L) {
     int rc = check_lat(L);
     if (rc) return rc;
     if (!L->snap || L->snapSteps < 0) return fail(STRH_ERR_INVALID, "no snapshot taken");
-    HIP_TRY(hipMemcpyAsync(L->f, L->snap, 19 * slot_bytes(L), hipMemcpyDeviceToDevice, L->ctx->stream), "restore");
+    HIP_TRY(hipMemcpyAsync(L->f, L->snap, ddf_bytes(L), hipMemcpyDeviceToDevice, L->ctx->stream), "restore");
     L->stepsDone = L->snapSteps;
     return sync(L, "restore");
 }
@@ -984,7 +1025,7 @@ int strh_batch_autotune(strh_batch* B, int* chosen) {
     std::vector<void*> saved(B->lats.size(), nullptr);
     auto release = [&]() { for (void* p : saved) if (p) hipFree(p); };
     for (size_t i = 0; i < B->lats.size(); i++) {
-        const size_t bytes = 19 * slot_bytes(B->lats[i]);
+        const size_t bytes = ddf_bytes(B->lats[i]);
         if (hipMalloc(&saved[i], bytes) != hipSuccess) { release(); return fail(STRH_ERR_ALLOC, "autotune copy"); }
         if (hipMemcpy(saved[i], B->lats[i]->f, bytes, hipMemcpyDeviceToDevice) != hipSuccess) {
             release(); return fail(STRH_ERR_DEVICE, "autotune copy");
@@ -1004,7 +1045,7 @@ int strh_batch_autotune(strh_batch* B, int* chosen) {
         if (sec < best * 0.98) { best = sec; bestBs = bs; }
     }
     for (size_t i = 0; i < B->lats.size(); i++)
-        hipMemcpy(B->lats[i]->f, saved[i], 19 * slot_bytes(B->lats[i]), hipMemcpyDeviceToDevice);
+        hipMemcpy(B->lats[i]->f, saved[i], ddf_bytes(B->lats[i]), hipMemcpyDeviceToDevice);
     release();
     if (rc) { B->bs = keep; return rc; }
     B->bs = bestBs;
@@ -1143,7 +1184,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     if ((rc = adj_plan(L, o, P))) return rc;
     const int64_t T = P.T;
     const size_t N = L->N, S = L->S;
-    const size_t ddf = 19 * slot_bytes(L);
+    const size_t ddf = ddf_bytes(L);
     if (K < -1) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= -1");
     if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
     if (K == -1) {
