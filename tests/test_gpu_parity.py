@@ -208,6 +208,65 @@ def test_row_staged_odd_pass(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+def test_pow2_walls_blocked_layout(gpu, precision):
+    """Wall lattices with nx a power of two (64, 128, 512) and the blocked DDF
+    layout, across block boundaries: bounce-back, moving walls, dormant ABB,
+    NT partial cells with LES and momentum exchange; same bits as the oracle,
+    DDFs and per-cell forces."""
+    rng = np.random.default_rng(21)
+    # channel with parabolic inflow, sponge and TRT
+    g, r = make_pair(gpu, precision, 64, 12, 3, 1.5, 1.7, uin=0.06, ramp_steps=5,
+                     flags=_flags("channel", 64, 12, 3))
+    g.sponge = (40.0, 16.0, 1.0)
+    r.sponge = g.sponge
+    _seed_both(g, r, precision, seed=41)
+    _run_compare(g, r, [1, 2, 3], precision)
+    # every flag kind scattered, moving lid, body force, forces probed
+    fl = rng.choice([0] * 8 + [1, 2, 3, 4], size=(4, 6, 128)).astype(np.uint8)
+    g, r = make_pair(gpu, precision, 128, 6, 4, 1.4, 1.2, lid=(0.05, 0.0, 0.01), uin=0.04,
+                     force=(1e-5, -2e-5, 0.0), flags=fl, wants_forces=True)
+    _seed_both(g, r, precision, seed=43)
+    _run_compare(g, r, [1, 3], precision)
+    fg = g.probe_force((0, 127), (0, 5))
+    fr = r.probe_force((0, 127), (0, 5))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+    assert_bit_exact(g.download_force(), r.force_field().astype(np.float32), "per-cell force")
+    scale = np.abs(r.force_field()[:, :3]).sum() + 1e-300
+    assert np.all(np.abs(fg - fr) <= 1e-13 * scale), (fg, fr)
+    # NT partial cells with LES in a 512-wide channel (512-thread workgroups)
+    nx, ny, nz = 512, 6, 2
+    eps = np.zeros((nz, ny, nx), dtype=np.float32)
+    eps[:, 2:4, 100:140] = rng.uniform(0.0, 1.0, size=(nz, 2, 40)).astype(np.float32)
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.8, None, uin=0.05, c_smago=0.04,
+                     flags=_flags("channel", nx, ny, nz), wants_forces=True)
+    g.set_solid_fractions(eps)
+    r.set_solid_fractions(eps)
+    _seed_both(g, r, precision, seed=47)
+    _run_compare(g, r, [1, 3], precision)
+    fg = g.probe_force((90, 150), (0, 5))
+    fr = r.probe_force((90, 150), (0, 5))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after NT force probe")
+    assert_bit_exact(g.download_force(), r.force_field().astype(np.float32), "per-cell NT force")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("kind", ["channel", "periodic"])
+def test_pow2_brinkman_blocked_layout(gpu, precision, kind):
+    nx, ny, nz = 64, 10, 3
+    rng = np.random.default_rng(5)
+    alpha = rng.uniform(0.0, 0.5, size=(nz, ny, nx)).astype(np.float32)
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.3, 1.6, uin=0.05, force=(1e-5, 0.0, 2e-6),
+                     flags=_flags(kind, nx, ny, nz), wants_forces=True)
+    g.set_porosity(alpha)
+    r.set_porosity(alpha)
+    _seed_both(g, r, precision, seed=53)
+    _run_compare(g, r, [1, 2, 3], precision)
+    fg = g.probe_force((1, nx - 2), (1, ny - 2))
+    fr = r.probe_force((1, nx - 2), (1, ny - 2))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 def test_row_staged_noble_torczynski_force(gpu, precision):
     nx, ny, nz = 64, 12, 4
     rng = np.random.default_rng(8)
