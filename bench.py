@@ -6,7 +6,9 @@ Re = 1600 on L = N / 2 pi, tau = 3 nu + 1/2), initialised on the GPU
 (initField mode 2, Kernels.metal:414-438) and stepped by the HIP kernels in
 two variants: TRT Lambda = 3/16 with FP32 storage (the headline `value`) and
 SRT with FP16 shifted storage. A "step" is one LBM time step of all 256^3
-cells; the DDFs are resident in HBM before timing starts. MLUPS counts every
+cells; the DDFs are resident in HBM before timing starts. The line also
+carries the adjoint/forward step-time ratio of the porous-channel objective
+(512x256x1, 6000 steps; strh_vjp against strh_run, HIP events). MLUPS counts every
 cell, as the reference's bench does (Sources/strouhal/Cases.swift:141).
 
 The roofline is HBM: algorithmic bytes per cell per step are 19 DDF reads +
@@ -94,6 +96,33 @@ def cpu_baseline(n: int, sample_steps: int):
     }
 
 
+def adjoint_ratio(spinup: int = 2000, avg: int = 4000):
+    """Adjoint/forward time of the porous-channel objective (512x256x1, Re 50,
+    Brinkman design field): strh_vjp against strh_run over the same T steps,
+    both from HIP events, with the default checkpoint plan (all moments fit)
+    and with ceil(sqrt(T)) checkpoints."""
+    from strouhal_amd.channel_adjoint import channel_case
+    T = spinup + avg
+    obj = dict(inlet_x=8, outlet_x=503, spinup=spinup, avg=avg, w_p=1.0, w_d=1.0)
+    fwd, _ = channel_case(None)
+    fwd.run(2)
+    g0 = fwd.gpu_seconds
+    fwd.run(T)
+    t_fwd = fwd.gpu_seconds - g0
+    out = {"workload": f"porous channel 512x256x1 Re 50, Brinkman design field, T = {T} steps",
+           "forward_ms_per_step": round(t_fwd / T * 1e3, 5)}
+    for name, k in (("auto", -1), ("sqrt", 0)):
+        sim, _ = channel_case(None)
+        _, _, _, info = sim.vjp(**obj, checkpoint_every=k)
+        t = info["forward_seconds"] + info["recompute_seconds"] + info["backward_seconds"]
+        out[f"ratio_{name}"] = round(t / t_fwd, 3)
+        out[f"checkpoint_every_{name}"] = info["checkpoint_every"]
+        out[f"workspace_bytes_{name}"] = info["workspace_bytes"]
+        sim.close()
+    fwd.close()
+    return out
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "step_kernel_traffic_tgv256.json")
     if not os.path.exists(p):
@@ -111,6 +140,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-sample-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -135,6 +165,7 @@ def main():
         "traffic": traffic,
     }
     fp16_achieved = fp16["bytes_per_cell"] * cells / (fp16["kernel_ms"] * 1e-3) / 1e9
+    adj = None if args.no_adjoint else adjoint_ratio()
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_sample_steps)
     line = {
         "metric": "MLUPS + % HBM roofline, D3Q19 256³ FP32/FP16; adjoint/forward step-time ratio",
@@ -163,6 +194,7 @@ def main():
         "fp16s": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in fp16.items()},
         "fp16s_roofline": {"achieved": round(fp16_achieved, 1), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(fp16_achieved / HBM_PEAK_GBS, 4)},
+        "adjoint": adj,
         "vs_baseline_ref": "reference README.md:24-28, Apple M5 256^3 FP32 ~785 MLUPS",
     }
     print(json.dumps(line), flush=True)

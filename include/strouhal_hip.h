@@ -153,6 +153,17 @@ STRH_API int strh_download_force(strh_lattice* lat, float* out);
 
 /* ---- checkpoint / resume (new): device-side snapshot of the DDFs and the
  * step count; restore rewinds both. One snapshot slot per lattice. */
+/* State files (no reference counterpart: the reference keeps state only in
+ * memory and replays to resume, Cases.swift:989-999). A 128-byte
+ * little-endian header -- char magic[8] "STRHDDF1", uint32 version (1),
+ * uint32 precision, int32 nx, ny, nz, uint32 bytes per value, int64 steps
+ * done, uint64 image bytes, uint8 digest[32] (SHA-256, at offset 48), 48
+ * reserved bytes -- followed by the canonical image (the bytes
+ * strh_state_digest hashes).
+ * load checks magic, precision, dimensions and the digest before it uploads
+ * anything, and restores steps done. */
+STRH_API int strh_save_state(strh_lattice* lat, const char* path);
+STRH_API int strh_load_state(strh_lattice* lat, const char* path);
 STRH_API int strh_snapshot(strh_lattice* lat);
 STRH_API int strh_restore(strh_lattice* lat);
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "strh_set_solid_fractions", "strh_set_porosity", "strh_init_field", "strh_run", "strh_steps_done",
     "strh_gpu_seconds", "strh_probe_moments", "strh_probe_force", "strh_state_digest",
     "strh_mass_sum", "strh_kinetic_energy", "strh_dissipation", "strh_download_ddf", "strh_upload_ddf",
-    "strh_download_force", "strh_snapshot", "strh_restore", "strh_sha256",
+    "strh_download_force", "strh_save_state", "strh_load_state", "strh_snapshot", "strh_restore", "strh_sha256",
     "strh_sha256_force_portable", "strh_batch_create", "strh_batch_destroy",
     "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
     "strh_batch_set_block_size", "strh_batch_autotune", "strh_batch_set_graph_steps",
@@ -121,6 +121,8 @@ def lib():
         "strh_download_ddf": [vp, vp, sz],
         "strh_upload_ddf": [vp, vp, sz],
         "strh_download_force": [vp, vp],
+        "strh_save_state": [vp, ctypes.c_char_p],
+        "strh_load_state": [vp, ctypes.c_char_p],
         "strh_snapshot": [vp],
         "strh_restore": [vp],
         "strh_sha256": [vp, sz, vp],

@@ -626,12 +626,14 @@ int strh_download_force(strh_lattice* L, float* out) {
     return sync(L, "download force");
 }
 
-// SHA-256 over the canonical image, streamed through two pinned staging
-// buffers so the next chunk's copy overlaps hashing of the current one.
-int strh_state_digest(strh_lattice* L, uint8_t out[32]) {
-    int rc = check_lat(L);
-    if (rc) return rc;
-    if (!out) return fail(STRH_ERR_INVALID, "null output");
+}  // extern "C"
+
+namespace {
+
+// Stream the canonical image (19 slots of N values, slot-major) through two
+// pinned staging buffers of up to 32 MiB: the copy of the next chunk overlaps
+// `consume(bytes, len)` on the current one. `consume` returns false to stop.
+template <class F> int stream_canonical(strh_lattice* L, F&& consume) {
     const size_t row = (size_t)L->N * L->B;
     const size_t total = 19 * row;
     const size_t chunk = std::min<size_t>(total, (size_t)32 << 20);
@@ -664,25 +666,128 @@ int strh_state_digest(strh_lattice* L, uint8_t out[32]) {
         }
         return hipEventRecord(done[k], L->ctx->stream);
     };
-    Sha256 sha;
     size_t off = 0;
     int k = 0;
+    bool ok = true;
     hipError_t e = issue(0, std::min(chunk, total), 0);
-    while (e == hipSuccess && off < total) {
+    while (e == hipSuccess && off < total && ok) {
         const size_t len = std::min(chunk, total - off);
         const size_t next = off + len;
         if (next < total) e = issue(next, std::min(chunk, total - next), k ^ 1);
         if (e != hipSuccess) break;
         e = hipEventSynchronize(done[k]);
         if (e != hipSuccess) break;
-        sha.update(stage[k], len);
+        ok = consume((const uint8_t*)stage[k], len);
         off = next;
         k ^= 1;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(L->ctx->stream);
     cleanup();
-    if (e != hipSuccess) return hip_fail(e, "state digest");
+    if (e != hipSuccess) return hip_fail(e, "canonical image copy");
+    return ok ? STRH_OK : STRH_ERR_INVALID;
+}
+
+// State file: a 128-byte little-endian header, then the canonical image.
+struct StateHeader {
+    char magic[8];          // "STRHDDF1"
+    uint32_t version;       // 1
+    uint32_t precision;     // strh_precision
+    int32_t nx, ny, nz;
+    uint32_t bytes_per_value;
+    int64_t steps_done;
+    uint64_t image_bytes;   // 19 * N * bytes_per_value
+    uint8_t digest[32];     // SHA-256 of the image (the state digest)
+    uint8_t reserved[48];
+};
+static_assert(sizeof(StateHeader) == 128, "state header is 128 bytes");
+
+}  // namespace
+
+extern "C" {
+
+// SHA-256 over the canonical image (Sim.swift:441-444).
+int strh_state_digest(strh_lattice* L, uint8_t out[32]) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    Sha256 sha;
+    if ((rc = stream_canonical(L, [&](const uint8_t* b, size_t n) { sha.update(b, n); return true; }))) return rc;
     sha.finish(out);
+    return STRH_OK;
+}
+
+int strh_save_state(strh_lattice* L, const char* path) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!path) return fail(STRH_ERR_INVALID, "null path");
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return fail(STRH_ERR_INVALID, "cannot open %s for writing", path);
+    StateHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, "STRHDDF1", 8);
+    h.version = 1;
+    h.precision = (uint32_t)L->precision;
+    h.nx = L->nx; h.ny = L->ny; h.nz = L->nz;
+    h.bytes_per_value = (uint32_t)L->B;
+    h.steps_done = L->stepsDone;
+    h.image_bytes = 19ull * L->N * L->B;
+    bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
+    Sha256 sha;
+    if (ok) rc = stream_canonical(L, [&](const uint8_t* b, size_t n) {
+        sha.update(b, n);
+        return fwrite(b, 1, n, fp) == n;
+    });
+    if (ok && rc == STRH_OK) {
+        sha.finish(h.digest);
+        ok = fseek(fp, 0, SEEK_SET) == 0 && fwrite(&h, sizeof(h), 1, fp) == 1;
+    }
+    ok = (fclose(fp) == 0) && ok;
+    if (rc) return rc == STRH_ERR_INVALID ? fail(STRH_ERR_INVALID, "write to %s failed", path) : rc;
+    if (!ok) return fail(STRH_ERR_INVALID, "write to %s failed", path);
+    return STRH_OK;
+}
+
+int strh_load_state(strh_lattice* L, const char* path) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!path) return fail(STRH_ERR_INVALID, "null path");
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return fail(STRH_ERR_INVALID, "cannot open %s", path);
+    StateHeader h;
+    const size_t row = (size_t)L->N * L->B;
+    auto bad = [&](const char* why) { fclose(fp); return fail(STRH_ERR_INVALID, "%s: %s", path, why); };
+    if (fread(&h, sizeof(h), 1, fp) != 1) return bad("truncated header");
+    if (memcmp(h.magic, "STRHDDF1", 8) != 0 || h.version != 1) return bad("not a state file (magic / version)");
+    if (h.precision != (uint32_t)L->precision || h.bytes_per_value != L->B)
+        return bad("precision differs from the lattice's");
+    if (h.nx != L->nx || h.ny != L->ny || h.nz != L->nz) return bad("dimensions differ from the lattice's");
+    if (h.image_bytes != 19 * row) return bad("image size does not match");
+    // pass 1: the image must hash to the header's digest before anything is uploaded
+    std::vector<uint8_t> buf((size_t)std::min<uint64_t>(h.image_bytes, 32ull << 20));
+    Sha256 sha;
+    for (uint64_t left = h.image_bytes; left;) {
+        const size_t n = (size_t)std::min<uint64_t>(left, buf.size());
+        if (fread(buf.data(), 1, n, fp) != n) return bad("truncated image");
+        sha.update(buf.data(), n);
+        left -= n;
+    }
+    uint8_t dg[32];
+    sha.finish(dg);
+    if (memcmp(dg, h.digest, 32) != 0) return bad("image does not match its digest");
+    // pass 2: upload slot by slot
+    if (fseek(fp, (long)sizeof(h), SEEK_SET) != 0) return bad("seek failed");
+    for (uint32_t i = 0; i < 19; i++) {
+        for (size_t done = 0; done < row;) {
+            const size_t n = std::min(row - done, buf.size());
+            if (fread(buf.data(), 1, n, fp) != n) return bad("truncated image");
+            hipError_t e = slot_copy(L, i, done / L->B, n / L->B, buf.data(), true, L->ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(L->ctx->stream);
+            if (e != hipSuccess) { fclose(fp); return hip_fail(e, "upload state"); }
+            done += n;
+        }
+    }
+    fclose(fp);
+    L->stepsDone = h.steps_done;
     return STRH_OK;
 }
 

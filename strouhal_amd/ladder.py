@@ -114,6 +114,15 @@ def coefficients(v: Variant, forces: np.ndarray):
     return cd, cl, t
 
 
+def write_qoi_csv(path: str, t, cd, cl):
+    """The app's quantity-of-interest export (StrouhalApp.swift:906-915):
+    header `step,C_D,C_L`, one `%.0f,%.6f,%.6f` row per sample."""
+    with open(path, "w") as fh:
+        fh.write("step,C_D,C_L\n")
+        for a, b, c in zip(t, cd, cl):
+            fh.write("%.0f,%.6f,%.6f\n" % (a, b, c))
+
+
 def build(gpu, precision, resolutions, scale_steps, base_umax):
     out = []
     for D in resolutions:
@@ -160,6 +169,7 @@ def main():
     ap.add_argument("--resolutions", default="16,24,32,48,64")
     ap.add_argument("--mach", type=float, default=MA_DEFAULT)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--csv", default="", help="directory for one step,C_D,C_L file per rung")
     args = ap.parse_args()
     res = [int(r) for r in args.resolutions.split(",")]
     base_umax = 1.5 * args.mach / math.sqrt(3.0)      # Ma of U_mean = umax * 2/3 * sqrt(3)
@@ -176,6 +186,9 @@ def main():
     for v, s in zip(vb, sb):
         cd, cl, t = coefficients(v, s)
         st = cycle_stats(cl, cd, t, v.D, v.case.u_mean)
+        if args.csv:
+            os.makedirs(args.csv, exist_ok=True)
+            write_qoi_csv(os.path.join(args.csv, f"qoi_D{v.D}_ma{v.mach_scale:g}.csv"), t, cd, cl)
         rungs.append({"D": v.D, "mach_scale": v.mach_scale, "steps": v.total_steps,
                       "samples": len(s), "finite": bool(np.all(np.isfinite(s))), **st,
                       "digest": v.case.sim.state_digest[:16]})

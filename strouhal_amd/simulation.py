@@ -17,6 +17,7 @@ odd-step preconditions (Sim.swift:335, 360) raise ``StrouhalError`` with
 from __future__ import annotations
 
 import ctypes
+import os
 import enum
 
 import numpy as np
@@ -362,6 +363,14 @@ class Simulation:
         v = ctypes.c_double()
         check(lib().strh_kinetic_energy(self._h, ctypes.byref(v)))
         return v.value
+
+    def save_state(self, path: str):
+        """Write the DDFs, step count and digest to a state file."""
+        check(lib().strh_save_state(self._h, os.fsencode(path)))
+
+    def load_state(self, path: str):
+        """Restore a state file written by save_state (same precision and size)."""
+        check(lib().strh_load_state(self._h, os.fsencode(path)))
 
     def dissipation(self) -> float:
         """FP64 sum over fluid cells of 2 nu S:S, S from the non-equilibrium

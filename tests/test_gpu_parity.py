@@ -428,6 +428,50 @@ def test_snapshot_restore(gpu):
     assert sim.state_digest == after
 
 
+@pytest.mark.parametrize("precision,shape", [("fp32", (40, 24, 18)), ("fp16s", (33, 7, 5)),
+                                              ("bf16", (128, 80, 2))])
+def test_state_file_round_trip(gpu, precision, shape, tmp_path):
+    """save_state / load_state: the file holds the canonical image, the step
+    count and the digest; a loaded lattice continues bit for bit; a corrupted
+    or mismatched file is refused before anything is uploaded."""
+    import hashlib
+    from strouhal_amd import Simulation, StrouhalError
+    nx, ny, nz = shape
+    fl = _flags("mixed", nx, ny, nz, seed=3)
+    mk = lambda: Simulation(gpu, precision, nx=nx, ny=ny, nz=nz, omega=1.6, omega_minus=1.3,
+                            lid=(0.04, 0.0, 0.0), uin=0.05, ramp_steps=7, flags=fl)
+    a = mk()
+    a.upload_ddf(to_storage(random_state(a.cells, 61), precision))
+    a.run(9)
+    path = str(tmp_path / "state.strh")
+    a.save_state(path)
+    raw = open(path, "rb").read()
+    img = a.download_ddf()
+    assert raw[:8] == b"STRHDDF1" and len(raw) == 128 + img.nbytes
+    assert raw[128:] == img.tobytes()
+    assert raw[48:80].hex() == a.state_digest == hashlib.sha256(img.tobytes()).hexdigest()
+    b = mk()
+    b.load_state(path)
+    assert b.steps_done == 9 and b.state_digest == a.state_digest
+    a.run(5)
+    b.run(5)
+    assert b.state_digest == a.state_digest
+    bad = bytearray(raw)
+    bad[200] ^= 0x40
+    open(path, "wb").write(bytes(bad))
+    c = mk()
+    before = c.state_digest
+    with pytest.raises(StrouhalError):
+        c.load_state(path)
+    assert c.state_digest == before and c.steps_done == 0
+    open(path, "wb").write(raw)
+    other = Simulation(gpu, precision, nx=nx + 1, ny=ny, nz=nz, omega=1.6)
+    with pytest.raises(StrouhalError):
+        other.load_state(path)
+    with pytest.raises(StrouhalError):
+        c.load_state(str(tmp_path / "missing.strh"))
+
+
 def test_error_paths(gpu):
     from strouhal_amd import Simulation, StrouhalError
     sim = Simulation(gpu, "fp32", nx=8, ny=8, nz=8, omega=1.0)

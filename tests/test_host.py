@@ -177,3 +177,11 @@ def test_tgv_post_processing():
     assert dns["eps_star"][0] == pytest.approx(3.0 / (4.0 * 1600.0), rel=1e-3)
     L, tau, tc_, stride = tgv_setup(256)
     assert stride == 80 and tau == pytest.approx(0.503820, abs=1e-6)
+
+
+def test_qoi_csv_format(tmp_path):
+    """The ladder's QoI export writes the app's `step,C_D,C_L` CSV."""
+    from strouhal_amd.ladder import write_qoi_csv
+    p = tmp_path / "q.csv"
+    write_qoi_csv(str(p), [100.0, 202.0], [3.2, 3.25], [-0.1234567, 0.5])
+    assert p.read_text() == "step,C_D,C_L\n100,3.200000,-0.123457\n202,3.250000,0.500000\n"
