@@ -201,17 +201,21 @@ STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);
 /* ---- discrete adjoint (no reference counterpart: the reference has no
  * gradients) ------------------------------------------------------------------
  * Objective over the T = spinup + avg steps from the current state, evaluated
- * on the pre-collision moments of each step k >= spinup and averaged over avg:
- *   J = (1/avg) sum_k [ w_p (rho_in - rho_out) / 3 + w_d sum_x alpha(x) J_x(x) ]
+ * on the pre-collision state of each step k >= spinup and averaged over avg:
+ *   J = (1/avg) sum_k [ w_p (rho_in - rho_out) / 3 + w_d sum_x alpha(x) J_x(x)
+ *                       + w_f sum_x ntF_x(x) ]
  * rho_in / rho_out: mean density of the fluid cells of column x = inlet_x /
  * outlet_x (-1: no column); J_x = (p_x + F_x/2) / (1 + alpha/2) is the
- * momentum the Brinkman drag acts on. The lattice may use walls of every
- * kind, Guo forcing, TRT/SRT and porosity; partial cells, LES and the sponge
- * give STRH_ERR_UNSUPPORTED. */
+ * momentum the Brinkman drag acts on; ntF_x = -2 sum_i c_ix B (f_opp(i) - f_i)
+ * is the x momentum exchange of a partial cell (Kernels.metal:201-227), so
+ * the w_f term is the drag on a Noble-Torczynski body. The lattice may use
+ * walls of every kind, Guo forcing, TRT/SRT, and porosity or partial cells
+ * (not both); LES and the sponge give STRH_ERR_UNSUPPORTED. */
 typedef struct strh_objective {
     int32_t inlet_x, outlet_x;
     double w_p, w_d;
     int64_t spinup, avg;
+    double w_f;
 } strh_objective;
 
 typedef struct strh_vjp_info {
@@ -226,8 +230,11 @@ typedef struct strh_vjp_info {
 /* Advance T steps and return J (FP64, fixed-order reduction). */
 STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, double* J);
 /* Advance T steps, return J and its gradient with respect to the porosity of
- * every cell (grad_alpha: N doubles, or NULL; zero on non-fluid cells) and to
- * the inlet velocity target uin (grad_uin, or NULL). The state after the call
+ * every cell (grad_alpha: N doubles, or NULL; zero on non-fluid cells), to the
+ * solid fraction of every fluid cell whose solid fraction is non-zero
+ * (grad_eps: N doubles, or NULL; zero elsewhere) and to the inlet velocity
+ * target uin (grad_uin, or NULL). Partial-cell lattices also record the 19
+ * populations of those cells per step. The state after the call
  * is the state after strh_run(lat, T), bit for bit. The forward pass keeps a
  * DDF checkpoint at the start of every segment of K = checkpoint_every steps;
  * the reverse pass replays one segment at a time, recording the four
@@ -240,7 +247,8 @@ STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, d
  * on K. Device memory: (ceil(T/K) - 1) DDF images + K * N * 16 bytes +
  * 2 * 19 * N floats + 3 * N doubles + one DDF image. info may be NULL. */
 STRH_API int strh_vjp(strh_lattice* lat, const strh_objective* obj, int64_t checkpoint_every,
-                      double* J, double* grad_alpha, double* grad_uin, strh_vjp_info* info);
+                      double* J, double* grad_alpha, double* grad_eps, double* grad_uin,
+                      strh_vjp_info* info);
 
 /* ---- host utilities (no counterpart in the reference, which hashes with
  * CryptoKit): SHA-256 of a host buffer, and a switch that forces the portable

@@ -56,6 +56,7 @@ class Objective(ctypes.Structure):
         ("inlet_x", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
         ("w_p", ctypes.c_double), ("w_d", ctypes.c_double),
         ("spinup", ctypes.c_int64), ("avg", ctypes.c_int64),
+        ("w_f", ctypes.c_double),
     ]
 
 
@@ -82,7 +83,8 @@ def lib():
         L.orc_set_porosity.argtypes = [vp, ctypes.c_void_p]
         L.orc_objective.argtypes = [vp, ctypes.POINTER(Objective)]
         L.orc_objective.restype = ctypes.c_double
-        L.orc_vjp.argtypes = [vp, ctypes.POINTER(Objective), ctypes.c_void_p, ctypes.c_void_p, i64]
+        L.orc_vjp.argtypes = [vp, ctypes.POINTER(Objective), ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_void_p, i64]
         L.orc_vjp.restype = ctypes.c_double
         L.orc_init_field.argtypes = [vp, i32, ctypes.c_double]
         L.orc_run.argtypes = [vp, i64]
@@ -214,28 +216,35 @@ class OracleSim:
         lib().orc_set_porosity(self._h, _ptr(a))
 
     @staticmethod
-    def _objective(inlet_x, outlet_x, w_p, w_d, spinup, avg):
+    def _objective(inlet_x, outlet_x, w_p, w_d, spinup, avg, w_f=0.0):
         o = Objective()
         o.inlet_x, o.outlet_x, o.w_p, o.w_d = int(inlet_x), int(outlet_x), float(w_p), float(w_d)
-        o.spinup, o.avg = int(spinup), int(avg)
+        o.spinup, o.avg, o.w_f = int(spinup), int(avg), float(w_f)
         return o
 
-    def objective(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0) -> float:
+    def objective(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0, w_f=0.0) -> float:
         """Run spinup + avg steps (FP64 only) and return the time-averaged objective."""
         self._push()
-        o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg)
+        o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg, w_f)
         return float(lib().orc_objective(self._h, ctypes.byref(o)))
 
-    def vjp(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0, checkpoint_every=0):
-        """FP64 discrete adjoint: (J, dJ/dalpha per cell, dJ/d uin_target).
+    def vjp(self, inlet_x, outlet_x, spinup, avg, w_p=1.0, w_d=1.0, checkpoint_every=0, w_f=0.0,
+            with_eps=False):
+        """FP64 discrete adjoint: (J, dJ/dalpha per cell, dJ/d uin_target), or
+        (J, dJ/dalpha, dJ/deps, dJ/d uin_target) with with_eps (dJ/deps on the
+        cells whose solid fraction is non-zero, 0 elsewhere).
 
         Replays segments of checkpoint_every steps (0: ceil(sqrt(T))) from DDF
         checkpoints; the result does not depend on the interval."""
         self._push()
-        o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg)
+        o = self._objective(inlet_x, outlet_x, w_p, w_d, spinup, avg, w_f)
         ga = np.zeros(self.cells, dtype=np.float64)
+        ge = np.zeros(self.cells, dtype=np.float64)
         gu = np.zeros(1, dtype=np.float64)
-        J = float(lib().orc_vjp(self._h, ctypes.byref(o), _ptr(ga), _ptr(gu), int(checkpoint_every)))
+        J = float(lib().orc_vjp(self._h, ctypes.byref(o), _ptr(ga), _ptr(ge), _ptr(gu),
+                                int(checkpoint_every)))
+        if with_eps:
+            return J, ga, ge, float(gu[0])
         return J, ga, float(gu[0])
 
     def init_field(self, mode: int, amplitude: float):

@@ -145,10 +145,13 @@ struct HostParams {
 //   w_p * (mean rho(inlet column) - mean rho(outlet column)) / 3
 // + w_d * sum_x alpha(x) J_x(x)       (the Brinkman drag, J = rho u)
 // with rho, J the pre-collision moments of each step.
+//   + w_f * sum_x ntF_x(x)                (the partial-cell momentum exchange,
+//                                          the drag on a Noble-Torczynski body)
 struct Objective {
     int32_t inlet_x, outlet_x;
     double w_p, w_d;
     int64_t spinup, avg;
+    double w_f;
 };
 
 // Per-dispatch block, Kernels.metal:49-63.
@@ -192,7 +195,8 @@ struct Base {
     virtual void upload(const void* in) = 0;
     virtual void collideCell(const double* fin, double eps, int x, double* fout,
                              double* rho, double* ntf) = 0;
-    virtual double vjp(const Objective& o, double* gAlpha, double* gUin, int64_t checkpointEvery) = 0;
+    virtual double vjp(const Objective& o, double* gAlpha, double* gEps, double* gUin,
+                       int64_t checkpointEvery) = 0;
     virtual double objective(const Objective& o) = 0;
 };
 
@@ -695,16 +699,36 @@ template <class S> struct Lattice final : Base {
         }
     }
 
-    double cellObjective(const double* g, double a, const Params<R>& p, double cpn, const Objective& o) const {
+    // Partial-cell bounce weight B = eps tw / ((1 - eps) + tw), tw = 1/w+ - 1/2
+    // (Kernels.metal:201-203), and dB/deps.
+    static double ntB(double e, double wp) {
+        const double tw = 1.0 / wp - 0.5;
+        return e * tw / ((1.0 - e) + tw);
+    }
+    static double ntdB(double e, double wp) {
+        const double tw = 1.0 / wp - 0.5;
+        const double d = (1.0 - e) + tw;
+        return tw * (1.0 + tw) / (d * d);
+    }
+
+    double cellObjective(const double* g, double a, double e, const Params<R>& p, double cpn,
+                         const Objective& o) const {
         double rho = 1.0, px = 0.0;
         for (int i = 0; i < 19; i++) { rho += g[i]; px += CX[i] * g[i]; }
         const double gam = 1.0 + 0.5 * a;
         const double Jx = (px + 0.5 * (double)p.fx) / gam;
-        return o.w_p * cpn * rho + o.w_d * a * Jx;
+        double v = o.w_p * cpn * rho + o.w_d * a * Jx;
+        if (e != 0.0 && o.w_f != 0.0) {                 // ntF_x = -2 sum c_x B (g_opp - g)
+            const double B = ntB(e, (double)p.omega);
+            double nt = 0.0;
+            for (int i = 1; i < 19; i += 2) nt -= 2.0 * CX[i] * B * (g[i + 1] - g[i]);
+            v += o.w_f * nt;
+        }
+        return v;
     }
 
     bool adjointSupported() const {
-        return !(hp.c_smago > 0.0 || hp.sponge_on || useEps);
+        return !(hp.c_smago > 0.0 || hp.sponge_on || (useEps && useAlpha));
     }
 
     // Objective terms of step k's pre-collision state g, summed over the
@@ -716,7 +740,8 @@ template <class S> struct Lattice final : Base {
         for (size_t n = 0; n < N; n++) {
             if (flags[n] != FLUID) continue;
             for (int i = 0; i < 19; i++) gc[i] = g[i * N + n];
-            sum += cellObjective(gc, useAlpha ? (double)alpha[n] : 0.0, p, cp[n], o);
+            sum += cellObjective(gc, useAlpha ? (double)alpha[n] : 0.0, useEps ? (double)eps[n] : 0.0,
+                                 p, cp[n], o);
         }
         return sum;
     }
@@ -747,7 +772,7 @@ template <class S> struct Lattice final : Base {
     //   E_i = w+ feq+_i + w- feq-_i + S_i(Guo),  J = (p + Fb/2)/(1 + alpha/2),
     //   F = Fb - alpha J, feq+ = w(rho - 1 + 4.5 (c.J)^2/rho - 1.5 J^2/rho),
     //   feq- = 3 w c.J, S = w[a+ (9 (c.J)(c.F) - 3 J.F)/rho + 3 a- c.F].
-    double vjp(const Objective& o, double* gAlpha, double* gUin, int64_t K) override {
+    double vjp(const Objective& o, double* gAlpha, double* gEps, double* gUin, int64_t K) override {
         if (!std::is_same<R, double>::value || !adjointSupported()) return std::nan("");
         const int64_t total = o.spinup + o.avg;
         if (K <= 0) K = (int64_t)std::ceil(std::sqrt((double)total));
@@ -772,7 +797,7 @@ template <class S> struct Lattice final : Base {
         const int64_t tEnd = stepsDone;
         std::vector<double> G((size_t)K * 19 * N);
         std::vector<double> lamNext(19 * N, 0.0), lam(19 * N, 0.0);
-        std::vector<double> ga(N, 0.0), guc(N, 0.0);
+        std::vector<double> ga(N, 0.0), ge(N, 0.0), guc(N, 0.0);
         for (int64_t seg = C - 1; seg >= 0; seg--) {
           // replay the segment from its checkpoint, keeping every state
           const int64_t k0 = seg * K, len = std::min(K, total - k0);
@@ -864,15 +889,61 @@ template <class S> struct Lattice final : Base {
                     gF[a1] = ap * (9.0 * MJ[a1] - 3.0 * Jv[a1] * M0) / rho + 3.0 * am * Mc[a1];
                     gP[a1] = (gJ[a1] - a * gF[a1]) / gam;
                 }
-                // objective: w_p cp rho + w_d alpha J_x
+                // objective: w_p cp rho + w_d alpha J_x (+ w_f ntF_x on partial cells)
                 const double dObjRho = wobj * o.w_p * cp[n];
                 const double dObjPx = wobj * o.w_d * a / gam;
-                for (int j = 0; j < 19; j++) {
-                    const double mp = j == 0 ? mu[0] : 0.5 * (mu[j] + mu[OPP[j]]);
-                    const double mm = j == 0 ? 0.0 : 0.5 * (mu[j] - mu[OPP[j]]);
-                    lam[j * N + n] = mu[j] - wp * mp - wm * mm + gRho
-                                   + CX[j] * gP[0] + CY[j] * gP[1] + CZ[j] * gP[2]
-                                   + dObjRho + dObjPx * CX[j];
+                const double e = useEps ? (double)eps[n] : 0.0;
+                if (e == 0.0) {
+                    for (int j = 0; j < 19; j++) {
+                        const double mp = j == 0 ? mu[0] : 0.5 * (mu[j] + mu[OPP[j]]);
+                        const double mm = j == 0 ? 0.0 : 0.5 * (mu[j] - mu[OPP[j]]);
+                        lam[j * N + n] = mu[j] - wp * mp - wm * mm + gRho
+                                       + CX[j] * gP[0] + CY[j] * gP[1] + CZ[j] * gP[2]
+                                       + dObjRho + dObjPx * CX[j];
+                    }
+                } else {
+                    // Noble-Torczynski cell (Kernels.metal:201-227):
+                    //   f*_j = f_j + (1 - B) Omega_j(f) + B (f_opp(j) - f_j),  j >= 1,
+                    //   f*_0 = f_0 + (1 - B) Omega_0(f),
+                    // Omega the golden-path increment; B depends on eps only.
+                    const double B = ntB(e, wp), omB = 1.0 - B;
+                    const double* gc = gk + n;
+                    auto G = [&](int i) { return gc[(size_t)i * N]; };
+                    // Omega from the recorded state (no porosity with partial cells)
+                    double rm1 = 0.0;
+                    for (int i = 0; i < 19; i++) rm1 += G(i);
+                    const double ux = Jv[0] / rho, uy = Jv[1] / rho, uz = Jv[2] / rho;
+                    const double u2 = ux * ux + uy * uy + uz * uz;
+                    const double uF = ux * Fb[0] + uy * Fb[1] + uz * Fb[2];
+                    double Om[19];
+                    Om[0] = wp * (weight<double>(0) * (rm1 - 1.5 * rho * u2) - G(0))
+                          + weight<double>(0) * ap * (-3.0 * uF);
+                    for (int i = 1; i < 19; i += 2) {
+                        const double W = weight<double>(i);
+                        const double cu = 3.0 * (CX[i] * ux + CY[i] * uy + CZ[i] * uz);
+                        const double cF = CX[i] * Fb[0] + CY[i] * Fb[1] + CZ[i] * Fb[2];
+                        const double feqp = W * (rm1 + rho * (0.5 * cu * cu - 1.5 * u2));
+                        const double feqm = W * rho * cu;
+                        const double dp = wp * (feqp - 0.5 * (G(i) + G(i + 1)));
+                        const double dm = wm * (feqm - 0.5 * (G(i) - G(i + 1)));
+                        const double sm = W * am * (3.0 * cF);
+                        const double sp = W * ap * (3.0 * cu * cF - 3.0 * uF);
+                        Om[i] = dp + dm + sp + sm;
+                        Om[i + 1] = dp - dm + sp - sm;
+                    }
+                    double gB = 0.0, dNt = 0.0;
+                    for (int i = 0; i < 19; i++) gB -= mu[i] * Om[i];
+                    for (int i = 1; i < 19; i++) gB += mu[i] * (G(OPP[i]) - G(i));
+                    for (int i = 1; i < 19; i += 2) dNt -= 2.0 * CX[i] * (G(i + 1) - G(i));
+                    const double dObjNt = wobj * o.w_f * 2.0 * B;     // d ntF_x / d g_j = 2 B c_jx
+                    for (int j = 0; j < 19; j++) {
+                        const double mp = j == 0 ? mu[0] : 0.5 * (mu[j] + mu[OPP[j]]);
+                        const double mm = j == 0 ? 0.0 : 0.5 * (mu[j] - mu[OPP[j]]);
+                        const double om = -wp * mp - wm * mm + gRho + CX[j] * gP[0] + CY[j] * gP[1] + CZ[j] * gP[2];
+                        const double bounce = j == 0 ? 0.0 : B * (mu[OPP[j]] - mu[j]);
+                        lam[j * N + n] = mu[j] + omB * om + bounce + dObjRho + (dObjPx + dObjNt) * CX[j];
+                    }
+                    ge[n] += (gB + wobj * o.w_f * dNt) * ntdB(e, wp);
                 }
                 const double gJdotJ = gJ[0] * Jv[0] + gJ[1] * Jv[1] + gJ[2] * Jv[2];
                 const double gFdotJ = gF[0] * Jv[0] + gF[1] * Jv[1] + gF[2] * Jv[2];
@@ -887,6 +958,7 @@ template <class S> struct Lattice final : Base {
         double gu = 0.0;
         for (size_t n = 0; n < N; n++) gu += guc[n];
         if (gAlpha) for (size_t n = 0; n < N; n++) gAlpha[n] = ga[n];
+        if (gEps) for (size_t n = 0; n < N; n++) gEps[n] = ge[n];
         if (gUin) *gUin = gu;
         return J;
     }
@@ -929,9 +1001,9 @@ void orc_set_solid_fractions(void* h, const float* eps) { ((Base*)h)->setEps(eps
 void orc_set_porosity(void* h, const float* alpha) { ((Base*)h)->setAlpha(alpha); }
 // FP64 lattices only (NaN otherwise, or with LES / sponge / NT cells).
 double orc_objective(void* h, const Objective* o) { return ((Base*)h)->objective(*o); }
-double orc_vjp(void* h, const Objective* o, double* grad_alpha, double* grad_uin,
+double orc_vjp(void* h, const Objective* o, double* grad_alpha, double* grad_eps, double* grad_uin,
                int64_t checkpoint_every) {
-    return ((Base*)h)->vjp(*o, grad_alpha, grad_uin, checkpoint_every);
+    return ((Base*)h)->vjp(*o, grad_alpha, grad_eps, grad_uin, checkpoint_every);
 }
 void orc_init_field(void* h, int mode, double amplitude) { ((Base*)h)->initField(mode, amplitude); }
 void orc_run(void* h, int64_t steps) { ((Base*)h)->run(steps, -1); }

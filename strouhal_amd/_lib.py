@@ -56,6 +56,7 @@ class Objective(ctypes.Structure):
         ("inlet_x", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
         ("w_p", ctypes.c_double), ("w_d", ctypes.c_double),
         ("spinup", ctypes.c_int64), ("avg", ctypes.c_int64),
+        ("w_f", ctypes.c_double),
     ]
 
 
@@ -137,7 +138,7 @@ def lib():
         "strh_batch_set_graph_steps": [vp, i32],
         "strh_batch_gpu_seconds": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_objective_eval": [vp, ctypes.POINTER(Objective), ctypes.POINTER(ctypes.c_double)],
-        "strh_vjp": [vp, ctypes.POINTER(Objective), i64, ctypes.POINTER(ctypes.c_double), vp, vp,
+        "strh_vjp": [vp, ctypes.POINTER(Objective), i64, ctypes.POINTER(ctypes.c_double), vp, vp, vp,
                      ctypes.POINTER(VjpInfo)],
     }
     for name, args in sig.items():

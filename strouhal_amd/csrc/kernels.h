@@ -64,6 +64,11 @@ struct AdjHooks {
     int32_t inletX, outletX; // columns of the pressure-drop term
     double cin, cout;        // w_p / (3 n_in T), w_p / (3 n_out T)
     double wd;               // w_d / T
+    // partial cells (lattices with solid fractions)
+    const int32_t* ntIdx;    // [N] index of the cell among the M cells with eps > 0, or -1
+    float* recNt;            // [19][M] pre-collision populations of those cells, or null
+    uint32_t M;
+    double wf;               // w_f / T (partial-cell drag)
 };
 
 // One reverse step: lambda^t from lambda^{t+1} and the recorded moments of g^t.
@@ -76,15 +81,20 @@ struct AdjStep {
     int32_t inflowUniform;
     int32_t inletX, outletX;     // objective of step t (weights 0 outside the window)
     float cin, cout, wd;
+    float wf;                    // partial-cell drag weight of step t
+    uint32_t M;                  // cells with a solid fraction
 };
 
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);
 // Forward step with the adjoint hooks; L.alpha must be set (zeros for a
-// lattice without porosity).
+// lattice without porosity) unless the lattice has solid fractions (L.nt).
 hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s);
+// Partial-cell lattices pass eps, ntIdx, recNt (this step's [19][M]) and gEps ([M]).
 hipError_t launch_adjoint_step(const AdjStep& q, bool walls, const float4* rec, const float* lamNext,
                                float* lam, const uint8_t* flags, const uint32_t* link,
-                               const float* alpha, double* gAlpha, double* gUin, hipStream_t s);
+                               const float* alpha, double* gAlpha, double* gUin,
+                               const float* eps, const int32_t* ntIdx, const float* recNt,
+                               double* gEps, hipStream_t s);
 hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,
                               double* out, hipStream_t s);
 hipError_t launch_batch_step(int precision, int bs, const BatchLat* lats, int L,

@@ -371,7 +371,20 @@ constexpr uint32_t kStepBS = 256;
 //   cp(x) rho + wd alpha J_x,   J_x = (p_x + F_x / 2) / (1 + alpha / 2),
 // with cp = +cin on the inlet column and -cout on the outlet column.
 __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[19], uint32_t n,
-                                          int x, float alpha, float fx) {
+                                          int x, float alpha, float eps, float omega, float fx) {
+    if (hk.recNt && eps > 0.0f) {
+        const int32_t k = hk.ntIdx[n];
+#pragma unroll
+        for (int i = 0; i < 19; i++) hk.recNt[(size_t)i * hk.M + k] = fh[i];
+    }
+    if (hk.jacc && hk.wf != 0.0 && eps > 0.0f) {   // partial-cell drag ntF_x = -2 sum c_x B (g_opp - g)
+        const double tw = 1.0 / (double)omega - 0.5, e = (double)eps;
+        const double B = e * tw / ((1.0 - e) + tw);
+        double nt = 0.0;
+#pragma unroll
+        for (int i = 1; i < 19; i += 2) nt -= 2.0 * (double)kCx[i] * B * ((double)fh[i + 1] - (double)fh[i]);
+        hk.jacc[n] += hk.wf * nt;
+    }
     if (!hk.rec && !hk.jacc) return;
     double rm1 = 0.0, px = 0.0, py = 0.0, pz = 0.0;
 #pragma unroll
@@ -429,7 +442,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 #pragma unroll
         for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nb);
         const float alpha = BRINK ? alphaBuf[n] : 0.0f;
-        if (HOOK) run_hooks(*hk, fh, n, x, alpha, p.fx);
+        if (HOOK) run_hooks(*hk, fh, n, x, alpha, eps, p.omega, p.fx);
         collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);
         // A separate copy of the offset keeps the stores on the scalar-base
         // form too (otherwise the shared base+offset sum is kept as 19
@@ -520,7 +533,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     }
 
     const float alpha = BRINK ? alphaBuf[n] : 0.0f;
-    if (HOOK) run_hooks(*hk, fh, n, x, alpha, p.fx);
+    if (HOOK) run_hooks(*hk, fh, n, x, alpha, eps, p.omega, p.fx);
     collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);  // 315-316
     if (NT) {
         ax += ntx;
@@ -588,15 +601,17 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
 // Forward step of the differentiable configuration (Guo forcing, Brinkman
 // porosity) with the adjoint hooks. The arithmetic is that of step_kernel:
 // with alpha = 0 and no body force the DDFs are the plain update's bits.
-template <class St, int PARITY, bool WALLS>
+template <class St, int PARITY, bool WALLS, bool NT>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                    const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
-                   StepParams p, const float* __restrict__ alphaBuf, AdjHooks hk) {
+                   const float* __restrict__ epsBuf, StepParams p, const float* __restrict__ alphaBuf,
+                   AdjHooks hk) {
     const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * kStepBS + threadIdx.x;
     if (n >= p.N) return;
-    step_cell<St, PARITY, WALLS, false, false, true, true, true>(f, flags, link, lidm, nullptr,
-                                                                 nullptr, p, n, alphaBuf, &hk);
+    // partial cells and porosity exclude each other
+    step_cell<St, PARITY, WALLS, NT, false, true, !NT, true>(f, flags, link, lidm, nullptr, epsBuf, p, n,
+                                                             alphaBuf, &hk);
 }
 
 // ---- discrete adjoint: one reverse step ------------------------------------
@@ -624,12 +639,14 @@ template <int I> __device__ __forceinline__ float wgt_f() {
     return I == 0 ? 1.0f / 3.0f : (I <= 6 ? 1.0f / 18.0f : 1.0f / 36.0f);
 }
 
-template <bool WALLS>
+template <bool WALLS, bool NT>
 __global__ void __launch_bounds__(256)
 adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ lamN,
                     float* __restrict__ lam, const uint8_t* __restrict__ flags,
                     const uint32_t* __restrict__ link, const float* __restrict__ alphaBuf,
-                    double* __restrict__ gAlpha, double* __restrict__ gUin, AdjStep q) {
+                    double* __restrict__ gAlpha, double* __restrict__ gUin,
+                    const float* __restrict__ epsBuf, const int32_t* __restrict__ ntIdx,
+                    const float* __restrict__ recNt, double* __restrict__ gEps, AdjStep q) {
     const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
     if (n >= q.N) return;
     if (flags[n] != kFluid) return;
@@ -729,6 +746,73 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     const float wd = q.wd;
     const float dPx = gPx + wd * a * ig;
     const float base = gRho + cp;
+    if (NT) {
+        const int32_t k = ntIdx[n];
+        if (k >= 0) {
+            // Noble-Torczynski cell (Kernels.metal:201-227):
+            //   f*_j = f_j + (1 - B) Omega_j(f) + B (f_opp(j) - f_j),  f*_0 = f_0 + (1 - B) Omega_0,
+            // so lambda_j = mu_j + (1 - B) [Omega^T mu]_j + B (mu_opp(j) - mu_j) + objective,
+            // dJ/dB = -mu.Omega + sum_i mu_i (g_opp(i) - g_i) + w_f d(ntF_x)/dB.
+            const uint32_t M = q.M;
+            float g[19];
+#pragma unroll
+            for (int i = 0; i < 19; i++) g[i] = recNt[(size_t)i * M + k];
+            const float e = epsBuf[n];
+            const float tw = 1.0f / wp - 0.5f;
+            const float den = (1.0f - e) + tw;
+            const float B = e * tw / den, omB = 1.0f - B;
+            const float dBde = tw * (1.0f + tw) / (den * den);
+            const float ux = Jx * ir, uy = Jy * ir, uz = Jz * ir;
+            const float u2 = ux * ux + uy * uy + uz * uz;
+            const float uF = ux * fx + uy * fy + uz * fz;
+            const float rm1 = m.x;
+            float Om[19];
+            Om[0] = wp * ((1.0f / 3.0f) * (rm1 - 1.5f * rho * u2) - g[0]) + (1.0f / 3.0f) * ap * (-3.0f * uF);
+#define STRH_OMEGA(I)                                                                         \
+            {                                                                                 \
+                const float W = wgt_f<I>();                                                   \
+                const float cu = 3.0f * ((float)kCx[I] * ux + (float)kCy[I] * uy + (float)kCz[I] * uz); \
+                const float cF = (float)kCx[I] * fx + (float)kCy[I] * fy + (float)kCz[I] * fz; \
+                const float dp = wp * (W * (rm1 + rho * (0.5f * cu * cu - 1.5f * u2)) - 0.5f * (g[I] + g[I + 1])); \
+                const float dm = wm * (W * rho * cu - 0.5f * (g[I] - g[I + 1]));              \
+                const float sm = W * am * (3.0f * cF);                                        \
+                const float sp = W * ap * (3.0f * cu * cF - 3.0f * uF);                       \
+                Om[I] = dp + dm + sp + sm;                                                    \
+                Om[I + 1] = dp - dm + sp - sm;                                                \
+            }
+            STRH_OMEGA(1) STRH_OMEGA(3) STRH_OMEGA(5) STRH_OMEGA(7) STRH_OMEGA(9)
+            STRH_OMEGA(11) STRH_OMEGA(13) STRH_OMEGA(15) STRH_OMEGA(17)
+#undef STRH_OMEGA
+            float gB = 0.0f, dNt = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 19; i++) gB -= mu[i] * Om[i];
+#pragma unroll
+            for (int i = 1; i < 19; i++) gB += mu[i] * (g[kOpp[i]] - g[i]);
+#pragma unroll
+            for (int i = 1; i < 19; i += 2) if (kCx[i]) dNt -= 2.0f * (float)kCx[i] * (g[i + 1] - g[i]);
+            const float dObjX = q.wf * 2.0f * B;          // d ntF_x / d g_j = 2 B c_jx
+            lam[n] = mu[0] + omB * (-wp * mu[0] + gRho) + cp;
+#define STRH_NTLAM(I)                                                                         \
+            {                                                                                 \
+                const float mp = 0.5f * (mu[I] + mu[kOpp[I]]);                                \
+                const float mm = 0.5f * (mu[I] - mu[kOpp[I]]);                                \
+                float om = -wp * mp - wm * mm + gRho;                                         \
+                if (kCx[I]) om += (float)kCx[I] * gPx;                                        \
+                if (kCy[I]) om += (float)kCy[I] * gPy;                                        \
+                if (kCz[I]) om += (float)kCz[I] * gPz;                                        \
+                float v = mu[I] + omB * om + B * (mu[kOpp[I]] - mu[I]) + cp;                  \
+                if (kCx[I]) v += (float)kCx[I] * dObjX;                                       \
+                lam[I * S + n] = v;                                                           \
+            }
+            STRH_NTLAM(1) STRH_NTLAM(2) STRH_NTLAM(3) STRH_NTLAM(4) STRH_NTLAM(5) STRH_NTLAM(6)
+            STRH_NTLAM(7) STRH_NTLAM(8) STRH_NTLAM(9) STRH_NTLAM(10) STRH_NTLAM(11) STRH_NTLAM(12)
+            STRH_NTLAM(13) STRH_NTLAM(14) STRH_NTLAM(15) STRH_NTLAM(16) STRH_NTLAM(17) STRH_NTLAM(18)
+#undef STRH_NTLAM
+            gEps[k] += (double)((gB + q.wf * dNt) * dBde);
+            if (gu != 0.0) gUin[n] += gu;
+            return;
+        }
+    }
     lam[n] = mu[0] - wp * mu[0] + base;
 #define STRH_LAM(I)                                                                   \
     {                                                                                 \
@@ -1187,8 +1271,12 @@ static hipError_t launch_hooked_t(const StepLaunch& L, const AdjHooks& hk, hipSt
     using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
     StepParams p = L.p;
     p.blocks = (p.N + kStepBS - 1) / kStepBS;
-    hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS>), dim3(p.blocks), dim3(kStepBS), 0, s,
-                       (T*)L.f, L.flags, L.link, L.lidm, p, L.alpha, hk);
+    if (L.nt)
+        hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, true>), dim3(p.blocks), dim3(kStepBS), 0, s,
+                           (T*)L.f, L.flags, L.link, L.lidm, L.eps, p, (const float*)nullptr, hk);
+    else
+        hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, false>), dim3(p.blocks), dim3(kStepBS), 0, s,
+                           (T*)L.f, L.flags, L.link, L.lidm, (const float*)nullptr, p, L.alpha, hk);
     return hipGetLastError();
 }
 
@@ -1200,7 +1288,7 @@ static hipError_t launch_hooked_c(const StepLaunch& L, const AdjHooks& hk, hipSt
 }
 
 hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
-    if (!L.alpha || L.nt || L.les) return hipErrorInvalidValue;
+    if (L.les || (L.nt == !!L.alpha)) return hipErrorInvalidValue;   // exactly one of eps, alpha
     switch (L.precision) {
         case kFP32: return launch_hooked_c<ConvF32>(L, hk, s);
         case kFP16S: return launch_hooked_c<ConvF16>(L, hk, s);
@@ -1211,15 +1299,20 @@ hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream
 
 hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float4* rec, const float* lamNext,
                                float* lam, const uint8_t* flags, const uint32_t* link,
-                               const float* alpha, double* gAlpha, double* gUin, hipStream_t s) {
+                               const float* alpha, double* gAlpha, double* gUin,
+                               const float* eps, const int32_t* ntIdx, const float* recNt,
+                               double* gEps, hipStream_t s) {
     AdjStep q = q0;
     q.blocks = (q.N + 255u) / 256u;
-    if (walls)
-        hipLaunchKernelGGL(adjoint_step_kernel<true>, dim3(q.blocks), dim3(256), 0, s, rec, lamNext,
-                           lam, flags, link, alpha, gAlpha, gUin, q);
-    else
-        hipLaunchKernelGGL(adjoint_step_kernel<false>, dim3(q.blocks), dim3(256), 0, s, rec, lamNext,
-                           lam, flags, link, alpha, gAlpha, gUin, q);
+    const dim3 g(q.blocks), b(256);
+#define STRH_ADJ(W, NTV) hipLaunchKernelGGL((adjoint_step_kernel<W, NTV>), g, b, 0, s, rec, lamNext, lam, \
+                                            flags, link, alpha, gAlpha, gUin, eps, ntIdx, recNt, gEps, q)
+    if (ntIdx) {
+        if (walls) STRH_ADJ(true, true); else STRH_ADJ(false, true);
+    } else {
+        if (walls) STRH_ADJ(true, false); else STRH_ADJ(false, false);
+    }
+#undef STRH_ADJ
     return hipGetLastError();
 }
 

@@ -1181,7 +1181,10 @@ struct AdjWork {
 
 struct AdjPlan {
     int64_t T = 0;
-    double cin = 0.0, cout = 0.0, wd = 0.0;   // weights of one window step
+    double cin = 0.0, cout = 0.0, wd = 0.0, wf = 0.0;   // weights of one window step
+    // partial cells: the fluid cells with eps > 0, numbered 0..M-1
+    std::vector<int32_t> ntIdx;
+    std::vector<uint32_t> ntCells;
 };
 
 int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
@@ -1189,13 +1192,24 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
     if (o->spinup < 0 || o->avg <= 0) return fail(STRH_ERR_INVALID, "objective needs spinup >= 0 and avg > 0");
     if (o->inlet_x < -1 || o->inlet_x >= L->nx || o->outlet_x < -1 || o->outlet_x >= L->nx)
         return fail(STRH_ERR_INVALID, "objective columns must lie in [-1, nx)");
-    if (!std::isfinite(o->w_p) || !std::isfinite(o->w_d)) return fail(STRH_ERR_INVALID, "objective weights must be finite");
-    if (L->useEps || L->cSmago > 0.0f || L->spongeOn)
-        return fail(STRH_ERR_UNSUPPORTED, "the adjoint covers walls, Guo forcing and Brinkman porosity; "
-                                          "partial cells, LES and the sponge are not differentiated");
+    if (!std::isfinite(o->w_p) || !std::isfinite(o->w_d) || !std::isfinite(o->w_f))
+        return fail(STRH_ERR_INVALID, "objective weights must be finite");
+    if (L->cSmago > 0.0f || L->spongeOn)
+        return fail(STRH_ERR_UNSUPPORTED, "the adjoint covers walls, Guo forcing, Brinkman porosity and "
+                                          "partial cells; LES and the sponge are not differentiated");
     P.T = o->spinup + o->avg;
     std::vector<uint8_t> fl(L->N);
     HIP_TRY(hipMemcpy(fl.data(), L->flags, L->N, hipMemcpyDeviceToHost), "copy flags");
+    if (L->useEps) {
+        std::vector<float> e(L->N);
+        HIP_TRY(hipMemcpy(e.data(), L->eps, (size_t)L->N * 4, hipMemcpyDeviceToHost), "copy eps");
+        P.ntIdx.assign(L->N, -1);
+        for (uint32_t n = 0; n < L->N; n++)
+            if (fl[n] == kFluid && e[n] > 0.0f) {
+                P.ntIdx[n] = (int32_t)P.ntCells.size();
+                P.ntCells.push_back(n);
+            }
+    }
     int64_t nin = 0, nout = 0;
     for (uint32_t n = 0; n < L->N; n++) {
         if (fl[n] != kFluid) continue;
@@ -1207,19 +1221,24 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
     P.cin = nin ? o->w_p * inv / (3.0 * (double)nin) : 0.0;
     P.cout = nout ? o->w_p * inv / (3.0 * (double)nout) : 0.0;
     P.wd = o->w_d * inv;
+    P.wf = o->w_f * inv;
     return STRH_OK;
 }
 
 // Hooked forward steps k0 .. k0+count-1 of the objective window on buffer f
 // (absolute steps t0 + k): pre-collision moments recorded into rec (count * N
 // float4) when given, objective accumulated into jacc when given.
+// Partial-cell lattices also record the 19 populations of their M cells with
+// eps > 0 into recNt (count * 19 * M floats) when given.
 int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_t k0, int64_t count,
-                const strh_objective* o, const AdjPlan& P, float4* rec, double* jacc) {
+                const strh_objective* o, const AdjPlan& P, float4* rec, double* jacc,
+                const int32_t* ntIdx = nullptr, float* recNt = nullptr) {
+    const uint32_t M = (uint32_t)P.ntCells.size();
     for (int64_t s = 0; s < count; s++) {
         const int64_t k = k0 + s;
         StepLaunch sl = make_launch(L, t0 + k, false);
         sl.f = f;
-        sl.alpha = alpha;
+        sl.alpha = L->useEps ? nullptr : alpha;
         AdjHooks hk;
         hk.rec = rec ? rec + (size_t)s * L->N : nullptr;
         const bool win = k >= o->spinup;
@@ -1229,6 +1248,10 @@ int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_
         hk.cin = win ? P.cin : 0.0;
         hk.cout = win ? P.cout : 0.0;
         hk.wd = win ? P.wd : 0.0;
+        hk.ntIdx = ntIdx;
+        hk.recNt = (recNt && ntIdx) ? recNt + (size_t)s * 19 * M : nullptr;
+        hk.M = M;
+        hk.wf = win ? P.wf : 0.0;
         HIP_TRY(launch_step_hooked(sl, hk, L->ctx->stream), "hooked step launch");
     }
     return STRH_OK;
@@ -1263,14 +1286,19 @@ int strh_objective_eval(strh_lattice* L, const strh_objective* o, double* J) {
     if ((rc = W.get(&jacc, (size_t)L->N * 8, "objective accumulator"))) return rc;
     HIP_TRY(launch_fill_zero(jacc, (size_t)L->N * 8, L->ctx->stream), "clear");
     const float* alpha = L->alpha;
-    if (!alpha) {
+    if (!alpha && !L->useEps) {
         if ((rc = W.get(&zeroAlpha, (size_t)L->N * 4, "zero porosity"))) return rc;
         HIP_TRY(launch_fill_zero(zeroAlpha, (size_t)L->N * 4, L->ctx->stream), "clear");
         alpha = zeroAlpha;
     }
+    int32_t* ntIdx = nullptr;
+    if (L->useEps) {
+        if ((rc = W.get(&ntIdx, (size_t)L->N * 4, "partial-cell index"))) return rc;
+        HIP_TRY(hipMemcpy(ntIdx, P.ntIdx.data(), (size_t)L->N * 4, hipMemcpyHostToDevice), "copy index");
+    }
     strh_ctx* c = L->ctx;
     HIP_TRY(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
-    if ((rc = adj_forward(L, L->f, alpha, L->stepsDone, 0, P.T, o, P, nullptr, jacc))) return rc;
+    if ((rc = adj_forward(L, L->f, alpha, L->stepsDone, 0, P.T, o, P, nullptr, jacc, ntIdx, nullptr))) return rc;
     HIP_TRY(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
     HIP_TRY(hipEventSynchronize(c->ev1), "objective steps");
     L->gpuSeconds += elapsed(c);
@@ -1281,7 +1309,7 @@ int strh_objective_eval(strh_lattice* L, const strh_objective* o, double* J) {
 }
 
 int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, double* gradAlpha,
-             double* gradUin, strh_vjp_info* info) {
+             double* gradEps, double* gradUin, strh_vjp_info* info) {
     int rc = check_lat(L);
     if (rc) return rc;
     if (!J) return fail(STRH_ERR_INVALID, "null J");
@@ -1290,6 +1318,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     const int64_t T = P.T;
     const size_t N = L->N, S = L->S;
     const size_t ddf = ddf_bytes(L);
+    const size_t M = P.ntCells.size();
     if (K < -1) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= -1");
     if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
     if (K == -1) {
@@ -1299,7 +1328,8 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
         const double fixed = (double)N * (3 * 8 + 4) + 2.0 * 19 * S * 4 + 2.0 * ddf;
         const double budget = 0.5 * (double)freeB - fixed;
-        const double perStep = (double)N * 16 + (double)ddf / 64.0;   // recording + its share of checkpoints
+        // recording (moments, partial-cell populations) + its share of checkpoints
+        const double perStep = (double)N * 16 + (double)M * 76 + (double)ddf / 64.0;
         K = budget > perStep ? (int64_t)(budget / perStep) : 1;
     }
     if (K > T) K = T;
@@ -1321,10 +1351,21 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     if ((rc = W.get(&ck, (size_t)(C > 1 ? C - 1 : 1) * ddf, "checkpoints"))) return rc;
     if ((rc = W.get(&work, ddf, "replay DDFs"))) return rc;
     const float* alpha = L->alpha;
-    if (!alpha) {
+    if (!alpha && !L->useEps) {
         if ((rc = W.get(&zeroAlpha, N * 4, "zero porosity"))) return rc;
         HIP_TRY(launch_fill_zero(zeroAlpha, N * 4, st), "clear");
         alpha = zeroAlpha;
+    }
+    int32_t* ntIdx = nullptr;
+    float* recNt = nullptr;
+    double* gE = nullptr;
+    if (L->useEps) {
+        const size_t m = std::max<size_t>(M, 1);
+        if ((rc = W.get(&ntIdx, N * 4, "partial-cell index"))) return rc;
+        if ((rc = W.get(&recNt, (size_t)K * 19 * m * 4, "recorded partial cells"))) return rc;
+        if ((rc = W.get(&gE, m * 8, "solid-fraction gradient"))) return rc;
+        HIP_TRY(hipMemcpy(ntIdx, P.ntIdx.data(), N * 4, hipMemcpyHostToDevice), "copy index");
+        HIP_TRY(launch_fill_zero(gE, m * 8, st), "clear");
     }
     for (void* p : {(void*)jacc, (void*)gA, (void*)gU})
         HIP_TRY(launch_fill_zero(p, N * 8, st), "clear");
@@ -1340,7 +1381,8 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         const bool last = seg == C - 1;
         if (!last)
             HIP_TRY(hipMemcpyAsync(ck + (size_t)seg * ddf, L->f, ddf, hipMemcpyDeviceToDevice, st), "checkpoint");
-        if ((rc = adj_forward(L, L->f, alpha, t0, k0, len, o, P, last ? rec : nullptr, jacc))) return rc;
+        if ((rc = adj_forward(L, L->f, alpha, t0, k0, len, o, P, last ? rec : nullptr, jacc, ntIdx,
+                              last ? recNt : nullptr))) return rc;
     }
     HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
     HIP_TRY(hipEventSynchronize(c->ev1), "forward pass");
@@ -1356,7 +1398,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         if (seg != C - 1) {
             HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
             HIP_TRY(hipMemcpyAsync(work, ck + (size_t)seg * ddf, ddf, hipMemcpyDeviceToDevice, st), "restore");
-            if ((rc = adj_forward(L, work, alpha, t0, k0, len, o, P, rec, nullptr))) return rc;
+            if ((rc = adj_forward(L, work, alpha, t0, k0, len, o, P, rec, nullptr, ntIdx, recNt))) return rc;
             HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
             HIP_TRY(hipEventSynchronize(c->ev1), "replay");
             tr += elapsed(c);
@@ -1380,8 +1422,12 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
             q.cin = win ? (float)P.cin : 0.0f;
             q.cout = win ? (float)P.cout : 0.0f;
             q.wd = win ? (float)P.wd : 0.0f;
+            q.wf = win ? (float)P.wf : 0.0f;
+            q.M = (uint32_t)M;
             HIP_TRY(launch_adjoint_step(q, L->walls, rec + (size_t)(k - k0) * N, lamN, lamT,
-                                        L->flags, L->link, alpha, gA, gU, st), "adjoint step launch");
+                                        L->flags, L->link, alpha, gA, gU, L->useEps ? L->eps : nullptr,
+                                        ntIdx, recNt ? recNt + (size_t)(k - k0) * 19 * M : nullptr, gE, st),
+                    "adjoint step launch");
             std::swap(lamN, lamT);
         }
         HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
@@ -1391,6 +1437,14 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     if ((rc = reduce_to_host(L, jacc, J))) return rc;
     if (gradUin && (rc = reduce_to_host(L, gU, gradUin))) return rc;
     if (gradAlpha) HIP_TRY(hipMemcpy(gradAlpha, gA, N * 8, hipMemcpyDeviceToHost), "copy gradient");
+    if (gradEps) {
+        std::fill(gradEps, gradEps + N, 0.0);
+        if (M) {
+            std::vector<double> ge(M);
+            HIP_TRY(hipMemcpy(ge.data(), gE, M * 8, hipMemcpyDeviceToHost), "copy gradient");
+            for (size_t k = 0; k < M; k++) gradEps[P.ntCells[k]] = ge[k];
+        }
+    }
     if (info) {
         info->checkpoint_every = K;
         info->checkpoints = C - 1;

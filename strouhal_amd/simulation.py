@@ -278,37 +278,43 @@ class Simulation:
 
     # ---- differentiation (no reference counterpart) ----------------------
     @staticmethod
-    def _objective(inlet_x, outlet_x, spinup, avg, w_p, w_d):
+    def _objective(inlet_x, outlet_x, spinup, avg, w_p, w_d, w_f=0.0):
         from ._lib import Objective
         o = Objective()
         o.inlet_x, o.outlet_x = int(inlet_x), int(outlet_x)
-        o.w_p, o.w_d = float(w_p), float(w_d)
+        o.w_p, o.w_d, o.w_f = float(w_p), float(w_d), float(w_f)
         o.spinup, o.avg = int(spinup), int(avg)
         return o
 
     def objective(self, inlet_x: int, outlet_x: int, spinup: int, avg: int,
-                  w_p: float = 1.0, w_d: float = 1.0) -> float:
+                  w_p: float = 1.0, w_d: float = 1.0, w_f: float = 0.0) -> float:
         """Advance spinup + avg steps; return the time average over the last avg
-        steps of w_p (rho_in - rho_out)/3 + w_d sum alpha J_x (FP64)."""
-        o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d)
+        steps of w_p (rho_in - rho_out)/3 + w_d sum alpha J_x + w_f sum ntF_x (FP64)."""
+        o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d, w_f)
         J = ctypes.c_double()
         check(lib().strh_objective_eval(self._h, ctypes.byref(o), ctypes.byref(J)))
         return J.value
 
     def vjp(self, inlet_x: int, outlet_x: int, spinup: int, avg: int, w_p: float = 1.0,
-            w_d: float = 1.0, checkpoint_every: int = -1, want_alpha: bool = True):
+            w_d: float = 1.0, checkpoint_every: int = -1, want_alpha: bool = True,
+            w_f: float = 0.0, want_eps: bool = False):
         """Discrete adjoint of `objective`: returns (J, dJ/dalpha per cell as
-        float64 (nz, ny, nx) or None, dJ/d uin_target, info dict). Leaves the
-        lattice as `run(spinup + avg)` would. checkpoint_every: steps per
-        replayed segment; 0 = ceil(sqrt(T)), -1 = as long as memory allows."""
+        float64 (nz, ny, nx) or None, dJ/d uin_target, info dict); with
+        want_eps, info["grad_eps"] holds dJ/deps (nz, ny, nx) on the cells with
+        a solid fraction. Leaves the lattice as `run(spinup + avg)` would.
+        checkpoint_every: steps per replayed segment; 0 = ceil(sqrt(T)),
+        -1 = as long as memory allows."""
         from ._lib import VjpInfo
-        o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d)
+        o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d, w_f)
         J, gu, info = ctypes.c_double(), ctypes.c_double(), VjpInfo()
         ga = np.zeros(self.cells, dtype=np.float64) if want_alpha else None
+        ge = np.zeros(self.cells, dtype=np.float64) if want_eps else None
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
         check(lib().strh_vjp(self._h, ctypes.byref(o), int(checkpoint_every), ctypes.byref(J),
-                             ga.ctypes.data_as(ctypes.c_void_p) if want_alpha else None,
-                             ctypes.byref(gu), ctypes.byref(info)))
+                             ptr(ga), ptr(ge), ctypes.byref(gu), ctypes.byref(info)))
         meta = {k: getattr(info, k) for k, _ in VjpInfo._fields_}
+        if want_eps:
+            meta["grad_eps"] = ge.reshape(self.nz, self.ny, self.nx)
         return (J.value, ga.reshape(self.nz, self.ny, self.nx) if want_alpha else None,
                 gu.value, meta)
 

@@ -183,13 +183,15 @@ public final class Simulation {                   // Sim.swift:153-459
         public var inletX: Int32, outletX: Int32
         public var wP: Double, wD: Double
         public var spinup: Int, avg: Int
-        public init(inletX: Int32, outletX: Int32, wP: Double = 1, wD: Double = 1, spinup: Int, avg: Int) {
+        public var wF: Double
+        public init(inletX: Int32, outletX: Int32, wP: Double = 1, wD: Double = 1, spinup: Int, avg: Int,
+                    wF: Double = 0) {
             self.inletX = inletX; self.outletX = outletX; self.wP = wP; self.wD = wD
-            self.spinup = spinup; self.avg = avg
+            self.spinup = spinup; self.avg = avg; self.wF = wF
         }
         var c: strh_objective {
             strh_objective(inlet_x: inletX, outlet_x: outletX, w_p: wP, w_d: wD,
-                           spinup: Int64(spinup), avg: Int64(avg))
+                           spinup: Int64(spinup), avg: Int64(avg), w_f: wF)
         }
     }
 
@@ -200,16 +202,17 @@ public final class Simulation {                   // Sim.swift:153-459
         return j
     }
 
-    /// J, dJ/dalpha per cell and dJ/d(uin target); checkpointEvery 0 = sqrt(T),
-    /// -1 = as long as the recording fits half the free device memory.
+    /// J, dJ/dalpha and dJ/deps per cell and dJ/d(uin target); checkpointEvery
+    /// 0 = sqrt(T), -1 = as long as the recording fits half the free device memory.
     public func vjp(_ o: Objective, checkpointEvery: Int = -1) throws
-        -> (j: Double, gradAlpha: [Double], gradUin: Double) {
+        -> (j: Double, gradAlpha: [Double], gradEps: [Double], gradUin: Double) {
         var oc = o.c
         var j = 0.0, gu = 0.0
         var ga = [Double](repeating: 0, count: cells)
+        var ge = [Double](repeating: 0, count: cells)
         var info = strh_vjp_info()
-        try check(strh_vjp(lat, &oc, Int64(checkpointEvery), &j, &ga, &gu, &info))
-        return (j, ga, gu)
+        try check(strh_vjp(lat, &oc, Int64(checkpointEvery), &j, &ga, &ge, &gu, &info))
+        return (j, ga, ge, gu)
     }
 }
 

@@ -174,3 +174,54 @@ def test_vjp_refuses_unsupported_lattices(gpu):
     with pytest.raises(StrouhalError):
         g.vjp(**dict(OBJ, avg=0))
     assert g.steps_done == 0
+
+
+def _nt_pair(gpu, precision, nx=20, ny=12, nz=1, sphere=False):
+    from strouhal_amd import Simulation, trt_omegas
+    from strouhal_amd.cases import disk_solid_fractions, sphere_solid_fractions
+    fl = _channel(nx, ny, nz)
+    if sphere:
+        eps = sphere_solid_fractions(nx, ny, nz, 8.3, ny / 2 - 0.3, nz / 2 + 0.2, 2.7).astype(np.float32)
+    else:
+        eps = np.repeat(disk_solid_fractions(nx, ny, 8.3, 5.7, 2.6)[None], nz, axis=0).astype(np.float32)
+    wp, wm = trt_omegas(0.7, 3.0 / 16.0)
+    g = Simulation(gpu, precision, nx=nx, ny=ny, nz=nz, omega=wp, omega_minus=wm, uin=0.05,
+                   ramp_steps=4, force=(5e-6, 0.0, 0.0), flags=fl)
+    g.set_solid_fractions(eps)
+    g.init_field(0, 0.0)
+    r = OracleSim(nx, ny, nz, g.omega, g.omega_minus, arith="f64", storage="f64", uin=g.uin_target,
+                  ramp_steps=g.ramp_steps, force=g.force, flags=fl, threads=8)
+    r.set_solid_fractions(eps)
+    r.init_field(0, 0.0)
+    return g, r, eps
+
+
+NT_OBJ = dict(inlet_x=2, outlet_x=17, spinup=5, avg=10, w_p=1.0, w_d=0.0, w_f=3.0)
+
+
+@pytest.mark.parametrize("sphere", [False, True])
+def test_vjp_solid_fractions_match_fp64_oracle(gpu, sphere):
+    """dJ/deps (pressure drop + partial-cell drag) on a Noble-Torczynski disk
+    in 2-D and a sphere in 3-D, against the FP64 oracle adjoint."""
+    nz = 9 if sphere else 1
+    g, r, eps = _nt_pair(gpu, "fp32", nz=nz, sphere=sphere)
+    J, _, gu, info = g.vjp(**NT_OBJ, checkpoint_every=4, want_eps=True)
+    Jr, _, ger, gur = r.vjp(**NT_OBJ, with_eps=True)
+    ge = info["grad_eps"].reshape(-1)
+    assert J == pytest.approx(Jr, rel=1e-6)
+    _close(ge, ger, 1e-4)
+    assert gu == pytest.approx(gur, rel=1e-4)
+    assert np.all(ge[eps.reshape(-1) == 0] == 0.0)
+    assert np.count_nonzero(ge) > 10
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16s"])
+def test_nt_recording_forward_keeps_the_state_and_interval(gpu, precision):
+    a, _, _ = _nt_pair(gpu, precision)
+    b, _, _ = _nt_pair(gpu, precision)
+    c, _, _ = _nt_pair(gpu, precision)
+    a.run(15)
+    J1, _, u1, i1 = b.vjp(**NT_OBJ, checkpoint_every=0, want_eps=True)
+    J2, _, u2, i2 = c.vjp(**NT_OBJ, checkpoint_every=-1, want_eps=True)
+    assert a.state_digest == b.state_digest == c.state_digest
+    assert J1 == J2 and u1 == u2 and np.array_equal(i1["grad_eps"], i2["grad_eps"])

@@ -179,3 +179,70 @@ def test_adjoint_rejects_unsupported_features():
     s.c_smago = 0.01
     J, ga, gu = s.vjp(**OBJ)
     assert math.isnan(J)
+
+
+def _nt_case(seed=0):
+    """Channel with a Noble-Torczynski disk: partial cells on its rim, solid core."""
+    from strouhal_amd.cases import disk_solid_fractions
+    nx, ny = 20, 12
+    eps = disk_solid_fractions(nx, ny, 8.3, 5.7, 2.6).astype(np.float32)
+    fl = _channel(nx, ny)
+
+    def make(e=eps, u=0.05):
+        tau = 0.7
+        wp = 1.0 / tau
+        wm = 1.0 / (0.5 + 0.1875 / (tau - 0.5))
+        s = OracleSim(nx, ny, 1, wp, wm, arith="f64", storage="f64", uin=u, ramp_steps=4,
+                      force=(5e-6, 0.0, 0.0), flags=fl)
+        s.set_solid_fractions(e)
+        s.init_field(0, 0.0)
+        return s
+    return make, eps
+
+
+NT_OBJ = dict(inlet_x=2, outlet_x=17, spinup=5, avg=10, w_p=1.0, w_d=0.0, w_f=3.0)
+
+
+def test_adjoint_solid_fraction_gradient_matches_central_differences():
+    """dJ/deps of the pressure drop plus the partial-cell drag, on rim cells
+    (0 < eps < 1) and core cells (eps = 1), against central differences."""
+    make, eps = _nt_case()
+    J, ga, ge, gu = make().vjp(**NT_OBJ, with_eps=True)
+    assert J == make().objective(**NT_OBJ)
+    ny, nx = eps.shape
+    flat = eps.reshape(-1)
+    rim = [k for k in range(flat.size) if 0.0 < flat[k] < 1.0]
+    core = [k for k in range(flat.size) if flat[k] == 1.0]
+    assert len(rim) > 8 and len(core) > 4
+    cells = rim[::max(1, len(rim) // 6)][:6] + core[:2]
+    h = 1e-4
+    for k in cells:
+        ep, em = flat.copy(), flat.copy()
+        ep[k] = np.float32(flat[k] + h)
+        em[k] = np.float32(flat[k] - h)
+        d = float(ep[k]) - float(em[k])
+        fd = (make(ep.reshape(eps.shape)).objective(**NT_OBJ)
+              - make(em.reshape(eps.shape)).objective(**NT_OBJ)) / d
+        assert abs(ge[k] - fd) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(ge))), (k, flat[k], ge[k], fd)
+    assert np.all(ge[flat == 0.0] == 0.0)       # only cells with a solid fraction are design cells
+    u0, hu = 0.05, 1e-5
+    fdu = (make(u=u0 + hu).objective(**NT_OBJ) - make(u=u0 - hu).objective(**NT_OBJ)) / (2 * hu)
+    assert gu == pytest.approx(fdu, rel=1e-6)
+
+
+def test_adjoint_drag_term_alone():
+    """The partial-cell drag objective alone (w_p = 0): J is the time-averaged
+    x momentum exchange of the NT cells, its gradient matches differences."""
+    make, eps = _nt_case()
+    obj = dict(NT_OBJ, w_p=0.0, w_f=1.0)
+    J, ga, ge, gu = make().vjp(**obj, with_eps=True)
+    assert J > 0.0                                # the body is dragged downstream
+    flat = eps.reshape(-1)
+    k = int(np.argmax((flat > 0.2) & (flat < 0.8)))
+    h = 1e-4
+    ep, em = flat.copy(), flat.copy()
+    ep[k] += np.float32(h)
+    em[k] -= np.float32(h)
+    d = float(ep[k]) - float(em[k])
+    fd = (make(ep.reshape(eps.shape)).objective(**obj) - make(em.reshape(eps.shape)).objective(**obj)) / d
+    assert ge[k] == pytest.approx(fd, rel=1e-6)
