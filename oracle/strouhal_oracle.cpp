@@ -711,15 +711,73 @@ template <class S> struct Lattice final : Base {
         return tw * (1.0 + tw) / (d * d);
     }
 
+    // The relaxation rates a cell collides with (Kernels.metal:112-167): the
+    // sponge ramp by x, then the Smagorinsky rate from Q = |Pi_neq|, with the
+    // pieces the adjoint differentiates. g: the 19 populations, stride `st`.
+    struct LocalRates {
+        double wp, wm;
+        bool les = false, srt = true;
+        double tau = 0.0, s = 0.0, K = 0.0, Q = 0.0, lambda = 0.0;
+        double Pi[6] = {0, 0, 0, 0, 0, 0};   // xx yy zz xy xz yz
+    };
+    LocalRates localRates(const double* g, size_t st, int x, const Params<R>& p) const {
+        LocalRates r;
+        double wp = (double)p.omega, wm = (double)p.omegaMinus;
+        r.srt = p.omegaMinus == p.omega;
+        if ((double)x > (double)p.spongeX0) {
+            const double frac = std::fmin(((double)x - (double)p.spongeX0) * (double)p.spongeInvW, 1.0);
+            const double tau0 = 1.0 / wp;
+            const double tauS = tau0 + frac * ((double)p.spongeTau - tau0);
+            if (r.srt) { wp = 1.0 / tauS; wm = wp; }
+            else {
+                const double lam = (1.0 / wp - 0.5) * (1.0 / wm - 0.5);
+                wp = 1.0 / tauS;
+                wm = 1.0 / (0.5 + lam / (tauS - 0.5));
+            }
+        }
+        if ((double)p.cSmago > 0.0) {
+            double rho = 1.0, pv[3] = {0, 0, 0};
+            for (int i = 0; i < 19; i++) {
+                const double v = g[(size_t)i * st];
+                rho += v; pv[0] += CX[i] * v; pv[1] += CY[i] * v; pv[2] += CZ[i] * v;
+            }
+            const double rm1 = rho - 1.0;
+            const double ux = (pv[0] + 0.5 * (double)p.fx) / rho, uy = (pv[1] + 0.5 * (double)p.fy) / rho,
+                         uz = (pv[2] + 0.5 * (double)p.fz) / rho;
+            const double u2 = ux * ux + uy * uy + uz * uz;
+            double* P = r.Pi;
+            for (int i = 1; i < 19; i++) {
+                const double cu = 3.0 * (CX[i] * ux + CY[i] * uy + CZ[i] * uz);
+                const double d = g[(size_t)i * st] - weight<double>(i) * (rm1 + rho * (cu + 0.5 * cu * cu - 1.5 * u2));
+                P[0] += CX[i] * CX[i] * d; P[1] += CY[i] * CY[i] * d; P[2] += CZ[i] * CZ[i] * d;
+                P[3] += CX[i] * CY[i] * d; P[4] += CX[i] * CZ[i] * d; P[5] += CY[i] * CZ[i] * d;
+            }
+            r.Q = std::sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2] + 2.0 * (P[3] * P[3] + P[4] * P[4] + P[5] * P[5]));
+            r.K = smagoConst<double>() * (double)p.cSmago;
+            const double tau0 = 1.0 / wp;
+            r.s = std::sqrt(tau0 * tau0 + r.K * r.Q / rho);
+            r.tau = 0.5 * (tau0 + r.s);
+            r.les = true;
+            if (r.srt) { wp = 1.0 / r.tau; wm = wp; }
+            else {
+                r.lambda = (1.0 / wp - 0.5) * (1.0 / wm - 0.5);
+                wp = 1.0 / r.tau;
+                wm = 1.0 / (0.5 + r.lambda / (r.tau - 0.5));
+            }
+        }
+        r.wp = wp; r.wm = wm;
+        return r;
+    }
+
     double cellObjective(const double* g, double a, double e, const Params<R>& p, double cpn,
-                         const Objective& o) const {
+                         const Objective& o, int x) const {
         double rho = 1.0, px = 0.0;
         for (int i = 0; i < 19; i++) { rho += g[i]; px += CX[i] * g[i]; }
         const double gam = 1.0 + 0.5 * a;
         const double Jx = (px + 0.5 * (double)p.fx) / gam;
         double v = o.w_p * cpn * rho + o.w_d * a * Jx;
         if (e != 0.0 && o.w_f != 0.0) {                 // ntF_x = -2 sum c_x B (g_opp - g)
-            const double B = ntB(e, (double)p.omega);
+            const double B = ntB(e, localRates(g, 1, x, p).wp);
             double nt = 0.0;
             for (int i = 1; i < 19; i += 2) nt -= 2.0 * CX[i] * B * (g[i + 1] - g[i]);
             v += o.w_f * nt;
@@ -728,7 +786,7 @@ template <class S> struct Lattice final : Base {
     }
 
     bool adjointSupported() const {
-        return !(hp.c_smago > 0.0 || hp.sponge_on || (useEps && useAlpha));
+        return !(useAlpha && (useEps || hp.c_smago > 0.0));
     }
 
     // Objective terms of step k's pre-collision state g, summed over the
@@ -741,7 +799,7 @@ template <class S> struct Lattice final : Base {
             if (flags[n] != FLUID) continue;
             for (int i = 0; i < 19; i++) gc[i] = g[i * N + n];
             sum += cellObjective(gc, useAlpha ? (double)alpha[n] : 0.0, useEps ? (double)eps[n] : 0.0,
-                                 p, cp[n], o);
+                                 p, cp[n], o, (int)(n % (size_t)nx));
         }
         return sum;
     }
@@ -814,8 +872,6 @@ template <class S> struct Lattice final : Base {
             const int64_t todd = (t & 1) ? t : t + 1;
             const double rOdd = (double)ramp(todd);
             const double wobj = k >= o.spinup ? 1.0 / (double)o.avg : 0.0;
-            const double wp = (double)p.omega, wm = (double)p.omegaMinus;
-            const double ap = 1.0 - 0.5 * wp, am = 1.0 - 0.5 * wm;
             const double Fb[3] = {(double)p.fx, (double)p.fy, (double)p.fz};
             std::fill(lam.begin(), lam.end(), 0.0);
             const double* gk = &G[(size_t)(k - k0) * 19 * N];
@@ -852,14 +908,21 @@ template <class S> struct Lattice final : Base {
                     }
                 }
                 const double a = useAlpha ? (double)alpha[n] : 0.0;
+                const double* gc = gk + n;
+                auto G = [&](int i) { return gc[(size_t)i * N]; };
                 double rho = 1.0, pv[3] = {0, 0, 0};
                 for (int i = 0; i < 19; i++) {
-                    const double v = gk[i * N + n];
+                    const double v = G(i);
                     rho += v; pv[0] += CX[i] * v; pv[1] += CY[i] * v; pv[2] += CZ[i] * v;
                 }
                 const double gam = 1.0 + 0.5 * a;
                 double Jv[3], Fv[3];
                 for (int c = 0; c < 3; c++) { Jv[c] = (pv[c] + 0.5 * Fb[c]) / gam; Fv[c] = Fb[c] - a * Jv[c]; }
+                const double rm1 = rho - 1.0;
+                // local relaxation rates: sponge ramp, then Smagorinsky (Kernels.metal:112-167)
+                const LocalRates lr = localRates(gc, N, x, p);
+                const double wp = lr.wp, wm = lr.wm;
+                const double ap = 1.0 - 0.5 * wp, am = 1.0 - 0.5 * wm;
                 double M0 = 0, Mc[3] = {0, 0, 0}, Mcc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
                 for (int i = 0; i < 19; i++) {
                     const double wm_ = mu[i] * weight<double>(i);
@@ -889,10 +952,36 @@ template <class S> struct Lattice final : Base {
                     gF[a1] = ap * (9.0 * MJ[a1] - 3.0 * Jv[a1] * M0) / rho + 3.0 * am * Mc[a1];
                     gP[a1] = (gJ[a1] - a * gF[a1]) / gam;
                 }
+                // the increment Omega of the golden path and its rate derivatives
+                const double ux = Jv[0] / rho, uy = Jv[1] / rho, uz = Jv[2] / rho;
+                const double u2 = ux * ux + uy * uy + uz * uz;
+                const double uF = ux * Fv[0] + uy * Fv[1] + uz * Fv[2];
+                double Om[19], dOwp[19], dOwm[19];
+                {
+                    const double feq0 = weight<double>(0) * (rm1 - 1.5 * rho * u2);
+                    const double x0 = -3.0 * uF;
+                    Om[0] = wp * (feq0 - G(0)) + weight<double>(0) * ap * x0;
+                    dOwp[0] = (feq0 - G(0)) - 0.5 * weight<double>(0) * x0;
+                    dOwm[0] = 0.0;
+                }
+                for (int i = 1; i < 19; i += 2) {
+                    const double W = weight<double>(i);
+                    const double cu = 3.0 * (CX[i] * ux + CY[i] * uy + CZ[i] * uz);
+                    const double cF = CX[i] * Fv[0] + CY[i] * Fv[1] + CZ[i] * Fv[2];
+                    const double ep = W * (rm1 + rho * (0.5 * cu * cu - 1.5 * u2)) - 0.5 * (G(i) + G(i + 1));
+                    const double em = W * rho * cu - 0.5 * (G(i) - G(i + 1));
+                    const double xp = 3.0 * cu * cF - 3.0 * uF, xm = 3.0 * cF;
+                    Om[i] = wp * ep + wm * em + W * ap * xp + W * am * xm;
+                    Om[i + 1] = wp * ep - wm * em + W * ap * xp - W * am * xm;
+                    dOwp[i] = dOwp[i + 1] = ep - 0.5 * W * xp;
+                    dOwm[i] = em - 0.5 * W * xm;
+                    dOwm[i + 1] = -dOwm[i];
+                }
                 // objective: w_p cp rho + w_d alpha J_x (+ w_f ntF_x on partial cells)
                 const double dObjRho = wobj * o.w_p * cp[n];
                 const double dObjPx = wobj * o.w_d * a / gam;
                 const double e = useEps ? (double)eps[n] : 0.0;
+                double gwp = 0.0, gwm = 0.0;
                 if (e == 0.0) {
                     for (int j = 0; j < 19; j++) {
                         const double mp = j == 0 ? mu[0] : 0.5 * (mu[j] + mu[OPP[j]]);
@@ -900,37 +989,15 @@ template <class S> struct Lattice final : Base {
                         lam[j * N + n] = mu[j] - wp * mp - wm * mm + gRho
                                        + CX[j] * gP[0] + CY[j] * gP[1] + CZ[j] * gP[2]
                                        + dObjRho + dObjPx * CX[j];
+                        gwp += mu[j] * dOwp[j];
+                        gwm += mu[j] * dOwm[j];
                     }
                 } else {
                     // Noble-Torczynski cell (Kernels.metal:201-227):
                     //   f*_j = f_j + (1 - B) Omega_j(f) + B (f_opp(j) - f_j),  j >= 1,
                     //   f*_0 = f_0 + (1 - B) Omega_0(f),
-                    // Omega the golden-path increment; B depends on eps only.
+                    // B = eps tw / ((1 - eps) + tw), tw = 1/w+ - 1/2 at the local rate.
                     const double B = ntB(e, wp), omB = 1.0 - B;
-                    const double* gc = gk + n;
-                    auto G = [&](int i) { return gc[(size_t)i * N]; };
-                    // Omega from the recorded state (no porosity with partial cells)
-                    double rm1 = 0.0;
-                    for (int i = 0; i < 19; i++) rm1 += G(i);
-                    const double ux = Jv[0] / rho, uy = Jv[1] / rho, uz = Jv[2] / rho;
-                    const double u2 = ux * ux + uy * uy + uz * uz;
-                    const double uF = ux * Fb[0] + uy * Fb[1] + uz * Fb[2];
-                    double Om[19];
-                    Om[0] = wp * (weight<double>(0) * (rm1 - 1.5 * rho * u2) - G(0))
-                          + weight<double>(0) * ap * (-3.0 * uF);
-                    for (int i = 1; i < 19; i += 2) {
-                        const double W = weight<double>(i);
-                        const double cu = 3.0 * (CX[i] * ux + CY[i] * uy + CZ[i] * uz);
-                        const double cF = CX[i] * Fb[0] + CY[i] * Fb[1] + CZ[i] * Fb[2];
-                        const double feqp = W * (rm1 + rho * (0.5 * cu * cu - 1.5 * u2));
-                        const double feqm = W * rho * cu;
-                        const double dp = wp * (feqp - 0.5 * (G(i) + G(i + 1)));
-                        const double dm = wm * (feqm - 0.5 * (G(i) - G(i + 1)));
-                        const double sm = W * am * (3.0 * cF);
-                        const double sp = W * ap * (3.0 * cu * cF - 3.0 * uF);
-                        Om[i] = dp + dm + sp + sm;
-                        Om[i + 1] = dp - dm + sp - sm;
-                    }
                     double gB = 0.0, dNt = 0.0;
                     for (int i = 0; i < 19; i++) gB -= mu[i] * Om[i];
                     for (int i = 1; i < 19; i++) gB += mu[i] * (G(OPP[i]) - G(i));
@@ -942,8 +1009,43 @@ template <class S> struct Lattice final : Base {
                         const double om = -wp * mp - wm * mm + gRho + CX[j] * gP[0] + CY[j] * gP[1] + CZ[j] * gP[2];
                         const double bounce = j == 0 ? 0.0 : B * (mu[OPP[j]] - mu[j]);
                         lam[j * N + n] = mu[j] + omB * om + bounce + dObjRho + (dObjPx + dObjNt) * CX[j];
+                        gwp += omB * mu[j] * dOwp[j];
+                        gwm += omB * mu[j] * dOwm[j];
                     }
-                    ge[n] += (gB + wobj * o.w_f * dNt) * ntdB(e, wp);
+                    const double gBtot = gB + wobj * o.w_f * dNt;
+                    ge[n] += gBtot * ntdB(e, wp);
+                    // B depends on the rate through tw = 1/w+ - 1/2
+                    const double tw = 1.0 / wp - 0.5, den = (1.0 - e) + tw;
+                    gwp += gBtot * (e * (1.0 - e) / (den * den)) * (-1.0 / (wp * wp));
+                }
+                if (lr.les) {
+                    // w+ = 1/tau, tau = (tau0 + s)/2, s = sqrt(tau0^2 + K Q / rho); w- = w+ (SRT)
+                    // or 1/(1/2 + Lambda/(tau - 1/2)) (TRT); Q = |Pi_neq| with
+                    // Pi_ab = sum_i c_ia c_ib f_i - delta_ab (rho-1)/3 - J_a J_b / rho.
+                    const double dwm = lr.srt ? -wp * wp : wm * wm * lr.lambda / ((lr.tau - 0.5) * (lr.tau - 0.5));
+                    const double gTau = gwp * (-wp * wp) + gwm * dwm;
+                    if (lr.Q > 0.0) {
+                        const double gQ = gTau * lr.K / (4.0 * rho * lr.s);
+                        const double gR = -gTau * lr.K * lr.Q / (4.0 * rho * rho * lr.s);
+                        const double* P3 = lr.Pi;   // xx yy zz xy xz yz
+                        const double Pm[3][3] = {{P3[0], P3[3], P3[4]}, {P3[3], P3[1], P3[5]}, {P3[4], P3[5], P3[2]}};
+                        const double tr = P3[0] + P3[1] + P3[2];
+                        double PJ[3], JPJ = 0.0;
+                        for (int a1 = 0; a1 < 3; a1++) {
+                            PJ[a1] = Pm[a1][0] * Jv[0] + Pm[a1][1] * Jv[1] + Pm[a1][2] * Jv[2];
+                            JPJ += Jv[a1] * PJ[a1];
+                        }
+                        for (int j = 0; j < 19; j++) {
+                            const int cc[3] = {CX[j], CY[j], CZ[j]};
+                            double cPc = 0.0, cPJ = 0.0;
+                            for (int a1 = 0; a1 < 3; a1++) {
+                                cPJ += cc[a1] * PJ[a1];
+                                for (int b1 = 0; b1 < 3; b1++) cPc += cc[a1] * Pm[a1][b1] * cc[b1];
+                            }
+                            const double dQ = (cPc - tr / 3.0 - 2.0 * cPJ / rho + JPJ / (rho * rho)) / lr.Q;
+                            lam[j * N + n] += gQ * dQ + gR;
+                        }
+                    }
                 }
                 const double gJdotJ = gJ[0] * Jv[0] + gJ[1] * Jv[1] + gJ[2] * Jv[2];
                 const double gFdotJ = gF[0] * Jv[0] + gF[1] * Jv[1] + gF[2] * Jv[2];

@@ -246,3 +246,43 @@ def test_adjoint_drag_term_alone():
     d = float(ep[k]) - float(em[k])
     fd = (make(ep.reshape(eps.shape)).objective(**obj) - make(em.reshape(eps.shape)).objective(**obj)) / d
     assert ge[k] == pytest.approx(fd, rel=1e-6)
+
+
+@pytest.mark.parametrize("lam", [None, 3.0 / 16.0])
+def test_adjoint_with_les_and_sponge_matches_central_differences(lam):
+    """Smagorinsky LES (rates from |Pi_neq|) and the outlet sponge on the NT
+    disk channel: dJ/deps and dJ/du_in against central differences, SRT and
+    TRT (the LES rate enters w+ and w-, and B through tw = 1/w+ - 1/2)."""
+    from strouhal_amd.cases import disk_solid_fractions
+    nx, ny = 22, 12
+    eps = disk_solid_fractions(nx, ny, 8.3, 5.7, 2.6).astype(np.float32)
+    fl = _channel(nx, ny)
+
+    def make(e=eps, u=0.06):
+        tau = 0.56
+        wp = 1.0 / tau
+        wm = wp if lam is None else 1.0 / (0.5 + lam / (tau - 0.5))
+        s = OracleSim(nx, ny, 1, wp, wm, arith="f64", storage="f64", uin=u, ramp_steps=3,
+                      c_smago=0.3, flags=fl)
+        s.sponge = (15.0, 5.0, 0.9)
+        s.set_solid_fractions(e)
+        s.init_field(0, 0.0)
+        return s
+    obj = dict(inlet_x=2, outlet_x=19, spinup=4, avg=9, w_p=1.0, w_d=0.0, w_f=2.0)
+    J, ga, ge, gu = make().vjp(**obj, with_eps=True)
+    assert np.isfinite(J) and J == make().objective(**obj)
+    flat = eps.reshape(-1)
+    cells = [k for k in range(flat.size) if flat[k] > 0][::4][:7]
+    # tau - 1/2 = 0.06 makes B(eps) strongly curved at eps = 1: a smaller step
+    # keeps the central difference's truncation error under the bound
+    h = 1e-5
+    for k in cells:
+        ep, em = flat.copy(), flat.copy()
+        ep[k] = np.float32(flat[k] + h)
+        em[k] = np.float32(flat[k] - h)
+        d = float(ep[k]) - float(em[k])
+        fd = (make(ep.reshape(eps.shape)).objective(**obj) - make(em.reshape(eps.shape)).objective(**obj)) / d
+        assert abs(ge[k] - fd) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(ge))), (k, flat[k], ge[k], fd)
+    u0, hu = 0.06, 1e-5
+    fdu = (make(u=u0 + hu).objective(**obj) - make(u=u0 - hu).objective(**obj)) / (2 * hu)
+    assert gu == pytest.approx(fdu, rel=1e-6)
