@@ -69,6 +69,7 @@ struct AdjHooks {
     float* recNt;            // [19][M] pre-collision populations of those cells, or null
     uint32_t M;
     double wf;               // w_f / T (partial-cell drag)
+    float* rec19;            // [19][N] pre-collision populations of every cell (LES), or null
 };
 
 // One reverse step: lambda^t from lambda^{t+1} and the recorded moments of g^t.
@@ -83,18 +84,24 @@ struct AdjStep {
     float cin, cout, wd;
     float wf;                    // partial-cell drag weight of step t
     uint32_t M;                  // cells with a solid fraction
+    float spongeX0, spongeInvW, spongeTau;   // Params' sponge (x0 = 1e9: none)
+    float cSmago;                // Smagorinsky Cs^2 (LES lattices)
+    int32_t srt;                 // omega- == omega+ (the reference's SRT test)
 };
 
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);
 // Forward step with the adjoint hooks; L.alpha must be set (zeros for a
-// lattice without porosity) unless the lattice has solid fractions (L.nt).
+// lattice without porosity) unless the lattice has solid fractions (L.nt) or
+// Smagorinsky LES (L.les), which exclude porosity.
 hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s);
 // Partial-cell lattices pass eps, ntIdx, recNt (this step's [19][M]) and gEps ([M]).
+// LES lattices pass rec19 (this step's [19][N] populations of every cell)
+// instead of rec / recNt.
 hipError_t launch_adjoint_step(const AdjStep& q, bool walls, const float4* rec, const float* lamNext,
                                float* lam, const uint8_t* flags, const uint32_t* link,
                                const float* alpha, double* gAlpha, double* gUin,
                                const float* eps, const int32_t* ntIdx, const float* recNt,
-                               double* gEps, hipStream_t s);
+                               double* gEps, const float* rec19, hipStream_t s);
 hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,
                               double* out, hipStream_t s);
 hipError_t launch_batch_step(int precision, int bs, const BatchLat* lats, int L,

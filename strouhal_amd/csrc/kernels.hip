@@ -152,10 +152,11 @@ __device__ __forceinline__ void wall_vel(uint8_t wf, int wy, const StepParams& p
 }
 
 // ---- collide: Kernels.metal:105-228 -------------------------------------
-template <bool LES> __device__ __forceinline__ void les_rates(
-    const float (&fh)[19], const StepParams& p, float rhom1, float rho, float inv,
-    float ux, float uy, float uz, float u2, float& wp, float& wm) {
-    float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
+// Non-equilibrium stress Pi_ab = sum_i c_ia c_ib (f_i - feq_i), Kernels.metal:140-158.
+__device__ __forceinline__ void neq_stress(const float (&fh)[19], float rhom1, float rho, float ux,
+                                           float uy, float uz, float u2, float& pxx, float& pyy,
+                                           float& pzz, float& pxy, float& pxz, float& pyz) {
+    pxx = 0.0f; pyy = 0.0f; pzz = 0.0f; pxy = 0.0f; pxz = 0.0f; pyz = 0.0f;
 #define STRH_LES_TERM(I)                                                              \
     {                                                                                 \
         const float cu = 3.0f * cdot<I>(ux, uy, uz);                                  \
@@ -173,6 +174,13 @@ template <bool LES> __device__ __forceinline__ void les_rates(
     STRH_LES_TERM(11) STRH_LES_TERM(12) STRH_LES_TERM(13) STRH_LES_TERM(14) STRH_LES_TERM(15)
     STRH_LES_TERM(16) STRH_LES_TERM(17) STRH_LES_TERM(18)
 #undef STRH_LES_TERM
+}
+
+template <bool LES> __device__ __forceinline__ void les_rates(
+    const float (&fh)[19], const StepParams& p, float rhom1, float rho, float inv,
+    float ux, float uy, float uz, float u2, float& wp, float& wm) {
+    float pxx, pyy, pzz, pxy, pxz, pyz;
+    neq_stress(fh, rhom1, rho, ux, uy, uz, u2, pxx, pyy, pzz, pxy, pxz, pyz);
     const float Q = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz
                           + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
     const float tau0 = 1.0f / wp;
@@ -370,15 +378,55 @@ constexpr uint32_t kStepBS = 256;
 // float's 24 bits). The hook also accumulates the cell's objective term,
 //   cp(x) rho + wd alpha J_x,   J_x = (p_x + F_x / 2) / (1 + alpha / 2),
 // with cp = +cin on the inlet column and -cout on the outlet column.
+// The rate w+ a cell collides with (sponge ramp, then Smagorinsky), for the
+// hooks' partial-cell drag; the same float arithmetic as collide.
+template <bool LES>
+__device__ __forceinline__ float hook_rate(const float (&fh)[19], const StepParams& p, int x) {
+    float wp = p.omega, wm = p.omegaMinus;
+    if ((float)x > p.spongeX0) {
+        const float frac = fminf(((float)x - p.spongeX0) * p.spongeInvW, 1.0f);
+        const float tau0 = 1.0f / wp;
+        const float tauS = tau0 + frac * (p.spongeTau - tau0);
+        if (p.omegaMinus == p.omega) {
+            wp = 1.0f / tauS;
+            wm = wp;
+        } else {
+            const float lambda = (1.0f / wp - 0.5f) * (1.0f / wm - 0.5f);
+            wp = 1.0f / tauS;
+            wm = 1.0f / (0.5f + lambda / (tauS - 0.5f));
+        }
+    }
+    if (LES && p.cSmago > 0.0f) {
+        float rhom1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 19; i++) {
+            rhom1 += fh[i];
+            px += (float)kCx[i] * fh[i];
+            py += (float)kCy[i] * fh[i];
+            pz += (float)kCz[i] * fh[i];
+        }
+        const float rho = 1.0f + rhom1, inv = 1.0f / rho;
+        const float ux = (px + 0.5f * p.fx) * inv, uy = (py + 0.5f * p.fy) * inv, uz = (pz + 0.5f * p.fz) * inv;
+        les_rates<true>(fh, p, rhom1, rho, inv, ux, uy, uz, ux * ux + uy * uy + uz * uz, wp, wm);
+    }
+    return wp;
+}
+
+template <bool LES>
 __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[19], uint32_t n,
-                                          int x, float alpha, float eps, float omega, float fx) {
+                                          int x, float alpha, float eps, const StepParams& p) {
+    const float fx = p.fx;
+    if (hk.rec19) {
+#pragma unroll
+        for (int i = 0; i < 19; i++) __builtin_nontemporal_store(fh[i], hk.rec19 + (size_t)i * p.N + n);
+    }
     if (hk.recNt && eps > 0.0f) {
         const int32_t k = hk.ntIdx[n];
 #pragma unroll
         for (int i = 0; i < 19; i++) hk.recNt[(size_t)i * hk.M + k] = fh[i];
     }
     if (hk.jacc && hk.wf != 0.0 && eps > 0.0f) {   // partial-cell drag ntF_x = -2 sum c_x B (g_opp - g)
-        const double tw = 1.0 / (double)omega - 0.5, e = (double)eps;
+        const double tw = 1.0 / (double)hook_rate<LES>(fh, p, x) - 0.5, e = (double)eps;
         const double B = e * tw / ((1.0 - e) + tw);
         double nt = 0.0;
 #pragma unroll
@@ -442,7 +490,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 #pragma unroll
         for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nb);
         const float alpha = BRINK ? alphaBuf[n] : 0.0f;
-        if (HOOK) run_hooks(*hk, fh, n, x, alpha, eps, p.omega, p.fx);
+        if (HOOK) run_hooks<LES>(*hk, fh, n, x, alpha, eps, p);
         collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);
         // A separate copy of the offset keeps the stores on the scalar-base
         // form too (otherwise the shared base+offset sum is kept as 19
@@ -533,7 +581,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     }
 
     const float alpha = BRINK ? alphaBuf[n] : 0.0f;
-    if (HOOK) run_hooks(*hk, fh, n, x, alpha, eps, p.omega, p.fx);
+    if (HOOK) run_hooks<LES>(*hk, fh, n, x, alpha, eps, p);
     collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);  // 315-316
     if (NT) {
         ax += ntx;
@@ -601,7 +649,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
 // Forward step of the differentiable configuration (Guo forcing, Brinkman
 // porosity) with the adjoint hooks. The arithmetic is that of step_kernel:
 // with alpha = 0 and no body force the DDFs are the plain update's bits.
-template <class St, int PARITY, bool WALLS, bool NT>
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool BRINK>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                    const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
@@ -609,8 +657,7 @@ step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ f
                    AdjHooks hk) {
     const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * kStepBS + threadIdx.x;
     if (n >= p.N) return;
-    // partial cells and porosity exclude each other
-    step_cell<St, PARITY, WALLS, NT, false, true, !NT, true>(f, flags, link, lidm, nullptr, epsBuf, p, n,
+    step_cell<St, PARITY, WALLS, NT, LES, true, BRINK, true>(f, flags, link, lidm, nullptr, epsBuf, p, n,
                                                              alphaBuf, &hk);
 }
 
@@ -639,22 +686,15 @@ template <int I> __device__ __forceinline__ float wgt_f() {
     return I == 0 ? 1.0f / 3.0f : (I <= 6 ? 1.0f / 18.0f : 1.0f / 36.0f);
 }
 
-template <bool WALLS, bool NT>
-__global__ void __launch_bounds__(256)
-adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ lamN,
-                    float* __restrict__ lam, const uint8_t* __restrict__ flags,
-                    const uint32_t* __restrict__ link, const float* __restrict__ alphaBuf,
-                    double* __restrict__ gAlpha, double* __restrict__ gUin,
-                    const float* __restrict__ epsBuf, const int32_t* __restrict__ ntIdx,
-                    const float* __restrict__ recNt, double* __restrict__ gEps, AdjStep q) {
-    const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
-    if (n >= q.N) return;
-    if (flags[n] != kFluid) return;
+// mu = P^T lambda^{t+1} for cell n: the pull from each neighbour, or at a
+// wall link the cell's own opposite slot (negated for the anti-bounce-back
+// outlet). Returns the cell's share of dJ/d(uin target) from inflow links.
+template <bool WALLS>
+__device__ __forceinline__ double adj_gather(const AdjStep& q, uint32_t n, int x, int y, int z,
+                                             const float* __restrict__ lamN,
+                                             const uint8_t* __restrict__ flags,
+                                             const uint32_t* __restrict__ link, float (&mu)[19]) {
     const uint32_t S = q.S;
-    const uint32_t t = fdiv(n, q.divx);
-    const int x = (int)(n - t * q.nx);
-    const uint32_t z = fdiv(t, q.divy);
-    const int y = (int)(t - z * q.ny);
     const int nx = (int)q.nx, ny = (int)q.ny, nz = (int)q.nz;
     const int plane = nx * ny;
     Offsets o;   // element offsets to the wrapped neighbours
@@ -662,12 +702,9 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     o.xp = (x == nx - 1) ? 1 - nx : 1;
     o.ym = (y == 0) ? (ny - 1) * nx : -nx;
     o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
-    o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
-    o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
+    o.zm = (z == 0) ? (nz - 1) * plane : -plane;
+    o.zp = (z == nz - 1) ? -(nz - 1) * plane : plane;
     const uint32_t sM = WALLS ? (link[n] & 0x3ffffu) : 0u;
-
-    // mu = P^T lambda^{t+1}
-    float mu[19];
     double gu = 0.0;
     mu[0] = lamN[n];
 #define STRH_GATHER(I)                                                                \
@@ -695,16 +732,34 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     STRH_GATHER(7) STRH_GATHER(8) STRH_GATHER(9) STRH_GATHER(10) STRH_GATHER(11) STRH_GATHER(12)
     STRH_GATHER(13) STRH_GATHER(14) STRH_GATHER(15) STRH_GATHER(16) STRH_GATHER(17) STRH_GATHER(18)
 #undef STRH_GATHER
+    return gu;
+}
 
-    const float4 m = rec[n];
-    const float a = alphaBuf ? alphaBuf[n] : 0.0f;
-    const float rho = 1.0f + m.x, ir = 1.0f / rho, ir2 = ir * ir;
-    const float gam = 1.0f + 0.5f * a, ig = 1.0f / gam;
-    const float fx = q.fx, fy = q.fy, fz = q.fz;
-    const float Jx = (m.y + 0.5f * fx) * ig, Jy = (m.z + 0.5f * fy) * ig, Jz = (m.w + 0.5f * fz) * ig;
-    const float Fx = fx - a * Jx, Fy = fy - a * Jy, Fz = fz - a * Jz;
-    float M0 = 0.0f, Mx = 0.0f, My = 0.0f, Mz = 0.0f;
-    float Mxx = 0.0f, Myy = 0.0f, Mzz = 0.0f, Mxy = 0.0f, Mxz = 0.0f, Myz = 0.0f;
+// Relaxation rates of column x: the outlet sponge ramp (Kernels.metal:112-123).
+__device__ __forceinline__ void adj_sponge(const AdjStep& q, int x, float& wp, float& wm) {
+    wp = q.wp;
+    wm = q.wm;
+    if ((float)x > q.spongeX0) {
+        const float frac = fminf(((float)x - q.spongeX0) * q.spongeInvW, 1.0f);
+        const float tau0 = 1.0f / wp;
+        const float tauS = tau0 + frac * (q.spongeTau - tau0);
+        if (q.srt) {
+            wp = 1.0f / tauS;
+            wm = wp;
+        } else {
+            const float lambda = (1.0f / wp - 0.5f) * (1.0f / wm - 0.5f);
+            wp = 1.0f / tauS;
+            wm = 1.0f / (0.5f + lambda / (tauS - 0.5f));
+        }
+    }
+}
+
+// Weighted moments of mu: M0 = sum w mu, M_a = sum w c_a mu, M_ab = sum w c_a c_b mu.
+__device__ __forceinline__ void mu_moments(const float (&mu)[19], float& M0, float& Mx, float& My,
+                                           float& Mz, float& Mxx, float& Myy, float& Mzz, float& Mxy,
+                                           float& Mxz, float& Myz) {
+    M0 = 0.0f; Mx = 0.0f; My = 0.0f; Mz = 0.0f;
+    Mxx = 0.0f; Myy = 0.0f; Mzz = 0.0f; Mxy = 0.0f; Mxz = 0.0f; Myz = 0.0f;
 #define STRH_MSUM(I)                                                                  \
     {                                                                                 \
         const float v = wgt_f<I>() * mu[I];                                           \
@@ -720,6 +775,36 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     STRH_MSUM(7) STRH_MSUM(8) STRH_MSUM(9) STRH_MSUM(10) STRH_MSUM(11) STRH_MSUM(12)
     STRH_MSUM(13) STRH_MSUM(14) STRH_MSUM(15) STRH_MSUM(16) STRH_MSUM(17) STRH_MSUM(18)
 #undef STRH_MSUM
+}
+
+template <bool WALLS, bool NT>
+__global__ void __launch_bounds__(256)
+adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ lamN,
+                    float* __restrict__ lam, const uint8_t* __restrict__ flags,
+                    const uint32_t* __restrict__ link, const float* __restrict__ alphaBuf,
+                    double* __restrict__ gAlpha, double* __restrict__ gUin,
+                    const float* __restrict__ epsBuf, const int32_t* __restrict__ ntIdx,
+                    const float* __restrict__ recNt, double* __restrict__ gEps, AdjStep q) {
+    const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
+    if (n >= q.N) return;
+    if (flags[n] != kFluid) return;
+    const uint32_t S = q.S;
+    const uint32_t t = fdiv(n, q.divx);
+    const int x = (int)(n - t * q.nx);
+    const uint32_t z = fdiv(t, q.divy);
+    const int y = (int)(t - z * q.ny);
+    float mu[19];
+    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, link, mu);
+
+    const float4 m = rec[n];
+    const float a = alphaBuf ? alphaBuf[n] : 0.0f;
+    const float rho = 1.0f + m.x, ir = 1.0f / rho, ir2 = ir * ir;
+    const float gam = 1.0f + 0.5f * a, ig = 1.0f / gam;
+    const float fx = q.fx, fy = q.fy, fz = q.fz;
+    const float Jx = (m.y + 0.5f * fx) * ig, Jy = (m.z + 0.5f * fy) * ig, Jz = (m.w + 0.5f * fz) * ig;
+    const float Fx = fx - a * Jx, Fy = fy - a * Jy, Fz = fz - a * Jz;
+    float M0, Mx, My, Mz, Mxx, Myy, Mzz, Mxy, Mxz, Myz;
+    mu_moments(mu, M0, Mx, My, Mz, Mxx, Myy, Mzz, Mxy, Mxz, Myz);
     const float MJx = Mxx * Jx + Mxy * Jy + Mxz * Jz;
     const float MJy = Mxy * Jx + Myy * Jy + Myz * Jz;
     const float MJz = Mxz * Jx + Myz * Jy + Mzz * Jz;
@@ -730,7 +815,8 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     const float JMF = Jx * MFx + Jy * MFy + Jz * MFz;
     const float J2 = Jx * Jx + Jy * Jy + Jz * Jz;
     const float JF = Jx * Fx + Jy * Fy + Jz * Fz;
-    const float wp = q.wp, wm = q.wm;
+    float wp, wm;
+    adj_sponge(q, x, wp, wm);
     const float ap = 1.0f - 0.5f * wp, am = 1.0f - 0.5f * wm;
     const float gRho = wp * (M0 - 4.5f * JMJ * ir2 + 1.5f * J2 * M0 * ir2)
                      - ap * (9.0f * JMF - 3.0f * JF * M0) * ir2;
@@ -832,6 +918,196 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     const float gFdotJ = gFx * Jx + gFy * Jy + gFz * Jz;
     const float k = 1.0f - 0.5f * a * ig;
     gAlpha[n] += (double)(-0.5f * gJdotJ * ig - gFdotJ * k + wd * Jx * k);
+    if (gu != 0.0) gUin[n] += gu;
+}
+
+// Reverse step of a Smagorinsky lattice (Kernels.metal:140-167), optionally
+// with partial cells. The rates w+ and w- of a cell depend on its populations
+// through Q = |Pi_neq| and rho:
+//   tau = (tau0 + s)/2, s = sqrt(tau0^2 + K Q / rho), K = 18 sqrt(2) Cs^2,
+//   w+ = 1/tau, w- = w+ (SRT) or 1/(1/2 + Lambda/(tau - 1/2)) (TRT),
+// with tau0 = 1/w+ after the sponge ramp. The forward pass records all 19
+// pre-collision populations of every cell (rec19, [19][N]); the reverse step
+// adds to the fixed-rate product the rate chain
+//   lambda_j += dJ/dQ dQ/dg_j + dJ/drho,
+//   dJ/dtau = -w+^2 dJ/dw+ + dw-/dtau dJ/dw-,
+//   dQ/dg_j = (c.Pi.c - tr(Pi)/3 - 2 c.Pi.J/rho + J.Pi.J/rho^2) / Q,
+// where dJ/dw+- = sum_j mu_j dOmega_j/dw+- (scaled by 1 - B on partial cells,
+// which add the B(tw(w+)) term). Porosity and LES exclude each other, so F is
+// the body force and J = p + F/2.
+template <bool WALLS, bool NT>
+__global__ void __launch_bounds__(256)
+adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ lamN,
+                   float* __restrict__ lam, const uint8_t* __restrict__ flags,
+                   const uint32_t* __restrict__ link, double* __restrict__ gUin,
+                   const float* __restrict__ epsBuf, const int32_t* __restrict__ ntIdx,
+                   double* __restrict__ gEps, AdjStep q) {
+    const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
+    if (n >= q.N) return;
+    if (flags[n] != kFluid) return;
+    const uint32_t S = q.S;
+    const uint32_t t = fdiv(n, q.divx);
+    const int x = (int)(n - t * q.nx);
+    const uint32_t z = fdiv(t, q.divy);
+    const int y = (int)(t - z * q.ny);
+    float mu[19];
+    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, link, mu);
+
+    float g[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) g[i] = rec19[(size_t)i * q.N + n];
+    float rm1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        rm1 += g[i];
+        px += (float)kCx[i] * g[i];
+        py += (float)kCy[i] * g[i];
+        pz += (float)kCz[i] * g[i];
+    }
+    const float rho = 1.0f + rm1, ir = 1.0f / rho, ir2 = ir * ir;
+    const float fx = q.fx, fy = q.fy, fz = q.fz;
+    const float Jx = px + 0.5f * fx, Jy = py + 0.5f * fy, Jz = pz + 0.5f * fz;
+    const float ux = Jx * ir, uy = Jy * ir, uz = Jz * ir;
+    const float u2 = ux * ux + uy * uy + uz * uz;
+    const float uF = ux * fx + uy * fy + uz * fz;
+
+    // local rates: sponge, then Smagorinsky (the forward's float arithmetic)
+    float wp, wm;
+    adj_sponge(q, x, wp, wm);
+    float pxx, pyy, pzz, pxy, pxz, pyz;
+    neq_stress(g, rm1, rho, ux, uy, uz, u2, pxx, pyy, pzz, pxy, pxz, pyz);
+    const float Q = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const float tau0 = 1.0f / wp;
+    const float Kc = 25.455844f * q.cSmago;
+    const float sq = sqrtf(tau0 * tau0 + Kc * Q * ir);
+    const float tau = 0.5f * (tau0 + sq);
+    float lambda = 0.0f;
+    if (q.srt) {
+        wp = 1.0f / tau;
+        wm = wp;
+    } else {
+        lambda = (1.0f / wp - 0.5f) * (1.0f / wm - 0.5f);
+        wp = 1.0f / tau;
+        wm = 1.0f / (0.5f + lambda / (tau - 0.5f));
+    }
+    const float ap = 1.0f - 0.5f * wp, am = 1.0f - 0.5f * wm;
+
+    // vector-Jacobian product of E(rho, J) at fixed rates
+    float M0, Mx, My, Mz, Mxx, Myy, Mzz, Mxy, Mxz, Myz;
+    mu_moments(mu, M0, Mx, My, Mz, Mxx, Myy, Mzz, Mxy, Mxz, Myz);
+    const float MJx = Mxx * Jx + Mxy * Jy + Mxz * Jz;
+    const float MJy = Mxy * Jx + Myy * Jy + Myz * Jz;
+    const float MJz = Mxz * Jx + Myz * Jy + Mzz * Jz;
+    const float MFx = Mxx * fx + Mxy * fy + Mxz * fz;
+    const float MFy = Mxy * fx + Myy * fy + Myz * fz;
+    const float MFz = Mxz * fx + Myz * fy + Mzz * fz;
+    const float JMJ = Jx * MJx + Jy * MJy + Jz * MJz;
+    const float JMF = Jx * MFx + Jy * MFy + Jz * MFz;
+    const float J2 = Jx * Jx + Jy * Jy + Jz * Jz;
+    const float JF = Jx * fx + Jy * fy + Jz * fz;
+    const float gRho = wp * (M0 - 4.5f * JMJ * ir2 + 1.5f * J2 * M0 * ir2)
+                     - ap * (9.0f * JMF - 3.0f * JF * M0) * ir2;
+    const float gPx = wp * (9.0f * MJx - 3.0f * Jx * M0) * ir + 3.0f * wm * Mx + ap * (9.0f * MFx - 3.0f * fx * M0) * ir;
+    const float gPy = wp * (9.0f * MJy - 3.0f * Jy * M0) * ir + 3.0f * wm * My + ap * (9.0f * MFy - 3.0f * fy * M0) * ir;
+    const float gPz = wp * (9.0f * MJz - 3.0f * Jz * M0) * ir + 3.0f * wm * Mz + ap * (9.0f * MFz - 3.0f * fz * M0) * ir;
+
+    // mu . dOmega/dw+, mu . dOmega/dw-, and mu . Omega for partial cells
+    float gwp, gwm = 0.0f, muOm = 0.0f;
+    {
+        const float feq0 = (1.0f / 3.0f) * (rm1 - 1.5f * rho * u2);
+        const float x0 = -3.0f * uF;
+        gwp = mu[0] * ((feq0 - g[0]) - 0.5f * (1.0f / 3.0f) * x0);
+        if (NT) muOm = mu[0] * (wp * (feq0 - g[0]) + (1.0f / 3.0f) * ap * x0);
+    }
+#define STRH_DOM(I)                                                                          \
+    {                                                                                        \
+        const float W = wgt_f<I>();                                                          \
+        const float cu = 3.0f * ((float)kCx[I] * ux + (float)kCy[I] * uy + (float)kCz[I] * uz); \
+        const float cF = (float)kCx[I] * fx + (float)kCy[I] * fy + (float)kCz[I] * fz;     \
+        const float ep = W * (rm1 + rho * (0.5f * cu * cu - 1.5f * u2)) - 0.5f * (g[I] + g[I + 1]); \
+        const float em = W * rho * cu - 0.5f * (g[I] - g[I + 1]);                           \
+        const float xp = 3.0f * cu * cF - 3.0f * uF, xm = 3.0f * cF;                        \
+        gwp += (mu[I] + mu[I + 1]) * (ep - 0.5f * W * xp);                                  \
+        gwm += (mu[I] - mu[I + 1]) * (em - 0.5f * W * xm);                                   \
+        if (NT) {                                                                            \
+            const float sp = wp * ep + W * ap * xp, sm = wm * em + W * am * xm;              \
+            muOm += mu[I] * (sp + sm) + mu[I + 1] * (sp - sm);                               \
+        }                                                                                    \
+    }
+    STRH_DOM(1) STRH_DOM(3) STRH_DOM(5) STRH_DOM(7) STRH_DOM(9)
+    STRH_DOM(11) STRH_DOM(13) STRH_DOM(15) STRH_DOM(17)
+#undef STRH_DOM
+
+    const float cp = (x == q.inletX ? q.cin : 0.0f) - (x == q.outletX ? q.cout : 0.0f);
+    float B = 0.0f, omB = 1.0f;
+    bool nt = false;
+    if (NT) {
+        const int32_t k = ntIdx[n];
+        if (k >= 0) {
+            // Noble-Torczynski cell (Kernels.metal:201-227), B = eps tw/((1 - eps) + tw)
+            // with tw = 1/w+ - 1/2 at the local rate
+            nt = true;
+            const float e = epsBuf[n];
+            const float tw = 1.0f / wp - 0.5f;
+            const float den = (1.0f - e) + tw;
+            B = e * tw / den;
+            omB = 1.0f - B;
+            float gB = -muOm, dNt = 0.0f;
+#pragma unroll
+            for (int i = 1; i < 19; i++) gB += mu[i] * (g[kOpp[i]] - g[i]);
+#pragma unroll
+            for (int i = 1; i < 19; i += 2) if (kCx[i]) dNt -= 2.0f * (float)kCx[i] * (g[i + 1] - g[i]);
+            const float gBtot = gB + q.wf * dNt;
+            gEps[k] += (double)(gBtot * (tw * (1.0f + tw) / (den * den)));
+            gwp *= omB;
+            gwm *= omB;
+            gwp += gBtot * (e * (1.0f - e) / (den * den)) * (-1.0f / (wp * wp));
+        }
+    }
+
+    // rate chain through tau(Q, rho)
+    const float dwm = q.srt ? -wp * wp : wm * wm * lambda / ((tau - 0.5f) * (tau - 0.5f));
+    const float gTau = gwp * (-wp * wp) + gwm * dwm;
+    float A = 0.0f, lesC = 0.0f, PJx = 0.0f, PJy = 0.0f, PJz = 0.0f;
+    if (Q > 0.0f) {
+        const float gQ = gTau * Kc / (4.0f * rho * sq);
+        const float gR = -gTau * Kc * Q * ir / (4.0f * rho * sq);
+        A = gQ / Q;
+        PJx = pxx * Jx + pxy * Jy + pxz * Jz;
+        PJy = pxy * Jx + pyy * Jy + pyz * Jz;
+        PJz = pxz * Jx + pyz * Jy + pzz * Jz;
+        const float JPJ = Jx * PJx + Jy * PJy + Jz * PJz;
+        lesC = A * (JPJ * ir2 - (pxx + pyy + pzz) * (1.0f / 3.0f)) + gR;
+    }
+    const float A2 = 2.0f * A * ir;
+    const float base = cp + lesC;
+    const float dObjX = nt ? q.wf * 2.0f * B : 0.0f;      // d ntF_x / d g_j = 2 B c_jx
+    lam[n] = mu[0] + omB * (-wp * mu[0] + gRho) + base;
+#define STRH_LESLAM(I)                                                                       \
+    {                                                                                        \
+        constexpr int cx = kCx[I], cy = kCy[I], cz = kCz[I];                                 \
+        const float mp = 0.5f * (mu[I] + mu[kOpp[I]]);                                       \
+        const float mm = 0.5f * (mu[I] - mu[kOpp[I]]);                                       \
+        float om = -wp * mp - wm * mm + gRho;                                                \
+        if (cx) om += (float)cx * gPx;                                                       \
+        if (cy) om += (float)cy * gPy;                                                       \
+        if (cz) om += (float)cz * gPz;                                                       \
+        float cPc = 0.0f, cPJ = 0.0f;                                                        \
+        if (cx) { cPc += pxx; cPJ += (float)cx * PJx; }                                      \
+        if (cy) { cPc += pyy; cPJ += (float)cy * PJy; }                                      \
+        if (cz) { cPc += pzz; cPJ += (float)cz * PJz; }                                      \
+        if (cx * cy) cPc += (float)(2 * cx * cy) * pxy;                                      \
+        if (cx * cz) cPc += (float)(2 * cx * cz) * pxz;                                      \
+        if (cy * cz) cPc += (float)(2 * cy * cz) * pyz;                                      \
+        float v = mu[I] + omB * om + base + A * cPc - A2 * cPJ;                              \
+        if (NT) v += B * (mu[kOpp[I]] - mu[I]);                                              \
+        if (cx) v += (float)cx * dObjX;                                                      \
+        lam[I * S + n] = v;                                                                  \
+    }
+    STRH_LESLAM(1) STRH_LESLAM(2) STRH_LESLAM(3) STRH_LESLAM(4) STRH_LESLAM(5) STRH_LESLAM(6)
+    STRH_LESLAM(7) STRH_LESLAM(8) STRH_LESLAM(9) STRH_LESLAM(10) STRH_LESLAM(11) STRH_LESLAM(12)
+    STRH_LESLAM(13) STRH_LESLAM(14) STRH_LESLAM(15) STRH_LESLAM(16) STRH_LESLAM(17) STRH_LESLAM(18)
+#undef STRH_LESLAM
     if (gu != 0.0) gUin[n] += gu;
 }
 
@@ -1271,12 +1547,14 @@ static hipError_t launch_hooked_t(const StepLaunch& L, const AdjHooks& hk, hipSt
     using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
     StepParams p = L.p;
     p.blocks = (p.N + kStepBS - 1) / kStepBS;
-    if (L.nt)
-        hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, true>), dim3(p.blocks), dim3(kStepBS), 0, s,
-                           (T*)L.f, L.flags, L.link, L.lidm, L.eps, p, (const float*)nullptr, hk);
-    else
-        hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, false>), dim3(p.blocks), dim3(kStepBS), 0, s,
-                           (T*)L.f, L.flags, L.link, L.lidm, (const float*)nullptr, p, L.alpha, hk);
+    // porosity excludes partial cells and LES
+#define STRH_HOOKED(NTV, LESV, BR) hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, NTV, LESV, BR>), \
+        dim3(p.blocks), dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm, L.eps, p, L.alpha, hk)
+    if (L.alpha) STRH_HOOKED(false, false, true);
+    else if (L.nt && L.les) STRH_HOOKED(true, true, false);
+    else if (L.nt) STRH_HOOKED(true, false, false);
+    else STRH_HOOKED(false, true, false);
+#undef STRH_HOOKED
     return hipGetLastError();
 }
 
@@ -1288,7 +1566,8 @@ static hipError_t launch_hooked_c(const StepLaunch& L, const AdjHooks& hk, hipSt
 }
 
 hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
-    if (L.les || (L.nt == !!L.alpha)) return hipErrorInvalidValue;   // exactly one of eps, alpha
+    // porosity alone, or partial cells and/or LES
+    if (L.alpha ? (L.nt || L.les) : !(L.nt || L.les)) return hipErrorInvalidValue;
     switch (L.precision) {
         case kFP32: return launch_hooked_c<ConvF32>(L, hk, s);
         case kFP16S: return launch_hooked_c<ConvF16>(L, hk, s);
@@ -1301,10 +1580,21 @@ hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float4* rec,
                                float* lam, const uint8_t* flags, const uint32_t* link,
                                const float* alpha, double* gAlpha, double* gUin,
                                const float* eps, const int32_t* ntIdx, const float* recNt,
-                               double* gEps, hipStream_t s) {
+                               double* gEps, const float* rec19, hipStream_t s) {
     AdjStep q = q0;
     q.blocks = (q.N + 255u) / 256u;
     const dim3 g(q.blocks), b(256);
+    if (rec19) {
+#define STRH_ADJL(W, NTV) hipLaunchKernelGGL((adjoint_les_kernel<W, NTV>), g, b, 0, s, rec19, lamNext, lam, \
+                                             flags, link, gUin, eps, ntIdx, gEps, q)
+        if (ntIdx) {
+            if (walls) STRH_ADJL(true, true); else STRH_ADJL(false, true);
+        } else {
+            if (walls) STRH_ADJL(true, false); else STRH_ADJL(false, false);
+        }
+#undef STRH_ADJL
+        return hipGetLastError();
+    }
 #define STRH_ADJ(W, NTV) hipLaunchKernelGGL((adjoint_step_kernel<W, NTV>), g, b, 0, s, rec, lamNext, lam, \
                                             flags, link, alpha, gAlpha, gUin, eps, ntIdx, recNt, gEps, q)
     if (ntIdx) {

@@ -1194,9 +1194,9 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
         return fail(STRH_ERR_INVALID, "objective columns must lie in [-1, nx)");
     if (!std::isfinite(o->w_p) || !std::isfinite(o->w_d) || !std::isfinite(o->w_f))
         return fail(STRH_ERR_INVALID, "objective weights must be finite");
-    if (L->cSmago > 0.0f || L->spongeOn)
-        return fail(STRH_ERR_UNSUPPORTED, "the adjoint covers walls, Guo forcing, Brinkman porosity and "
-                                          "partial cells; LES and the sponge are not differentiated");
+    if (L->alpha && (L->useEps || L->cSmago > 0.0f))
+        return fail(STRH_ERR_UNSUPPORTED, "the adjoint does not combine Brinkman porosity with partial "
+                                          "cells or LES");
     P.T = o->spinup + o->avg;
     std::vector<uint8_t> fl(L->N);
     HIP_TRY(hipMemcpy(fl.data(), L->flags, L->N, hipMemcpyDeviceToHost), "copy flags");
@@ -1230,15 +1230,17 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
 // float4) when given, objective accumulated into jacc when given.
 // Partial-cell lattices also record the 19 populations of their M cells with
 // eps > 0 into recNt (count * 19 * M floats) when given.
+// LES lattices record all 19 populations of every cell into rec19 (count *
+// 19 * N floats) instead.
 int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_t k0, int64_t count,
                 const strh_objective* o, const AdjPlan& P, float4* rec, double* jacc,
-                const int32_t* ntIdx = nullptr, float* recNt = nullptr) {
+                const int32_t* ntIdx = nullptr, float* recNt = nullptr, float* rec19 = nullptr) {
     const uint32_t M = (uint32_t)P.ntCells.size();
     for (int64_t s = 0; s < count; s++) {
         const int64_t k = k0 + s;
         StepLaunch sl = make_launch(L, t0 + k, false);
         sl.f = f;
-        sl.alpha = L->useEps ? nullptr : alpha;
+        sl.alpha = (L->useEps || L->cSmago > 0.0f) ? nullptr : alpha;
         AdjHooks hk;
         hk.rec = rec ? rec + (size_t)s * L->N : nullptr;
         const bool win = k >= o->spinup;
@@ -1252,6 +1254,7 @@ int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_
         hk.recNt = (recNt && ntIdx) ? recNt + (size_t)s * 19 * M : nullptr;
         hk.M = M;
         hk.wf = win ? P.wf : 0.0;
+        hk.rec19 = rec19 ? rec19 + (size_t)s * 19 * L->N : nullptr;
         HIP_TRY(launch_step_hooked(sl, hk, L->ctx->stream), "hooked step launch");
     }
     return STRH_OK;
@@ -1286,7 +1289,7 @@ int strh_objective_eval(strh_lattice* L, const strh_objective* o, double* J) {
     if ((rc = W.get(&jacc, (size_t)L->N * 8, "objective accumulator"))) return rc;
     HIP_TRY(launch_fill_zero(jacc, (size_t)L->N * 8, L->ctx->stream), "clear");
     const float* alpha = L->alpha;
-    if (!alpha && !L->useEps) {
+    if (!alpha && !L->useEps && !(L->cSmago > 0.0f)) {
         if ((rc = W.get(&zeroAlpha, (size_t)L->N * 4, "zero porosity"))) return rc;
         HIP_TRY(launch_fill_zero(zeroAlpha, (size_t)L->N * 4, L->ctx->stream), "clear");
         alpha = zeroAlpha;
@@ -1319,6 +1322,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     const size_t N = L->N, S = L->S;
     const size_t ddf = ddf_bytes(L);
     const size_t M = P.ntCells.size();
+    const bool les = L->cSmago > 0.0f;
     if (K < -1) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= -1");
     if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
     if (K == -1) {
@@ -1328,8 +1332,9 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
         const double fixed = (double)N * (3 * 8 + 4) + 2.0 * 19 * S * 4 + 2.0 * ddf;
         const double budget = 0.5 * (double)freeB - fixed;
-        // recording (moments, partial-cell populations) + its share of checkpoints
-        const double perStep = (double)N * 16 + (double)M * 76 + (double)ddf / 64.0;
+        // recording (moments and partial-cell populations, or all populations
+        // under LES) + its share of checkpoints
+        const double perStep = (les ? (double)N * 76 : (double)N * 16 + (double)M * 76) + (double)ddf / 64.0;
         K = budget > perStep ? (int64_t)(budget / perStep) : 1;
     }
     if (K > T) K = T;
@@ -1339,7 +1344,8 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
 
     AdjWork W;
     double *jacc, *gA, *gU;
-    float4* rec;
+    float4* rec = nullptr;
+    float* rec19 = nullptr;
     float *lam0, *lam1, *zeroAlpha = nullptr;
     uint8_t *ck, *work;
     if ((rc = W.get(&jacc, N * 8, "objective accumulator"))) return rc;
@@ -1347,11 +1353,15 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     if ((rc = W.get(&gU, N * 8, "inlet gradient"))) return rc;
     if ((rc = W.get(&lam0, 19 * S * 4, "adjoint state"))) return rc;
     if ((rc = W.get(&lam1, 19 * S * 4, "adjoint state"))) return rc;
-    if ((rc = W.get(&rec, (size_t)K * N * 16, "recorded moments"))) return rc;
+    if (les) {
+        if ((rc = W.get(&rec19, (size_t)K * 19 * N * 4, "recorded populations"))) return rc;
+    } else {
+        if ((rc = W.get(&rec, (size_t)K * N * 16, "recorded moments"))) return rc;
+    }
     if ((rc = W.get(&ck, (size_t)(C > 1 ? C - 1 : 1) * ddf, "checkpoints"))) return rc;
     if ((rc = W.get(&work, ddf, "replay DDFs"))) return rc;
     const float* alpha = L->alpha;
-    if (!alpha && !L->useEps) {
+    if (!alpha && !L->useEps && !les) {
         if ((rc = W.get(&zeroAlpha, N * 4, "zero porosity"))) return rc;
         HIP_TRY(launch_fill_zero(zeroAlpha, N * 4, st), "clear");
         alpha = zeroAlpha;
@@ -1362,7 +1372,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     if (L->useEps) {
         const size_t m = std::max<size_t>(M, 1);
         if ((rc = W.get(&ntIdx, N * 4, "partial-cell index"))) return rc;
-        if ((rc = W.get(&recNt, (size_t)K * 19 * m * 4, "recorded partial cells"))) return rc;
+        if (!les && (rc = W.get(&recNt, (size_t)K * 19 * m * 4, "recorded partial cells"))) return rc;
         if ((rc = W.get(&gE, m * 8, "solid-fraction gradient"))) return rc;
         HIP_TRY(hipMemcpy(ntIdx, P.ntIdx.data(), N * 4, hipMemcpyHostToDevice), "copy index");
         HIP_TRY(launch_fill_zero(gE, m * 8, st), "clear");
@@ -1382,7 +1392,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         if (!last)
             HIP_TRY(hipMemcpyAsync(ck + (size_t)seg * ddf, L->f, ddf, hipMemcpyDeviceToDevice, st), "checkpoint");
         if ((rc = adj_forward(L, L->f, alpha, t0, k0, len, o, P, last ? rec : nullptr, jacc, ntIdx,
-                              last ? recNt : nullptr))) return rc;
+                              last ? recNt : nullptr, last ? rec19 : nullptr))) return rc;
     }
     HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
     HIP_TRY(hipEventSynchronize(c->ev1), "forward pass");
@@ -1398,7 +1408,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         if (seg != C - 1) {
             HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
             HIP_TRY(hipMemcpyAsync(work, ck + (size_t)seg * ddf, ddf, hipMemcpyDeviceToDevice, st), "restore");
-            if ((rc = adj_forward(L, work, alpha, t0, k0, len, o, P, rec, nullptr, ntIdx, recNt))) return rc;
+            if ((rc = adj_forward(L, work, alpha, t0, k0, len, o, P, rec, nullptr, ntIdx, recNt, rec19))) return rc;
             HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
             HIP_TRY(hipEventSynchronize(c->ev1), "replay");
             tr += elapsed(c);
@@ -1424,9 +1434,13 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
             q.wd = win ? (float)P.wd : 0.0f;
             q.wf = win ? (float)P.wf : 0.0f;
             q.M = (uint32_t)M;
-            HIP_TRY(launch_adjoint_step(q, L->walls, rec + (size_t)(k - k0) * N, lamN, lamT,
+            q.spongeX0 = sp.spongeX0; q.spongeInvW = sp.spongeInvW; q.spongeTau = sp.spongeTau;
+            q.cSmago = sp.cSmago;
+            q.srt = sp.omegaMinus == sp.omega ? 1 : 0;
+            HIP_TRY(launch_adjoint_step(q, L->walls, rec ? rec + (size_t)(k - k0) * N : nullptr, lamN, lamT,
                                         L->flags, L->link, alpha, gA, gU, L->useEps ? L->eps : nullptr,
-                                        ntIdx, recNt ? recNt + (size_t)(k - k0) * 19 * M : nullptr, gE, st),
+                                        ntIdx, recNt ? recNt + (size_t)(k - k0) * 19 * M : nullptr, gE,
+                                        rec19 ? rec19 + (size_t)(k - k0) * 19 * N : nullptr, st),
                     "adjoint step launch");
             std::swap(lamN, lamT);
         }

@@ -9,7 +9,9 @@ held to it:
   forward's drift from FP64) and the gradients within 1e-4 of the largest
   component of the FP64 oracle's (the bar for gradients is 1e-4);
 * the gradient does not depend on the checkpoint interval, bit for bit;
-* lattices with features the adjoint does not cover are refused.
+* lattices with features the adjoint does not cover are refused;
+* Smagorinsky LES and the outlet sponge (rates that depend on the state and
+  on x), with and without partial cells, SRT and TRT, match the FP64 oracle.
 """
 from __future__ import annotations
 
@@ -162,8 +164,8 @@ def test_objective_eval_matches_vjp(gpu):
 def test_vjp_refuses_unsupported_lattices(gpu):
     from strouhal_amd import StrouhalError
     fl = _channel(18, 10)
-    g, _ = _pair(gpu, "fp32", fl, None)
-    g.c_smago = 0.02
+    g, _ = _pair(gpu, "fp32", fl, _alpha(fl.shape, 1))
+    g.c_smago = 0.02                      # porosity with LES
     with pytest.raises(StrouhalError) as e:
         g.vjp(**OBJ)
     assert e.value.code == 8
@@ -225,3 +227,69 @@ def test_nt_recording_forward_keeps_the_state_and_interval(gpu, precision):
     J2, _, u2, i2 = c.vjp(**NT_OBJ, checkpoint_every=-1, want_eps=True)
     assert a.state_digest == b.state_digest == c.state_digest
     assert J1 == J2 and u1 == u2 and np.array_equal(i1["grad_eps"], i2["grad_eps"])
+
+
+def _les_pair(gpu, precision, lam, nt, nz=1):
+    """The oracle's LES + sponge case: tau0 = 0.56, Cs^2 = 0.3 (strong eddy
+    viscosity, so the rate chain carries weight), sponge over x > 15."""
+    from strouhal_amd import Simulation
+    from strouhal_amd.cases import disk_solid_fractions, sphere_solid_fractions
+    nx, ny = 22, 12
+    fl = _channel(nx, ny, nz)
+    tau = 0.56
+    wp = 1.0 / tau
+    wm = wp if lam is None else 1.0 / (0.5 + lam / (tau - 0.5))
+    g = Simulation(gpu, precision, nx=nx, ny=ny, nz=nz, omega=wp, omega_minus=wm, uin=0.06,
+                   ramp_steps=3, c_smago=0.3, flags=fl)
+    g.sponge = (15.0, 5.0, 0.9)
+    eps = None
+    if nt:
+        if nz > 1:
+            eps = sphere_solid_fractions(nx, ny, nz, 8.3, ny / 2 - 0.3, nz / 2 + 0.2, 2.7).astype(np.float32)
+        else:
+            eps = disk_solid_fractions(nx, ny, 8.3, 5.7, 2.6)[None].astype(np.float32)
+        g.set_solid_fractions(eps)
+    g.init_field(0, 0.0)
+    r = OracleSim(nx, ny, nz, g.omega, g.omega_minus, arith="f64", storage="f64", uin=g.uin_target,
+                  ramp_steps=g.ramp_steps, c_smago=g.c_smago, flags=fl, threads=8)
+    r.sponge = g.sponge
+    if nt:
+        r.set_solid_fractions(eps)
+    r.init_field(0, 0.0)
+    return g, r, eps
+
+
+LES_OBJ = dict(inlet_x=2, outlet_x=19, spinup=4, avg=9, w_p=1.0, w_d=0.0, w_f=2.0)
+
+
+@pytest.mark.parametrize("lam,nt,nz", [(None, True, 1), (3.0 / 16.0, True, 1), (None, False, 1),
+                                       (3.0 / 16.0, True, 7)])
+def test_vjp_les_and_sponge_match_fp64_oracle(gpu, lam, nt, nz):
+    """dJ/deps and dJ/du_in through the Smagorinsky rate chain and the sponge
+    ramp, SRT and TRT, 2-D disk, plain channel and 3-D sphere."""
+    g, r, eps = _les_pair(gpu, "fp32", lam, nt, nz)
+    J, _, gu, info = g.vjp(**LES_OBJ, checkpoint_every=5, want_eps=nt)
+    if nt:
+        Jr, _, ger, gur = r.vjp(**LES_OBJ, with_eps=True)
+    else:
+        Jr, _, gur = r.vjp(**LES_OBJ)
+    assert J == pytest.approx(Jr, rel=1e-6)
+    assert gu == pytest.approx(gur, rel=1e-4)
+    if nt:
+        ge = info["grad_eps"].reshape(-1)
+        _close(ge, ger, 1e-4)
+        assert np.all(ge[eps.reshape(-1) == 0] == 0.0)
+        assert np.count_nonzero(ge) > 10
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16s"])
+def test_les_recording_forward_keeps_the_state_and_interval(gpu, precision):
+    a, _, _ = _les_pair(gpu, precision, 3.0 / 16.0, True)
+    b, _, _ = _les_pair(gpu, precision, 3.0 / 16.0, True)
+    c, _, _ = _les_pair(gpu, precision, 3.0 / 16.0, True)
+    a.run(13)
+    J1, _, u1, i1 = b.vjp(**LES_OBJ, checkpoint_every=0, want_eps=True)
+    J2, _, u2, i2 = c.vjp(**LES_OBJ, checkpoint_every=-1, want_eps=True)
+    assert a.state_digest == b.state_digest == c.state_digest
+    assert J1 == J2 and u1 == u2 and np.array_equal(i1["grad_eps"], i2["grad_eps"])
+    assert i1["checkpoint_every"] == 4 and i2["checkpoint_every"] == 13
