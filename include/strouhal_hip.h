@@ -234,7 +234,11 @@ STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, d
  * solid fraction of every fluid cell whose solid fraction is non-zero
  * (grad_eps: N doubles, or NULL; zero elsewhere) and to the inlet velocity
  * target uin (grad_uin, or NULL). Partial-cell lattices also record the 19
- * populations of those cells per step. The state after the call
+ * populations of those cells per step; Smagorinsky LES lattices record the 19
+ * populations of every cell instead of the moments (the rates depend on the
+ * full non-equilibrium stress), and the outlet sponge enters as per-column
+ * rates. Porosity does not combine with partial cells or LES
+ * (STRH_ERR_UNSUPPORTED). The state after the call
  * is the state after strh_run(lat, T), bit for bit. The forward pass keeps a
  * DDF checkpoint at the start of every segment of K = checkpoint_every steps;
  * the reverse pass replays one segment at a time, recording the four
@@ -242,9 +246,11 @@ STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, d
  * collision's Jacobian depends on the state only through them), and runs the
  * reverse steps over it. The last segment is recorded in the forward pass, so
  * K >= T replays nothing. K = 0: ceil(sqrt(T)); K = -1: the largest K whose
- * recording fits half the free device memory. The adjoint state is FP32;
+ * recording and checkpoints fit 80% of the free device memory (the K needing
+ * least memory when none fits). The adjoint state is FP32;
  * every accumulation is FP64 in a fixed order, so the result does not depend
- * on K. Device memory: (ceil(T/K) - 1) DDF images + K * N * 16 bytes +
+ * on K. Device memory: (ceil(T/K) - 1) DDF images + K * N * 16 bytes (76
+ * under LES) +
  * 2 * 19 * N floats + 3 * N doubles + one DDF image. info may be NULL. */
 STRH_API int strh_vjp(strh_lattice* lat, const strh_objective* obj, int64_t checkpoint_every,
                       double* J, double* grad_alpha, double* grad_eps, double* grad_uin,

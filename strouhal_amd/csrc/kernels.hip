@@ -100,6 +100,17 @@ __device__ __forceinline__ uint32_t cell_elem(const DdfLayout& l, uint32_t n) {
     return (n >> l.lw) * l.bstride + (n & l.mask);
 }
 
+// Side arrays (adjoint state, recordings) at a 32-bit byte offset from a
+// uniform base, the scalar-base + vector-offset form: one VGPR per access
+// instead of a 64-bit address. Offsets stay below 4 GiB for every lattice of
+// at most 200M cells (two slots of N floats, or N float4).
+template <class T> __device__ __forceinline__ T ld_off(const T* base, uint32_t off) {
+    return *(const T*)((const char*)base + off);
+}
+template <class T> __device__ __forceinline__ void st_off(T* base, uint32_t off, T v) {
+    *(T*)((char*)base + off) = v;
+}
+
 // Workgroup b of G runs on XCD b % 8. Give each XCD a contiguous range of
 // logical blocks so neighbouring cell ranges share an L2.
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
@@ -418,7 +429,8 @@ __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[
     const float fx = p.fx;
     if (hk.rec19) {
 #pragma unroll
-        for (int i = 0; i < 19; i++) __builtin_nontemporal_store(fh[i], hk.rec19 + (size_t)i * p.N + n);
+        for (int i = 0; i < 19; i++)
+            __builtin_nontemporal_store(fh[i], (float*)((char*)(hk.rec19 + (size_t)i * p.N) + n * 4u));
     }
     if (hk.recNt && eps > 0.0f) {
         const int32_t k = hk.ntIdx[n];
@@ -433,7 +445,10 @@ __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[
         for (int i = 1; i < 19; i += 2) nt -= 2.0 * (double)kCx[i] * B * ((double)fh[i + 1] - (double)fh[i]);
         hk.jacc[n] += hk.wf * nt;
     }
-    if (!hk.rec && !hk.jacc) return;
+    // the FP64 moments serve the recording and the density / porous-drag terms
+    const double cp = (x == hk.inletX ? hk.cin : 0.0) - (x == hk.outletX ? hk.cout : 0.0);
+    const bool obj = hk.jacc && (cp != 0.0 || (hk.wd != 0.0 && alpha != 0.0f));
+    if (!hk.rec && !obj) return;
     double rm1 = 0.0, px = 0.0, py = 0.0, pz = 0.0;
 #pragma unroll
     for (int i = 0; i < 19; i++) {
@@ -448,13 +463,10 @@ __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[
         const f4v m = {(float)rm1, (float)px, (float)py, (float)pz};
         __builtin_nontemporal_store(m, reinterpret_cast<f4v*>(hk.rec) + n);
     }
-    if (hk.jacc) {
-        const double cp = (x == hk.inletX ? hk.cin : 0.0) - (x == hk.outletX ? hk.cout : 0.0);
-        if (cp != 0.0 || (hk.wd != 0.0 && alpha != 0.0f)) {
-            const double a = (double)alpha;
-            const double Jx = (px + 0.5 * (double)fx) / (1.0 + 0.5 * a);
-            hk.jacc[n] += cp * (1.0 + rm1) + hk.wd * a * Jx;
-        }
+    if (obj) {
+        const double a = (double)alpha;
+        const double Jx = (px + 0.5 * (double)fx) / (1.0 + 0.5 * a);
+        hk.jacc[n] += cp * (1.0 + rm1) + hk.wd * a * Jx;
     }
 }
 
@@ -532,26 +544,46 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     uint32_t deferred = 0u;
+    // Slots I and opp(I) are adjacent (kOpp pairs 2k-1, 2k), so both reads of
+    // direction I are addressed from the lower slot of the pair: the wall read
+    // (own cell, slot I) and the pull (neighbour, slot opp(I)) differ only in
+    // a per-lane 32-bit offset. The load is then unconditional and the 18
+    // pulls issue back to back instead of each waiting inside its own branch.
+    // Offsets stay below 4 GiB: a blocked image is under 4 GiB, a single-block
+    // one spans at most two slots of N values here.
+    const uint32_t WB = p.lay.W * (uint32_t)TB;
     fh[0] = St::rd(fs[0], nb);
 #define STRH_PULL(I)                                                                  \
     {                                                                                 \
         constexpr int srcBit = kOpp[I] - 1;                                           \
-        if (WALLS && ((sM >> srcBit) & 1u)) {                                         \
-            fh[I] = St::rd(fs[I], nb);                                                \
-            if ((lM >> srcBit) & 1u) deferred |= 1u << I;                             \
-            if (p.writeForce != 0) {                                                  \
-                ax = acc<-kCx[I]>(ax, fh[I]);                                         \
-                ay = acc<-kCy[I]>(ay, fh[I]);                                         \
-                az = acc<-kCz[I]>(az, fh[I]);                                         \
-            }                                                                         \
-        } else {                                                                      \
-            fh[I] = St::rd(fs[kOpp[I]], nbo(o.at<-kCx[I], -kCy[I], -kCz[I]>()));        \
-        }                                                                             \
+        constexpr int lo = I < kOpp[I] ? I : kOpp[I];                                 \
+        const uint32_t pull = nbo(o.at<-kCx[I], -kCy[I], -kCz[I]>());                 \
+        uint32_t off = pull + (kOpp[I] == lo ? 0u : WB);                              \
+        if (WALLS && ((sM >> srcBit) & 1u)) off = nb + (I == lo ? 0u : WB);          \
+        fh[I] = St::rd(fs[lo], off);                                                  \
     }
     STRH_PULL(1) STRH_PULL(2) STRH_PULL(3) STRH_PULL(4) STRH_PULL(5) STRH_PULL(6)
     STRH_PULL(7) STRH_PULL(8) STRH_PULL(9) STRH_PULL(10) STRH_PULL(11) STRH_PULL(12)
     STRH_PULL(13) STRH_PULL(14) STRH_PULL(15) STRH_PULL(16) STRH_PULL(17) STRH_PULL(18)
 #undef STRH_PULL
+    if (WALLS && sM != 0u) {   // wall links: moving-wall corrections to apply, force terms
+#define STRH_WALLIN(I)                                                                \
+        {                                                                             \
+            constexpr int srcBit = kOpp[I] - 1;                                       \
+            if ((sM >> srcBit) & 1u) {                                                \
+                if ((lM >> srcBit) & 1u) deferred |= 1u << I;                         \
+                if (p.writeForce != 0) {                                              \
+                    ax = acc<-kCx[I]>(ax, fh[I]);                                     \
+                    ay = acc<-kCy[I]>(ay, fh[I]);                                     \
+                    az = acc<-kCz[I]>(az, fh[I]);                                     \
+                }                                                                     \
+            }                                                                         \
+        }
+        STRH_WALLIN(1) STRH_WALLIN(2) STRH_WALLIN(3) STRH_WALLIN(4) STRH_WALLIN(5) STRH_WALLIN(6)
+        STRH_WALLIN(7) STRH_WALLIN(8) STRH_WALLIN(9) STRH_WALLIN(10) STRH_WALLIN(11) STRH_WALLIN(12)
+        STRH_WALLIN(13) STRH_WALLIN(14) STRH_WALLIN(15) STRH_WALLIN(16) STRH_WALLIN(17) STRH_WALLIN(18)
+#undef STRH_WALLIN
+    }
 
     if (WALLS && deferred != 0u) {                                 // 292-314
 #define STRH_DEFER(I)                                                                 \
@@ -596,35 +628,52 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     asm("" : "+v"(nip));
     auto nbp = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB; };
     St::wr(fs[0], nbs, fh[0]);
+    // Wall links store sign * f - corr into the cell's own opposite slot (the
+    // halfway bounce-back, with the moving-wall term or the outflow's sign
+    // flip); with sign 1 and corr 0 that is f itself, bit for bit. The
+    // corrections are computed in a block only wall cells enter; the store
+    // is addressed like the pull, from the lower slot of the pair.
+    if (WALLS && sM != 0u) {
+#define STRH_WALLOUT(I)                                                               \
+        {                                                                             \
+            constexpr int dstBit = I - 1;                                             \
+            if ((sM >> dstBit) & 1u) {                                                \
+                float corr = 0.0f;                                                    \
+                float sign = 1.0f;                                                    \
+                if ((lM >> dstBit) & 1u) {                                            \
+                    const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
+                    const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>()); \
+                    const uint8_t wf = flags[d];                                      \
+                    if (wf == kOutflow) {                                             \
+                        sign = -1.0f;                                                 \
+                    } else {                                                          \
+                        float uwx, uwy, uwz;                                          \
+                        wall_vel(wf, dy, p, uwx, uwy, uwz);                           \
+                        corr = 6.0f * wgt<I>()                                        \
+                             * ((float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz); \
+                    }                                                                 \
+                }                                                                     \
+                if (p.writeForce != 0) {                                              \
+                    const float v = sign * fh[I] + 2.0f * wgt<I>();                   \
+                    ax = acc<kCx[I]>(ax, v);                                          \
+                    ay = acc<kCy[I]>(ay, v);                                          \
+                    az = acc<kCz[I]>(az, v);                                          \
+                }                                                                     \
+                fh[I] = sign * fh[I] - corr;                                          \
+            }                                                                         \
+        }
+        STRH_WALLOUT(1) STRH_WALLOUT(2) STRH_WALLOUT(3) STRH_WALLOUT(4) STRH_WALLOUT(5) STRH_WALLOUT(6)
+        STRH_WALLOUT(7) STRH_WALLOUT(8) STRH_WALLOUT(9) STRH_WALLOUT(10) STRH_WALLOUT(11) STRH_WALLOUT(12)
+        STRH_WALLOUT(13) STRH_WALLOUT(14) STRH_WALLOUT(15) STRH_WALLOUT(16) STRH_WALLOUT(17) STRH_WALLOUT(18)
+#undef STRH_WALLOUT
+    }
 #define STRH_PUSH(I)                                                                  \
     {                                                                                 \
         constexpr int dstBit = I - 1;                                                 \
-        if (WALLS && ((sM >> dstBit) & 1u)) {                                         \
-            float corr = 0.0f;                                                        \
-            float sign = 1.0f;                                                        \
-            if ((lM >> dstBit) & 1u) {                                                \
-                const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
-                const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>());     \
-                const uint8_t wf = flags[d];                                          \
-                if (wf == kOutflow) {                                                 \
-                    sign = -1.0f;                                                     \
-                } else {                                                              \
-                    float uwx, uwy, uwz;                                              \
-                    wall_vel(wf, dy, p, uwx, uwy, uwz);                               \
-                    corr = 6.0f * wgt<I>()                                            \
-                         * ((float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz); \
-                }                                                                     \
-            }                                                                         \
-            St::wr(fs[kOpp[I]], nbs, sign * fh[I] - corr);                            \
-            if (p.writeForce != 0) {                                                  \
-                const float v = sign * fh[I] + 2.0f * wgt<I>();                       \
-                ax = acc<kCx[I]>(ax, v);                                              \
-                ay = acc<kCy[I]>(ay, v);                                              \
-                az = acc<kCz[I]>(az, v);                                              \
-            }                                                                         \
-        } else {                                                                      \
-            St::wr(fs[I], nbp(o.at<kCx[I], kCy[I], kCz[I]>()), fh[I]);                \
-        }                                                                             \
+        constexpr int lo = I < kOpp[I] ? I : kOpp[I];                                 \
+        uint32_t off = nbp(o.at<kCx[I], kCy[I], kCz[I]>()) + (I == lo ? 0u : WB);     \
+        if (WALLS && ((sM >> dstBit) & 1u)) off = nbs + (kOpp[I] == lo ? 0u : WB);    \
+        St::wr(fs[lo], off, fh[I]);                                                   \
     }
     STRH_PUSH(1) STRH_PUSH(2) STRH_PUSH(3) STRH_PUSH(4) STRH_PUSH(5) STRH_PUSH(6)
     STRH_PUSH(7) STRH_PUSH(8) STRH_PUSH(9) STRH_PUSH(10) STRH_PUSH(11) STRH_PUSH(12)
@@ -692,8 +741,8 @@ template <int I> __device__ __forceinline__ float wgt_f() {
 template <bool WALLS>
 __device__ __forceinline__ double adj_gather(const AdjStep& q, uint32_t n, int x, int y, int z,
                                              const float* __restrict__ lamN,
-                                             const uint8_t* __restrict__ flags,
-                                             const uint32_t* __restrict__ link, float (&mu)[19]) {
+                                             const uint8_t* __restrict__ flags, uint32_t sM,
+                                             float (&mu)[19]) {
     const uint32_t S = q.S;
     const int nx = (int)q.nx, ny = (int)q.ny, nz = (int)q.nz;
     const int plane = nx * ny;
@@ -704,16 +753,29 @@ __device__ __forceinline__ double adj_gather(const AdjStep& q, uint32_t n, int x
     o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
     o.zm = (z == 0) ? (nz - 1) * plane : -plane;
     o.zp = (z == nz - 1) ? -(nz - 1) * plane : plane;
-    const uint32_t sM = WALLS ? (link[n] & 0x3ffffu) : 0u;
     double gu = 0.0;
-    mu[0] = lamN[n];
+    mu[0] = ld_off(lamN, n * 4u);
+    // unconditional loads (a wall link reads the cell's own opposite slot),
+    // then the rare wall terms, so the 18 loads issue back to back
 #define STRH_GATHER(I)                                                                \
     {                                                                                 \
         const uint32_t d = (uint32_t)((int)n + o.at<kCx[I], kCy[I], kCz[I]>());       \
-        if (WALLS && ((sM >> (I - 1)) & 1u)) {                                        \
-            const float ln = lamN[kOpp[I] * S + n];                                   \
+        constexpr int lo = I < kOpp[I] ? I : kOpp[I];                                 \
+        uint32_t off = (d + (I == lo ? 0u : S)) * 4u;                                 \
+        if (WALLS && ((sM >> (I - 1)) & 1u)) off = (n + (kOpp[I] == lo ? 0u : S)) * 4u; \
+        mu[I] = ld_off(lamN + (size_t)lo * S, off);                                   \
+    }
+    STRH_GATHER(1) STRH_GATHER(2) STRH_GATHER(3) STRH_GATHER(4) STRH_GATHER(5) STRH_GATHER(6)
+    STRH_GATHER(7) STRH_GATHER(8) STRH_GATHER(9) STRH_GATHER(10) STRH_GATHER(11) STRH_GATHER(12)
+    STRH_GATHER(13) STRH_GATHER(14) STRH_GATHER(15) STRH_GATHER(16) STRH_GATHER(17) STRH_GATHER(18)
+#undef STRH_GATHER
+    if (WALLS && sM != 0u) {
+#define STRH_GWALL(I)                                                                 \
+        if ((sM >> (I - 1)) & 1u) {                                                   \
+            const uint32_t d = (uint32_t)((int)n + o.at<kCx[I], kCy[I], kCz[I]>());   \
             const uint8_t wf = flags[d];                                              \
-            mu[I] = wf == kOutflow ? -ln : ln;                                        \
+            const float ln = mu[I];                                                   \
+            if (wf == kOutflow) mu[I] = -ln;                                          \
             if (kCx[I] != 0 && wf == kInflow) {                                       \
                 const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
                 double prof = 1.0;                                                    \
@@ -724,14 +786,12 @@ __device__ __forceinline__ double adj_gather(const AdjStep& q, uint32_t n, int x
                 }                                                                     \
                 gu += (double)ln * (-6.0 * (double)wgt_f<I>() * (double)kCx[I] * q.rOdd * prof); \
             }                                                                         \
-        } else {                                                                      \
-            mu[I] = lamN[I * S + d];                                                  \
-        }                                                                             \
+        }
+        STRH_GWALL(1) STRH_GWALL(2) STRH_GWALL(3) STRH_GWALL(4) STRH_GWALL(5) STRH_GWALL(6)
+        STRH_GWALL(7) STRH_GWALL(8) STRH_GWALL(9) STRH_GWALL(10) STRH_GWALL(11) STRH_GWALL(12)
+        STRH_GWALL(13) STRH_GWALL(14) STRH_GWALL(15) STRH_GWALL(16) STRH_GWALL(17) STRH_GWALL(18)
+#undef STRH_GWALL
     }
-    STRH_GATHER(1) STRH_GATHER(2) STRH_GATHER(3) STRH_GATHER(4) STRH_GATHER(5) STRH_GATHER(6)
-    STRH_GATHER(7) STRH_GATHER(8) STRH_GATHER(9) STRH_GATHER(10) STRH_GATHER(11) STRH_GATHER(12)
-    STRH_GATHER(13) STRH_GATHER(14) STRH_GATHER(15) STRH_GATHER(16) STRH_GATHER(17) STRH_GATHER(18)
-#undef STRH_GATHER
     return gu;
 }
 
@@ -787,16 +847,22 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
                     const float* __restrict__ recNt, double* __restrict__ gEps, AdjStep q) {
     const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
     if (n >= q.N) return;
-    if (flags[n] != kFluid) return;
+    uint32_t sM = 0u;   // lattices without walls hold fluid cells only
+    if (WALLS) {
+        const uint32_t lw = link[n];
+        if (lw & kNotFluidBit) return;
+        sM = lw & 0x3ffffu;
+    }
     const uint32_t S = q.S;
+    const uint32_t n4 = n * 4u;
     const uint32_t t = fdiv(n, q.divx);
     const int x = (int)(n - t * q.nx);
     const uint32_t z = fdiv(t, q.divy);
     const int y = (int)(t - z * q.ny);
     float mu[19];
-    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, link, mu);
+    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, sM, mu);
 
-    const float4 m = rec[n];
+    const float4 m = ld_off(rec, n * 16u);
     const float a = alphaBuf ? alphaBuf[n] : 0.0f;
     const float rho = 1.0f + m.x, ir = 1.0f / rho, ir2 = ir * ir;
     const float gam = 1.0f + 0.5f * a, ig = 1.0f / gam;
@@ -877,7 +943,7 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
 #pragma unroll
             for (int i = 1; i < 19; i += 2) if (kCx[i]) dNt -= 2.0f * (float)kCx[i] * (g[i + 1] - g[i]);
             const float dObjX = q.wf * 2.0f * B;          // d ntF_x / d g_j = 2 B c_jx
-            lam[n] = mu[0] + omB * (-wp * mu[0] + gRho) + cp;
+            st_off(lam, n4, mu[0] + omB * (-wp * mu[0] + gRho) + cp);
 #define STRH_NTLAM(I)                                                                         \
             {                                                                                 \
                 const float mp = 0.5f * (mu[I] + mu[kOpp[I]]);                                \
@@ -888,7 +954,7 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
                 if (kCz[I]) om += (float)kCz[I] * gPz;                                        \
                 float v = mu[I] + omB * om + B * (mu[kOpp[I]] - mu[I]) + cp;                  \
                 if (kCx[I]) v += (float)kCx[I] * dObjX;                                       \
-                lam[I * S + n] = v;                                                           \
+                st_off(lam + (size_t)I * S, n4, v);                                            \
             }
             STRH_NTLAM(1) STRH_NTLAM(2) STRH_NTLAM(3) STRH_NTLAM(4) STRH_NTLAM(5) STRH_NTLAM(6)
             STRH_NTLAM(7) STRH_NTLAM(8) STRH_NTLAM(9) STRH_NTLAM(10) STRH_NTLAM(11) STRH_NTLAM(12)
@@ -899,7 +965,7 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
             return;
         }
     }
-    lam[n] = mu[0] - wp * mu[0] + base;
+    st_off(lam, n4, mu[0] - wp * mu[0] + base);
 #define STRH_LAM(I)                                                                   \
     {                                                                                 \
         const float mp = 0.5f * (mu[I] + mu[kOpp[I]]);                                \
@@ -908,7 +974,7 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
         if (kCx[I]) v += (float)kCx[I] * dPx;                                         \
         if (kCy[I]) v += (float)kCy[I] * gPy;                                         \
         if (kCz[I]) v += (float)kCz[I] * gPz;                                         \
-        lam[I * S + n] = v;                                                           \
+        st_off(lam + (size_t)I * S, n4, v);                                            \
     }
     STRH_LAM(1) STRH_LAM(2) STRH_LAM(3) STRH_LAM(4) STRH_LAM(5) STRH_LAM(6)
     STRH_LAM(7) STRH_LAM(8) STRH_LAM(9) STRH_LAM(10) STRH_LAM(11) STRH_LAM(12)
@@ -917,7 +983,7 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     const float gJdotJ = gJx * Jx + gJy * Jy + gJz * Jz;
     const float gFdotJ = gFx * Jx + gFy * Jy + gFz * Jz;
     const float k = 1.0f - 0.5f * a * ig;
-    gAlpha[n] += (double)(-0.5f * gJdotJ * ig - gFdotJ * k + wd * Jx * k);
+    st_off(gAlpha, n * 8u, ld_off(gAlpha, n * 8u) + (double)(-0.5f * gJdotJ * ig - gFdotJ * k + wd * Jx * k));
     if (gu != 0.0) gUin[n] += gu;
 }
 
@@ -944,38 +1010,53 @@ adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ la
                    double* __restrict__ gEps, AdjStep q) {
     const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
     if (n >= q.N) return;
-    if (flags[n] != kFluid) return;
+    uint32_t sM = 0u;   // lattices without walls hold fluid cells only
+    if (WALLS) {
+        const uint32_t lw = link[n];
+        if (lw & kNotFluidBit) return;
+        sM = lw & 0x3ffffu;
+    }
     const uint32_t S = q.S;
+    const uint32_t n4 = n * 4u;
     const uint32_t t = fdiv(n, q.divx);
     const int x = (int)(n - t * q.nx);
     const uint32_t z = fdiv(t, q.divy);
     const int y = (int)(t - z * q.ny);
     float mu[19];
-    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, link, mu);
+    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, sM, mu);
 
     float g[19];
 #pragma unroll
-    for (int i = 0; i < 19; i++) g[i] = rec19[(size_t)i * q.N + n];
+    for (int i = 0; i < 19; i++) g[i] = ld_off(rec19 + (size_t)i * q.N, n4);
+    // moments of g: rho - 1, p and the second moments sum c_a c_b g
     float rm1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 19; i++) {
-        rm1 += g[i];
-        px += (float)kCx[i] * g[i];
-        py += (float)kCy[i] * g[i];
-        pz += (float)kCz[i] * g[i];
+    float sxx = 0.0f, syy = 0.0f, szz = 0.0f, sxy = 0.0f, sxz = 0.0f, syz = 0.0f;
+#define STRH_GMOM(I)                                                                  \
+    {                                                                                 \
+        const float v = g[I];                                                         \
+        rm1 += v;                                                                     \
+        if (kCx[I]) { px += (float)kCx[I] * v; sxx += v; }                            \
+        if (kCy[I]) { py += (float)kCy[I] * v; syy += v; }                            \
+        if (kCz[I]) { pz += (float)kCz[I] * v; szz += v; }                            \
+        if (kCx[I] * kCy[I]) sxy += (float)(kCx[I] * kCy[I]) * v;                     \
+        if (kCx[I] * kCz[I]) sxz += (float)(kCx[I] * kCz[I]) * v;                     \
+        if (kCy[I] * kCz[I]) syz += (float)(kCy[I] * kCz[I]) * v;                     \
     }
+    STRH_GMOM(0) STRH_GMOM(1) STRH_GMOM(2) STRH_GMOM(3) STRH_GMOM(4) STRH_GMOM(5) STRH_GMOM(6)
+    STRH_GMOM(7) STRH_GMOM(8) STRH_GMOM(9) STRH_GMOM(10) STRH_GMOM(11) STRH_GMOM(12)
+    STRH_GMOM(13) STRH_GMOM(14) STRH_GMOM(15) STRH_GMOM(16) STRH_GMOM(17) STRH_GMOM(18)
+#undef STRH_GMOM
     const float rho = 1.0f + rm1, ir = 1.0f / rho, ir2 = ir * ir;
     const float fx = q.fx, fy = q.fy, fz = q.fz;
     const float Jx = px + 0.5f * fx, Jy = py + 0.5f * fy, Jz = pz + 0.5f * fz;
-    const float ux = Jx * ir, uy = Jy * ir, uz = Jz * ir;
-    const float u2 = ux * ux + uy * uy + uz * uz;
-    const float uF = ux * fx + uy * fy + uz * fz;
 
-    // local rates: sponge, then Smagorinsky (the forward's float arithmetic)
+    // local rates: sponge, then Smagorinsky with Pi_neq in closed form,
+    // Pi_ab = sum c_a c_b g - delta_ab (rho - 1)/3 - J_a J_b / rho
     float wp, wm;
     adj_sponge(q, x, wp, wm);
-    float pxx, pyy, pzz, pxy, pxz, pyz;
-    neq_stress(g, rm1, rho, ux, uy, uz, u2, pxx, pyy, pzz, pxy, pxz, pyz);
+    const float r3 = rm1 * (1.0f / 3.0f);
+    const float pxx = sxx - r3 - Jx * Jx * ir, pyy = syy - r3 - Jy * Jy * ir, pzz = szz - r3 - Jz * Jz * ir;
+    const float pxy = sxy - Jx * Jy * ir, pxz = sxz - Jx * Jz * ir, pyz = syz - Jy * Jz * ir;
     const float Q = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
     const float tau0 = 1.0f / wp;
     const float Kc = 25.455844f * q.cSmago;
@@ -1007,36 +1088,31 @@ adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ la
     const float JF = Jx * fx + Jy * fy + Jz * fz;
     const float gRho = wp * (M0 - 4.5f * JMJ * ir2 + 1.5f * J2 * M0 * ir2)
                      - ap * (9.0f * JMF - 3.0f * JF * M0) * ir2;
-    const float gPx = wp * (9.0f * MJx - 3.0f * Jx * M0) * ir + 3.0f * wm * Mx + ap * (9.0f * MFx - 3.0f * fx * M0) * ir;
-    const float gPy = wp * (9.0f * MJy - 3.0f * Jy * M0) * ir + 3.0f * wm * My + ap * (9.0f * MFy - 3.0f * fy * M0) * ir;
-    const float gPz = wp * (9.0f * MJz - 3.0f * Jz * M0) * ir + 3.0f * wm * Mz + ap * (9.0f * MFz - 3.0f * fz * M0) * ir;
+    float gPx = wp * (9.0f * MJx - 3.0f * Jx * M0) * ir + 3.0f * wm * Mx + ap * (9.0f * MFx - 3.0f * fx * M0) * ir;
+    float gPy = wp * (9.0f * MJy - 3.0f * Jy * M0) * ir + 3.0f * wm * My + ap * (9.0f * MFy - 3.0f * fy * M0) * ir;
+    float gPz = wp * (9.0f * MJz - 3.0f * Jz * M0) * ir + 3.0f * wm * Mz + ap * (9.0f * MFz - 3.0f * fz * M0) * ir;
 
-    // mu . dOmega/dw+, mu . dOmega/dw-, and mu . Omega for partial cells
-    float gwp, gwm = 0.0f, muOm = 0.0f;
-    {
-        const float feq0 = (1.0f / 3.0f) * (rm1 - 1.5f * rho * u2);
-        const float x0 = -3.0f * uF;
-        gwp = mu[0] * ((feq0 - g[0]) - 0.5f * (1.0f / 3.0f) * x0);
-        if (NT) muOm = mu[0] * (wp * (feq0 - g[0]) + (1.0f / 3.0f) * ap * x0);
-    }
-#define STRH_DOM(I)                                                                          \
-    {                                                                                        \
-        const float W = wgt_f<I>();                                                          \
-        const float cu = 3.0f * ((float)kCx[I] * ux + (float)kCy[I] * uy + (float)kCz[I] * uz); \
-        const float cF = (float)kCx[I] * fx + (float)kCy[I] * fy + (float)kCz[I] * fz;     \
-        const float ep = W * (rm1 + rho * (0.5f * cu * cu - 1.5f * u2)) - 0.5f * (g[I] + g[I + 1]); \
-        const float em = W * rho * cu - 0.5f * (g[I] - g[I + 1]);                           \
-        const float xp = 3.0f * cu * cF - 3.0f * uF, xm = 3.0f * cF;                        \
-        gwp += (mu[I] + mu[I + 1]) * (ep - 0.5f * W * xp);                                  \
-        gwm += (mu[I] - mu[I + 1]) * (em - 0.5f * W * xm);                                   \
-        if (NT) {                                                                            \
-            const float sp = wp * ep + W * ap * xp, sm = wm * em + W * am * xm;              \
-            muOm += mu[I] * (sp + sm) + mu[I + 1] * (sp - sm);                               \
-        }                                                                                    \
-    }
-    STRH_DOM(1) STRH_DOM(3) STRH_DOM(5) STRH_DOM(7) STRH_DOM(9)
-    STRH_DOM(11) STRH_DOM(13) STRH_DOM(15) STRH_DOM(17)
-#undef STRH_DOM
+    // Omega_j = w+ ep_j + w- em_j + W_j (a+ xp_j + a- xm_j) with
+    //   ep = feq+ - f+, em = feq- - f-, xp = 9 (c.u)(c.F) - 3 u.F, xm = 3 c.F;
+    // their mu-weighted sums from the moments of mu:
+    //   sum mu ep = M0 (rho - 1 - 1.5 J^2/rho) + 4.5 J.Mcc.J/rho - sum mu f+,
+    //   sum mu em = 3 Mc.J - sum mu f-,
+    //   sum mu W xp = 9 J.Mcc.F/rho - 3 (J.F/rho) M0,  sum mu W xm = 3 Mc.F.
+    float Sfp = 0.0f, Sfm = 0.0f;   // 2 sum over pairs of mu+ f+ and mu- f- terms
+#define STRH_FPM(I)                                                                   \
+    Sfp += (mu[I] + mu[I + 1]) * (g[I] + g[I + 1]);                                   \
+    Sfm += (mu[I] - mu[I + 1]) * (g[I] - g[I + 1]);
+    STRH_FPM(1) STRH_FPM(3) STRH_FPM(5) STRH_FPM(7) STRH_FPM(9)
+    STRH_FPM(11) STRH_FPM(13) STRH_FPM(15) STRH_FPM(17)
+#undef STRH_FPM
+    Sfp = mu[0] * g[0] + 0.5f * Sfp;
+    Sfm *= 0.5f;
+    const float Sep = M0 * (rm1 - 1.5f * J2 * ir) + 4.5f * JMJ * ir - Sfp;
+    const float Sem = 3.0f * (Mx * Jx + My * Jy + Mz * Jz) - Sfm;
+    const float Sxp = 9.0f * JMF * ir - 3.0f * JF * ir * M0;
+    const float Sxm = 3.0f * (Mx * fx + My * fy + Mz * fz);
+    float gwp = Sep - 0.5f * Sxp;
+    float gwm = Sem - 0.5f * Sxm;
 
     const float cp = (x == q.inletX ? q.cin : 0.0f) - (x == q.outletX ? q.cout : 0.0f);
     float B = 0.0f, omB = 1.0f;
@@ -1052,6 +1128,7 @@ adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ la
             const float den = (1.0f - e) + tw;
             B = e * tw / den;
             omB = 1.0f - B;
+            const float muOm = wp * Sep + wm * Sem + ap * Sxp + am * Sxm;
             float gB = -muOm, dNt = 0.0f;
 #pragma unroll
             for (int i = 1; i < 19; i++) gB += mu[i] * (g[kOpp[i]] - g[i]);
@@ -1065,49 +1142,62 @@ adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ la
         }
     }
 
-    // rate chain through tau(Q, rho)
+    // rate chain through tau(Q, rho):
+    //   lambda_j += A (c.Pi.c - 2 c.Pi.J/rho) + A (J.Pi.J/rho^2 - tr(Pi)/3) + dJ/drho,
+    // A = dJ/dQ / Q; the c.Pi.J term folds into the momentum part below
     const float dwm = q.srt ? -wp * wp : wm * wm * lambda / ((tau - 0.5f) * (tau - 0.5f));
     const float gTau = gwp * (-wp * wp) + gwm * dwm;
-    float A = 0.0f, lesC = 0.0f, PJx = 0.0f, PJy = 0.0f, PJz = 0.0f;
+    float A = 0.0f, lesC = 0.0f;
+    float Lx = 0.0f, Ly = 0.0f, Lz = 0.0f;
     if (Q > 0.0f) {
         const float gQ = gTau * Kc / (4.0f * rho * sq);
         const float gR = -gTau * Kc * Q * ir / (4.0f * rho * sq);
         A = gQ / Q;
-        PJx = pxx * Jx + pxy * Jy + pxz * Jz;
-        PJy = pxy * Jx + pyy * Jy + pyz * Jz;
-        PJz = pxz * Jx + pyz * Jy + pzz * Jz;
+        const float PJx = pxx * Jx + pxy * Jy + pxz * Jz;
+        const float PJy = pxy * Jx + pyy * Jy + pyz * Jz;
+        const float PJz = pxz * Jx + pyz * Jy + pzz * Jz;
         const float JPJ = Jx * PJx + Jy * PJy + Jz * PJz;
         lesC = A * (JPJ * ir2 - (pxx + pyy + pzz) * (1.0f / 3.0f)) + gR;
+        const float A2 = 2.0f * A * ir;
+        Lx = -A2 * PJx; Ly = -A2 * PJy; Lz = -A2 * PJz;
     }
-    const float A2 = 2.0f * A * ir;
-    const float base = cp + lesC;
+    // lambda_j = mu_j + (1 - B)(-w+ mu+_j - w- mu-_j + dRho + c_j.dP) + B (mu_opp - mu_j)
+    //          + cp + w_f 2 B c_jx + LES terms
+    const float wpB = omB * wp, wmB = omB * wm;
+    const float base = omB * gRho + cp + lesC;
     const float dObjX = nt ? q.wf * 2.0f * B : 0.0f;      // d ntF_x / d g_j = 2 B c_jx
-    lam[n] = mu[0] + omB * (-wp * mu[0] + gRho) + base;
-#define STRH_LESLAM(I)                                                                       \
+    gPx = omB * gPx + Lx + dObjX;
+    gPy = omB * gPy + Ly;
+    gPz = omB * gPz + Lz;
+    const float Dxx = A * pxx, Dyy = A * pyy, Dzz = A * pzz;
+    const float Dxy = 2.0f * A * pxy, Dxz = 2.0f * A * pxz, Dyz = 2.0f * A * pyz;
+    st_off(lam, n4, mu[0] - wpB * mu[0] + base);
+#define STRH_LESPAIR(I)                                                                      \
     {                                                                                        \
         constexpr int cx = kCx[I], cy = kCy[I], cz = kCz[I];                                 \
-        const float mp = 0.5f * (mu[I] + mu[kOpp[I]]);                                       \
-        const float mm = 0.5f * (mu[I] - mu[kOpp[I]]);                                       \
-        float om = -wp * mp - wm * mm + gRho;                                                \
-        if (cx) om += (float)cx * gPx;                                                       \
-        if (cy) om += (float)cy * gPy;                                                       \
-        if (cz) om += (float)cz * gPz;                                                       \
-        float cPc = 0.0f, cPJ = 0.0f;                                                        \
-        if (cx) { cPc += pxx; cPJ += (float)cx * PJx; }                                      \
-        if (cy) { cPc += pyy; cPJ += (float)cy * PJy; }                                      \
-        if (cz) { cPc += pzz; cPJ += (float)cz * PJz; }                                      \
-        if (cx * cy) cPc += (float)(2 * cx * cy) * pxy;                                      \
-        if (cx * cz) cPc += (float)(2 * cx * cz) * pxz;                                      \
-        if (cy * cz) cPc += (float)(2 * cy * cz) * pyz;                                      \
-        float v = mu[I] + omB * om + base + A * cPc - A2 * cPJ;                              \
-        if (NT) v += B * (mu[kOpp[I]] - mu[I]);                                              \
-        if (cx) v += (float)cx * dObjX;                                                      \
-        lam[I * S + n] = v;                                                                  \
+        const float mp = 0.5f * (mu[I] + mu[I + 1]);                                         \
+        const float mm = 0.5f * (mu[I] - mu[I + 1]);                                         \
+        float cPc = 0.0f, cgp = 0.0f;                                                        \
+        if (cx) { cPc += Dxx; cgp += (float)cx * gPx; }                                      \
+        if (cy) { cPc += Dyy; cgp += (float)cy * gPy; }                                      \
+        if (cz) { cPc += Dzz; cgp += (float)cz * gPz; }                                      \
+        if (cx * cy) cPc += (float)(cx * cy) * Dxy;                                          \
+        if (cx * cz) cPc += (float)(cx * cz) * Dxz;                                          \
+        if (cy * cz) cPc += (float)(cy * cz) * Dyz;                                          \
+        const float common = base - wpB * mp + cPc;                                          \
+        const float odd = cgp - wmB * mm;                                                    \
+        float v0 = mu[I] + common + odd, v1 = mu[I + 1] + common - odd;                      \
+        if (NT) {                                                                            \
+            const float bb = B * (mu[I + 1] - mu[I]);                                        \
+            v0 += bb;                                                                        \
+            v1 -= bb;                                                                        \
+        }                                                                                    \
+        st_off(lam + (size_t)I * S, n4, v0);                                                  \
+        st_off(lam + (size_t)(I + 1) * S, n4, v1);                                     \
     }
-    STRH_LESLAM(1) STRH_LESLAM(2) STRH_LESLAM(3) STRH_LESLAM(4) STRH_LESLAM(5) STRH_LESLAM(6)
-    STRH_LESLAM(7) STRH_LESLAM(8) STRH_LESLAM(9) STRH_LESLAM(10) STRH_LESLAM(11) STRH_LESLAM(12)
-    STRH_LESLAM(13) STRH_LESLAM(14) STRH_LESLAM(15) STRH_LESLAM(16) STRH_LESLAM(17) STRH_LESLAM(18)
-#undef STRH_LESLAM
+    STRH_LESPAIR(1) STRH_LESPAIR(3) STRH_LESPAIR(5) STRH_LESPAIR(7) STRH_LESPAIR(9)
+    STRH_LESPAIR(11) STRH_LESPAIR(13) STRH_LESPAIR(15) STRH_LESPAIR(17)
+#undef STRH_LESPAIR
     if (gu != 0.0) gUin[n] += gu;
 }
 

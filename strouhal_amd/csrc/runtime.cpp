@@ -1331,11 +1331,24 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         size_t freeB = 0, totalB = 0;
         HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
         const double fixed = (double)N * (3 * 8 + 4) + 2.0 * 19 * S * 4 + 2.0 * ddf;
-        const double budget = 0.5 * (double)freeB - fixed;
-        // recording (moments and partial-cell populations, or all populations
-        // under LES) + its share of checkpoints
-        const double perStep = (les ? (double)N * 76 : (double)N * 16 + (double)M * 76) + (double)ddf / 64.0;
-        K = budget > perStep ? (int64_t)(budget / perStep) : 1;
+        const double budget = 0.8 * (double)freeB - fixed;
+        // recording per step (moments and partial-cell populations, or all
+        // populations under LES); K steps of it plus ceil(T/K) - 1 checkpoints.
+        // When no K fits, the one that needs least (the allocation then
+        // reports the shortfall).
+        const double perStep = les ? (double)N * 76 : (double)N * 16 + (double)M * 76;
+        auto need = [&](int64_t k) { return (double)k * perStep + (double)((T + k - 1) / k - 1) * (double)ddf; };
+        int64_t best = 1;
+        K = 0;
+        for (int64_t k = std::min<int64_t>(T, budget > perStep ? (int64_t)(budget / perStep) : 1); k >= 1; k--) {
+            if (need(k) <= budget) { K = k; break; }
+            if (need(k) < need(best)) best = k;
+        }
+        if (K == 0) {
+            for (int64_t k = 1; k <= T && (double)k * perStep <= need(best); k++)
+                if (need(k) < need(best)) best = k;
+            K = best;
+        }
     }
     if (K > T) K = T;
     const int64_t C = (T + K - 1) / K;

@@ -13,7 +13,17 @@ Sim.swift:359-372) and reports the mean C_D over the samples, the Schiller-
 Naumann correlation, MLUPS and the device memory the lattice holds; FP16 and
 BF16 are compared with FP32.
 
+With `--adjoint-steps T` each precision instead develops the wake for
+`spinup` steps and then differentiates the time-averaged partial-cell drag
+(the x momentum exchange of the Noble-Torczynski cells, Kernels.metal:201-227)
+over the next T steps with respect to every solid fraction. It times T plain
+steps from the same state, the adjoint's recording forward, segment replays
+and reverse steps, checks that the adjoint leaves the lattice where the
+plain run does, and reports the gradient over the sphere's surface cells
+(0 < eps < 1) and the device memory the adjoint holds.
+
     python3 strouhal_amd/sphere_les.py --spinup 16000 --samples 40 --stride 100
+    python3 strouhal_amd/sphere_les.py --spinup 16000 --adjoint-steps 2000 --precisions fp16s,bf16
 """
 from __future__ import annotations
 
@@ -61,6 +71,64 @@ def run(gpu, precision: str, size, D: int, spinup: int, samples: int, stride: in
     return out
 
 
+def adjoint_run(gpu, precision: str, size, D: int, spinup: int, T: int, K: int = -1,
+                u: float = 0.05, re: float = 300.0, cs: float = 0.1):
+    from strouhal_amd._lib import lib
+    import ctypes
+    nx, ny, nz = size
+    case = sphere_case(gpu, precision, D=D, size=size, u=u, re=re, c_smago=cs * cs,
+                       center=(96.0, ny / 2.0, nz / 2.0), lam=None)
+    sim = case.sim
+    sz = ctypes.c_size_t()
+    lib().strh_ddf_bytes(sim._h, ctypes.byref(sz))
+    ddf = sz.value
+    sim.run(spinup)
+    sim.snapshot()
+    sim.run(2)                                     # warm the kernels at this state
+    sim.restore()
+    g0 = sim.gpu_seconds
+    sim.run(T)
+    fwd_s = sim.gpu_seconds - g0
+    digest_fwd = sim.state_digest
+    sim.restore()
+    obj = dict(inlet_x=-1, outlet_x=-1, spinup=0, avg=T, w_p=0.0, w_d=0.0, w_f=1.0)
+    t0 = time.perf_counter()
+    J, _, gu, info = sim.vjp(**obj, checkpoint_every=K, want_eps=True)
+    wall = time.perf_counter() - t0
+    same = sim.state_digest == digest_fwd
+    eps = np.asarray(sim_eps(case)).reshape(-1)
+    ge = info.pop("grad_eps").reshape(-1)
+    surf = (eps > 0.0) & (eps < 1.0)
+    core = eps >= 1.0
+    adj_s = info["forward_seconds"] + info["recompute_seconds"] + info["backward_seconds"]
+    n = sim.cells
+    # C_D of the averaged partial-cell drag, and its gradient in those units
+    cd_scale = 2.0 / (case.u * case.u * np.pi * D * D / 4.0)
+    out = {"precision": precision, "spinup": spinup, "adjoint_steps": T, "cells": n,
+           "forward_ms_per_step": fwd_s / T * 1e3, "forward_mlups": n * T / fwd_s / 1e6,
+           "vjp": info, "vjp_wall_s": wall,
+           "adjoint_over_forward": adj_s / fwd_s,
+           "recording_forward_ms_per_step": info["forward_seconds"] / T * 1e3,
+           "backward_ms_per_step": info["backward_seconds"] / T * 1e3,
+           "ddf_bytes": ddf, "adjoint_workspace_bytes": info["workspace_bytes"],
+           "device_bytes_during_adjoint": ddf * 2 + info["workspace_bytes"],
+           "J_ntF_x": J, "cd_from_J": J * cd_scale, "grad_uin": gu,
+           "state_bitwise_equal": bool(same),
+           "surface_cells": int(surf.sum()), "core_cells": int(core.sum()),
+           "grad_surface_sum": float(ge[surf].sum()), "grad_surface_max_abs": float(np.max(np.abs(ge[surf]))),
+           "grad_core_max_abs": float(np.max(np.abs(ge[core]))) if core.any() else 0.0,
+           "grad_nonzero_outside_body": int(np.count_nonzero(ge[eps == 0.0])),
+           "finite": bool(np.all(np.isfinite(ge)) and np.isfinite(J))}
+    sim.close()
+    return out
+
+
+def sim_eps(case):
+    nx, ny, nz = case.sim.nx, case.sim.ny, case.sim.nz
+    from strouhal_amd.cases import sphere_solid_fractions
+    return sphere_solid_fractions(nx, ny, nz, 96.0, ny / 2.0, nz / 2.0, case.D / 2.0)
+
+
 def main():
     import argparse
     from strouhal_amd import default_gpu
@@ -73,9 +141,18 @@ def main():
     ap.add_argument("--samples", type=int, default=40)
     ap.add_argument("--stride", type=int, default=100)
     ap.add_argument("--precisions", default="fp32,fp16s,bf16")
+    ap.add_argument("--adjoint-steps", type=int, default=0,
+                    help="differentiate the partial-cell drag over this many steps after the spin-up")
+    ap.add_argument("--checkpoint-every", type=int, default=-1,
+                    help="adjoint segment length; 0 = ceil(sqrt(T)), -1 = the longest that fits")
     args = ap.parse_args()
     gpu = default_gpu()
     size = (args.nx, args.ny, args.nz)
+    if args.adjoint_steps > 0:
+        for p in args.precisions.split(","):
+            r = adjoint_run(gpu, p, size, args.D, args.spinup, args.adjoint_steps, args.checkpoint_every)
+            print(json.dumps(r), flush=True)
+        return
     res = {}
     for p in args.precisions.split(","):
         res[p] = run(gpu, p, size, args.D, args.spinup, args.samples, args.stride)
