@@ -111,13 +111,6 @@ template <class T> __device__ __forceinline__ void st_off(T* base, uint32_t off,
     *(T*)((char*)base + off) = v;
 }
 
-// Workgroup b of G runs on XCD b % 8. Give each XCD a contiguous range of
-// logical blocks so neighbouring cell ranges share an L2.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
-    const uint32_t q = G >> 3, r = G & 7u, xcd = b & 7u, k = b >> 3;
-    return xcd < r ? xcd * (q + 1u) + k : r * (q + 1u) + (xcd - r) * q + k;
-}
-
 template <int C> __device__ __forceinline__ int pick(int m, int p) {
     return C < 0 ? m : (C > 0 ? p : 0);
 }
@@ -689,7 +682,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
             float4* __restrict__ force, const float* __restrict__ epsBuf, StepParams p,
             const float* __restrict__ alphaBuf) {
-    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * kStepBS + threadIdx.x;
+    const uint32_t n = blockIdx.x * kStepBS + threadIdx.x;
     if (n >= p.N) return;
     step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK>(f, flags, link, lidm, force, epsBuf, p, n,
                                                         alphaBuf);
@@ -704,7 +697,7 @@ step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ f
                    const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
                    const float* __restrict__ epsBuf, StepParams p, const float* __restrict__ alphaBuf,
                    AdjHooks hk) {
-    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * kStepBS + threadIdx.x;
+    const uint32_t n = blockIdx.x * kStepBS + threadIdx.x;
     if (n >= p.N) return;
     step_cell<St, PARITY, WALLS, NT, LES, true, BRINK, true>(f, flags, link, lidm, nullptr, epsBuf, p, n,
                                                              alphaBuf, &hk);
@@ -845,7 +838,7 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
                     double* __restrict__ gAlpha, double* __restrict__ gUin,
                     const float* __restrict__ epsBuf, const int32_t* __restrict__ ntIdx,
                     const float* __restrict__ recNt, double* __restrict__ gEps, AdjStep q) {
-    const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= q.N) return;
     uint32_t sM = 0u;   // lattices without walls hold fluid cells only
     if (WALLS) {
@@ -1008,7 +1001,7 @@ adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ la
                    const uint32_t* __restrict__ link, double* __restrict__ gUin,
                    const float* __restrict__ epsBuf, const int32_t* __restrict__ ntIdx,
                    double* __restrict__ gEps, AdjStep q) {
-    const uint32_t n = xcd_remap(blockIdx.x, q.blocks) * 256u + threadIdx.x;
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= q.N) return;
     uint32_t sM = 0u;   // lattices without walls hold fluid cells only
     if (WALLS) {
@@ -1218,7 +1211,7 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
     using T = typename St::T;
     __shared__ T row[10][BS];
     const uint32_t t = threadIdx.x;
-    const uint32_t n = xcd_remap(blockIdx.x, p.blocks) * BS + t;   // N % BS == 0
+    const uint32_t n = blockIdx.x * BS + t;   // N % BS == 0
     T* fs[19];
 #pragma unroll
     for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * p.lay.W;
@@ -1499,7 +1492,7 @@ template <class Conv, int BS>
 __global__ void __launch_bounds__(BS)
 step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
            const int64_t* __restrict__ base, int k) {
-    const uint32_t b = xcd_remap(blockIdx.x, totalBlocks);
+    const uint32_t b = blockIdx.x;
     const int l = batch_find(lats, L, b);
     const BatchLat& B = lats[l];
     const int64_t t = B.step0 + *base + k;

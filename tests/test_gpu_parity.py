@@ -198,6 +198,7 @@ def test_row_staged_odd_pass(gpu, precision):
     cases = [((64, 8, 4), dict(omega=1.7)),
              ((128, 6, 3), dict(omega=1.2, omega_minus=1.6, force=(2e-5, 1e-5, -1e-5))),
              ((256, 2, 2), dict(omega=1.9, omega_minus=1.1, c_smago=0.04)),
+             ((64, 4, 1), dict(omega=1.3)),
              ((512, 2, 2), dict(omega=1.6))]
     for (nx, ny, nz), kw in cases:
         om = kw.pop("omega")
@@ -205,6 +206,25 @@ def test_row_staged_odd_pass(gpu, precision):
         g, r = make_pair(gpu, precision, nx, ny, nz, om, omm, **kw)
         _seed_both(g, r, precision, seed=nx + ny)
         _run_compare(g, r, [1, 2, 3], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_row_staged_odd_pass_partial_cells(gpu, precision):
+    """The row-staged odd pass with Noble-Torczynski cells, LES and a body
+    force on a periodic 128 x 5 x 3 lattice: DDFs and the per-cell
+    momentum-exchange force of a force probe, bit for bit."""
+    rng = np.random.default_rng(17)
+    g, r = make_pair(gpu, precision, 128, 5, 3, 1.5, 1.3, force=(1e-5, 0.0, -2e-5), c_smago=0.02,
+                     wants_forces=True)
+    eps = np.where(rng.random(g.cells) < 0.3, rng.random(g.cells), 0.0).astype(np.float32)
+    g.set_solid_fractions(eps)
+    r.set_solid_fractions(eps)
+    _seed_both(g, r, precision, seed=59)
+    _run_compare(g, r, [1, 3], precision)          # the probe starts from an even step count
+    g.probe_force((0, 127), (0, 4))
+    r.probe_force((0, 127), (0, 4))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+    assert_bit_exact(g.download_force(), r.force_field().astype(np.float32), "per-cell force")
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
