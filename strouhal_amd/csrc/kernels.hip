@@ -1488,7 +1488,11 @@ __device__ __forceinline__ bool is_force_step(const BatchLat& B, int64_t t) {
     return d % B.fPeriod == 0 && d / B.fPeriod < B.fCount;
 }
 
-template <class Conv, int BS>
+// PAR = 0 / 1: the host guarantees that every active lattice is at that
+// parity in this launch (equal start parities, even graph lengths), so only
+// that pass is compiled in; PAR = -1 branches per lattice. LES = false when no
+// lattice of the batch has a Smagorinsky constant.
+template <class Conv, int BS, int PAR, bool LES>
 __global__ void __launch_bounds__(BS)
 step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
            const int64_t* __restrict__ base, int k) {
@@ -1500,7 +1504,7 @@ step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
     const uint32_t n = (b - B.block0) * BS + threadIdx.x;
     if (n >= B.p.N) return;
     StepParams p = B.p;
-    p.parity = (int)(t & 1);
+    p.parity = PAR >= 0 ? PAR : (int)(t & 1);
     const float r = t < B.rampSteps ? B.ramp[t] : 1.0f;      // Sim.swift:258-266
     p.uin = B.uinTarget * r;
     p.ulidX = B.lid[0] * r;
@@ -1509,14 +1513,14 @@ step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
     p.writeForce = is_force_step(B, t) ? 1 : 0;
     using T = typename Conv::T;
     T* f = (T*)B.f;
-    if (p.parity == 0) {
+    if (PAR != 1 && p.parity == 0) {
         using St = Store<Conv, kPolAligned>;
-        if (B.hasForce) step_cell<St, 0, true, true, true, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
-        else step_cell<St, 0, true, true, true, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
-    } else {
+        if (B.hasForce) step_cell<St, 0, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        else step_cell<St, 0, true, true, LES, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+    } else if (PAR != 0) {
         using St = Store<Conv, kPolShifted>;
-        if (B.hasForce) step_cell<St, 1, true, true, true, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
-        else step_cell<St, 1, true, true, true, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        if (B.hasForce) step_cell<St, 1, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        else step_cell<St, 1, true, true, LES, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
     }
 }
 
@@ -1796,24 +1800,37 @@ hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <class Conv, int BS, int PAR>
+static void launch_batch_par(bool les, const BatchLat* lats, int L, uint32_t totalBlocks,
+                             const int64_t* base, int k, hipStream_t s) {
+    if (les) hipLaunchKernelGGL((step_batch<Conv, BS, PAR, true>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k);
+    else hipLaunchKernelGGL((step_batch<Conv, BS, PAR, false>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k);
+}
+template <class Conv, int BS>
+static void launch_batch_conv(int parity, bool les, const BatchLat* lats, int L, uint32_t totalBlocks,
+                              const int64_t* base, int k, hipStream_t s) {
+    if (parity == 0) launch_batch_par<Conv, BS, 0>(les, lats, L, totalBlocks, base, k, s);
+    else if (parity == 1) launch_batch_par<Conv, BS, 1>(les, lats, L, totalBlocks, base, k, s);
+    else launch_batch_par<Conv, BS, -1>(les, lats, L, totalBlocks, base, k, s);
+}
 template <int BS>
-static hipError_t launch_batch_bs(int precision, const BatchLat* lats, int L, uint32_t totalBlocks,
-                                  const int64_t* base, int k, hipStream_t s) {
+static hipError_t launch_batch_bs(int precision, int parity, bool les, const BatchLat* lats, int L,
+                                  uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
     switch (precision) {
-        case kFP32: hipLaunchKernelGGL((step_batch<ConvF32, BS>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k); break;
-        case kFP16S: hipLaunchKernelGGL((step_batch<ConvF16, BS>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k); break;
-        case kBF16: hipLaunchKernelGGL((step_batch<ConvBF16, BS>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k); break;
+        case kFP32: launch_batch_conv<ConvF32, BS>(parity, les, lats, L, totalBlocks, base, k, s); break;
+        case kFP16S: launch_batch_conv<ConvF16, BS>(parity, les, lats, L, totalBlocks, base, k, s); break;
+        case kBF16: launch_batch_conv<ConvBF16, BS>(parity, les, lats, L, totalBlocks, base, k, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_batch_step(int precision, int bs, const BatchLat* lats, int L,
+hipError_t launch_batch_step(int precision, int bs, int parity, bool les, const BatchLat* lats, int L,
                              uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
     switch (bs) {
-        case 128: return launch_batch_bs<128>(precision, lats, L, totalBlocks, base, k, s);
-        case 256: return launch_batch_bs<256>(precision, lats, L, totalBlocks, base, k, s);
-        case 512: return launch_batch_bs<512>(precision, lats, L, totalBlocks, base, k, s);
+        case 128: return launch_batch_bs<128>(precision, parity, les, lats, L, totalBlocks, base, k, s);
+        case 256: return launch_batch_bs<256>(precision, parity, les, lats, L, totalBlocks, base, k, s);
+        case 512: return launch_batch_bs<512>(precision, parity, les, lats, L, totalBlocks, base, k, s);
     }
     return hipErrorInvalidValue;
 }

@@ -870,12 +870,17 @@ struct Segment {
     int L;                   // active lattices
     uint32_t blocks;         // workgroups of the active lattices
     BatchLat* dev;           // their descriptors (device)
+    int parity0;             // parity of every active lattice at k0, or -1 if they differ
+    bool les;                // some active lattice has a Smagorinsky constant
 };
 
-int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count) {
+// Launches steps k = 0 .. count-1 from base; parity0 is the common parity at
+// k = 0 (or -1), so step k runs the pass (parity0 + k) & 1 only.
+int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int parity0) {
     hipStream_t s = B->ctx->stream;
     for (int k = 0; k < count; k++) {
-        HIP_TRY(launch_batch_step(B->precision, bs, sg.dev, sg.L, sg.blocks, B->base, k, s),
+        const int par = parity0 < 0 ? -1 : (parity0 + k) & 1;
+        HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
         HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, s), "batch probe launch");
     }
@@ -885,8 +890,10 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count) {
 int batch_capture(strh_batch* B, const Segment& sg, hipGraphExec_t* out) {
     hipStream_t s = B->ctx->stream;
     hipGraph_t g = nullptr;
+    // replays advance base by graphSteps: fixed parities need an even length
+    const int par0 = (B->graphSteps % 2 == 0) ? sg.parity0 : -1;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-    int rc = batch_launch_steps(B, sg, B->bs, B->graphSteps);
+    int rc = batch_launch_steps(B, sg, B->bs, B->graphSteps, par0);
     hipError_t e2 = launch_advance_base(B->base, B->graphSteps, s);
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
@@ -918,9 +925,14 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
     for (int64_t e : ends) {
         Segment sg;
         sg.k0 = k0; sg.k1 = e; sg.L = 0; sg.blocks = 0; sg.dev = nullptr;
+        sg.parity0 = -2;
+        sg.les = false;
         for (int i = 0; i < n; i++) {
             if (steps[i] <= k0) continue;
             BatchLat d = B->host[i];
+            const int par = (int)((d.step0 + k0) & 1);
+            sg.parity0 = sg.parity0 == -2 ? par : (sg.parity0 == par ? par : -1);
+            sg.les = sg.les || B->lats[i]->cSmago > 0.0f;
             d.block0 = sg.blocks;
             d.nblocks = (B->lats[i]->N + (uint32_t)B->bs - 1) / (uint32_t)B->bs;
             if (!probes) d.fCount = 0;
@@ -964,7 +976,8 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             if (e != hipSuccess) return hip_fail(e, "graph replays");
         }
         if (done < K) {
-            int rc = batch_launch_steps(B, sg, B->bs, (int)(K - done));
+            const int par0 = sg.parity0 < 0 ? -1 : (int)((sg.parity0 + done) & 1);
+            int rc = batch_launch_steps(B, sg, B->bs, (int)(K - done), par0);
             if (rc) return rc;
         }
     }

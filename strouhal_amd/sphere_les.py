@@ -161,7 +161,7 @@ def main():
     if "fp32" in res:
         ref = res["fp32"]["cd_mean"]
         for p, r in res.items():
-            r["cd_rel_err_vs_fp32"] = abs(r["cd_mean"] - ref) / abs(ref)
+            r["cd_rel_err_vs_fp32"] = abs(r["cd_mean"] - ref) / abs(ref) if ref != 0.0 else None
     report = {"workload": f"sphere D={args.D} in {args.nx}x{args.ny}x{args.nz}, Re 300, U 0.05, "
                           f"SRT + Smagorinsky Cs 0.1, NT solid fractions, sponge; "
                           f"{args.spinup} steps + {args.samples} drag samples every {args.stride}",
