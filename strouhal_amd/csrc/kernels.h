@@ -105,9 +105,10 @@ hipError_t launch_adjoint_step(const AdjStep& q, bool walls, const float4* rec, 
 hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,
                               double* out, hipStream_t s);
 // parity: 0 / 1 when every active lattice is at that parity in this launch,
-// -1 to branch per lattice; les: some lattice has a Smagorinsky constant.
-hipError_t launch_batch_step(int precision, int bs, int parity, bool les, const BatchLat* lats, int L,
-                             uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
+// -1 to branch per lattice (then les and forced are taken as true); les: some
+// lattice has a Smagorinsky constant; forced: some lattice has a body force.
+hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, const BatchLat* lats,
+                             int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
 hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, hipStream_t s);
 hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
 hipError_t launch_set_base(int64_t* base, int64_t v, hipStream_t s);

@@ -620,6 +620,12 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     int nip = ni;
     asm("" : "+v"(nip));
     auto nbp = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB; };
+    // ... and the slot bases from a pinned copy of f, so the pull's 19 base
+    // pointers (38 SGPRs) are not held across the collision either
+    T* fpin = f;
+    asm("" : "+s"(fpin));
+#pragma unroll
+    for (int i = 0; i < 19; i++) fs[i] = fpin + (size_t)i * p.lay.W;
     St::wr(fs[0], nbs, fh[0]);
     // Wall links store sign * f - corr into the cell's own opposite slot (the
     // halfway bounce-back, with the moving-wall term or the outflow's sign
@@ -1491,8 +1497,9 @@ __device__ __forceinline__ bool is_force_step(const BatchLat& B, int64_t t) {
 // PAR = 0 / 1: the host guarantees that every active lattice is at that
 // parity in this launch (equal start parities, even graph lengths), so only
 // that pass is compiled in; PAR = -1 branches per lattice. LES = false when no
-// lattice of the batch has a Smagorinsky constant.
-template <class Conv, int BS, int PAR, bool LES>
+// lattice of the batch has a Smagorinsky constant, FORCED = false when none has
+// a body force (otherwise each lattice takes its own forcing path).
+template <class Conv, int BS, int PAR, bool LES, bool FORCED>
 __global__ void __launch_bounds__(BS)
 step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
            const int64_t* __restrict__ base, int k) {
@@ -1515,11 +1522,11 @@ step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
     T* f = (T*)B.f;
     if (PAR != 1 && p.parity == 0) {
         using St = Store<Conv, kPolAligned>;
-        if (B.hasForce) step_cell<St, 0, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        if (FORCED && B.hasForce) step_cell<St, 0, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
         else step_cell<St, 0, true, true, LES, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
     } else if (PAR != 0) {
         using St = Store<Conv, kPolShifted>;
-        if (B.hasForce) step_cell<St, 1, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        if (FORCED && B.hasForce) step_cell<St, 1, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
         else step_cell<St, 1, true, true, LES, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
     }
 }
@@ -1801,36 +1808,40 @@ hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
 }
 
 template <class Conv, int BS, int PAR>
-static void launch_batch_par(bool les, const BatchLat* lats, int L, uint32_t totalBlocks,
+static void launch_batch_par(bool les, bool forced, const BatchLat* lats, int L, uint32_t totalBlocks,
                              const int64_t* base, int k, hipStream_t s) {
-    if (les) hipLaunchKernelGGL((step_batch<Conv, BS, PAR, true>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k);
-    else hipLaunchKernelGGL((step_batch<Conv, BS, PAR, false>), dim3(totalBlocks), dim3(BS), 0, s, lats, L, totalBlocks, base, k);
+#define STRH_BATCH(LESV, FV) hipLaunchKernelGGL((step_batch<Conv, BS, PAR, LESV, FV>), dim3(totalBlocks), \
+                                                dim3(BS), 0, s, lats, L, totalBlocks, base, k)
+    if (les) { if (forced) STRH_BATCH(true, true); else STRH_BATCH(true, false); }
+    else { if (forced) STRH_BATCH(false, true); else STRH_BATCH(false, false); }
+#undef STRH_BATCH
 }
 template <class Conv, int BS>
-static void launch_batch_conv(int parity, bool les, const BatchLat* lats, int L, uint32_t totalBlocks,
-                              const int64_t* base, int k, hipStream_t s) {
-    if (parity == 0) launch_batch_par<Conv, BS, 0>(les, lats, L, totalBlocks, base, k, s);
-    else if (parity == 1) launch_batch_par<Conv, BS, 1>(les, lats, L, totalBlocks, base, k, s);
-    else launch_batch_par<Conv, BS, -1>(les, lats, L, totalBlocks, base, k, s);
+static void launch_batch_conv(int parity, bool les, bool forced, const BatchLat* lats, int L,
+                              uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
+    if (parity == 0) launch_batch_par<Conv, BS, 0>(les, forced, lats, L, totalBlocks, base, k, s);
+    else if (parity == 1) launch_batch_par<Conv, BS, 1>(les, forced, lats, L, totalBlocks, base, k, s);
+    else hipLaunchKernelGGL((step_batch<Conv, BS, -1, true, true>), dim3(totalBlocks), dim3(BS), 0, s,
+                            lats, L, totalBlocks, base, k);
 }
 template <int BS>
-static hipError_t launch_batch_bs(int precision, int parity, bool les, const BatchLat* lats, int L,
-                                  uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
+static hipError_t launch_batch_bs(int precision, int parity, bool les, bool forced, const BatchLat* lats,
+                                  int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
     switch (precision) {
-        case kFP32: launch_batch_conv<ConvF32, BS>(parity, les, lats, L, totalBlocks, base, k, s); break;
-        case kFP16S: launch_batch_conv<ConvF16, BS>(parity, les, lats, L, totalBlocks, base, k, s); break;
-        case kBF16: launch_batch_conv<ConvBF16, BS>(parity, les, lats, L, totalBlocks, base, k, s); break;
+        case kFP32: launch_batch_conv<ConvF32, BS>(parity, les, forced, lats, L, totalBlocks, base, k, s); break;
+        case kFP16S: launch_batch_conv<ConvF16, BS>(parity, les, forced, lats, L, totalBlocks, base, k, s); break;
+        case kBF16: launch_batch_conv<ConvBF16, BS>(parity, les, forced, lats, L, totalBlocks, base, k, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_batch_step(int precision, int bs, int parity, bool les, const BatchLat* lats, int L,
-                             uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
+hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, const BatchLat* lats,
+                             int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
     switch (bs) {
-        case 128: return launch_batch_bs<128>(precision, parity, les, lats, L, totalBlocks, base, k, s);
-        case 256: return launch_batch_bs<256>(precision, parity, les, lats, L, totalBlocks, base, k, s);
-        case 512: return launch_batch_bs<512>(precision, parity, les, lats, L, totalBlocks, base, k, s);
+        case 128: return launch_batch_bs<128>(precision, parity, les, forced, lats, L, totalBlocks, base, k, s);
+        case 256: return launch_batch_bs<256>(precision, parity, les, forced, lats, L, totalBlocks, base, k, s);
+        case 512: return launch_batch_bs<512>(precision, parity, les, forced, lats, L, totalBlocks, base, k, s);
     }
     return hipErrorInvalidValue;
 }

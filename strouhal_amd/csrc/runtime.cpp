@@ -872,6 +872,7 @@ struct Segment {
     BatchLat* dev;           // their descriptors (device)
     int parity0;             // parity of every active lattice at k0, or -1 if they differ
     bool les;                // some active lattice has a Smagorinsky constant
+    bool forced;             // some active lattice has a body force
 };
 
 // Launches steps k = 0 .. count-1 from base; parity0 is the common parity at
@@ -880,7 +881,7 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
     hipStream_t s = B->ctx->stream;
     for (int k = 0; k < count; k++) {
         const int par = parity0 < 0 ? -1 : (parity0 + k) & 1;
-        HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.dev, sg.L, sg.blocks, B->base, k, s),
+        HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
         HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, s), "batch probe launch");
     }
@@ -927,12 +928,14 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         sg.k0 = k0; sg.k1 = e; sg.L = 0; sg.blocks = 0; sg.dev = nullptr;
         sg.parity0 = -2;
         sg.les = false;
+        sg.forced = false;
         for (int i = 0; i < n; i++) {
             if (steps[i] <= k0) continue;
             BatchLat d = B->host[i];
             const int par = (int)((d.step0 + k0) & 1);
             sg.parity0 = sg.parity0 == -2 ? par : (sg.parity0 == par ? par : -1);
             sg.les = sg.les || B->lats[i]->cSmago > 0.0f;
+            sg.forced = sg.forced || d.hasForce != 0;
             d.block0 = sg.blocks;
             d.nblocks = (B->lats[i]->N + (uint32_t)B->bs - 1) / (uint32_t)B->bs;
             if (!probes) d.fCount = 0;
