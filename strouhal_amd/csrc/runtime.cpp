@@ -883,7 +883,8 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
         const int par = parity0 < 0 ? -1 : (parity0 + k) & 1;
         HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
-        HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, s), "batch probe launch");
+        // probe steps are odd (strh_batch_set_probe): an even pass needs no probe
+        if (par != 0) HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, s), "batch probe launch");
     }
     return STRH_OK;
 }

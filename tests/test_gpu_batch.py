@@ -56,12 +56,15 @@ def test_mixed_batch_matches_solo_runs(gpu, bs, graph):
     bt = Batch(batched)
     bt.set_block_size(bs)
     bt.set_graph_steps(graph)
-    bt.run(steps)
-    bt.run([3, 0, 8])
+    bt.run(steps)            # every lattice starts even: one pass per launch
+    bt.run([3, 0, 8])        # the two active lattices start odd
+    bt.run([4, 0, 19])       # 40 (even) with 29 (odd): launches branch per lattice
     for s, k in zip(solo, steps):
         s.run(k)
     solo[0].run(3)
     solo[2].run(8)
+    solo[0].run(4)
+    solo[2].run(19)
     for x, y in zip(batched, solo):
         assert x.steps_done == y.steps_done
         assert x.state_digest == y.state_digest
