@@ -249,8 +249,8 @@ STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, d
  * recording and checkpoints fit 80% of the free device memory (the K needing
  * least memory when none fits). The adjoint state is FP32;
  * every accumulation is FP64 in a fixed order, so the result does not depend
- * on K. Device memory: (ceil(T/K) - 1) DDF images + K * N * 16 bytes (76
- * under LES) +
+ * on K. Device memory: (ceil(T/K) - 1) DDF images + K * N * 16 bytes (19 * N
+ * stored values under LES) +
  * 2 * 19 * N floats + 3 * N doubles + one DDF image. info may be NULL. */
 STRH_API int strh_vjp(strh_lattice* lat, const strh_objective* obj, int64_t checkpoint_every,
                       double* J, double* grad_alpha, double* grad_eps, double* grad_uin,

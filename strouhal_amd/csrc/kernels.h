@@ -69,7 +69,8 @@ struct AdjHooks {
     float* recNt;            // [19][M] pre-collision populations of those cells, or null
     uint32_t M;
     double wf;               // w_f / T (partial-cell drag)
-    float* rec19;            // [19][N] pre-collision populations of every cell (LES), or null
+    void* rec19;             // [19][N] pre-collision populations of every cell (LES), in the
+                             // storage format, before the moving-wall corrections; or null
 };
 
 // One reverse step: lambda^t from lambda^{t+1} and the recorded moments of g^t.
@@ -87,6 +88,7 @@ struct AdjStep {
     float spongeX0, spongeInvW, spongeTau;   // Params' sponge (x0 = 1e9: none)
     float cSmago;                // Smagorinsky Cs^2 (LES lattices)
     int32_t srt;                 // omega- == omega+ (the reference's SRT test)
+    int32_t precision;           // storage of the recorded populations (LES lattices)
 };
 
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);
@@ -95,13 +97,15 @@ hipError_t launch_step(const StepLaunch& L, hipStream_t s);
 // Smagorinsky LES (L.les), which exclude porosity.
 hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s);
 // Partial-cell lattices pass eps, ntIdx, recNt (this step's [19][M]) and gEps ([M]).
-// LES lattices pass rec19 (this step's [19][N] populations of every cell)
+// LES lattices pass rec19 (this step's [19][N] populations of every cell, storage
+// format, before the moving-wall corrections), lidm and the step's parameters sp
 // instead of rec / recNt.
 hipError_t launch_adjoint_step(const AdjStep& q, bool walls, const float4* rec, const float* lamNext,
                                float* lam, const uint8_t* flags, const uint32_t* link,
                                const float* alpha, double* gAlpha, double* gUin,
                                const float* eps, const int32_t* ntIdx, const float* recNt,
-                               double* gEps, const float* rec19, hipStream_t s);
+                               double* gEps, const void* rec19, const uint32_t* lidm,
+                               const StepParams& sp, hipStream_t s);
 hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uint32_t max_partials,
                               double* out, hipStream_t s);
 // parity: 0 / 1 when every active lattice is at that parity in this launch,

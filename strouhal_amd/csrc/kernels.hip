@@ -420,11 +420,6 @@ template <bool LES>
 __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[19], uint32_t n,
                                           int x, float alpha, float eps, const StepParams& p) {
     const float fx = p.fx;
-    if (hk.rec19) {
-#pragma unroll
-        for (int i = 0; i < 19; i++)
-            __builtin_nontemporal_store(fh[i], (float*)((char*)(hk.rec19 + (size_t)i * p.N) + n * 4u));
-    }
     if (hk.recNt && eps > 0.0f) {
         const int32_t k = hk.ntIdx[n];
 #pragma unroll
@@ -463,6 +458,19 @@ __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[
     }
 }
 
+// LES recording: the 19 pre-collision populations of cell n in the storage
+// format (exact: each is a stored value before the odd pass's moving-wall
+// corrections, which the reverse step re-applies).
+template <class St>
+__device__ __forceinline__ void record_populations(const AdjHooks& hk, const float (&fh)[19], uint32_t N,
+                                                   uint32_t n) {
+    using T = typename St::T;
+    T* r = (T*)hk.rec19;
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+        __builtin_nontemporal_store(St::st(fh[i]), (T*)((char*)(r + (size_t)i * N) + n * (uint32_t)sizeof(T)));
+}
+
 // One cell of one AA pass (the body shared by the single-lattice kernel and
 // the multi-lattice batch kernel).
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
@@ -495,6 +503,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 #pragma unroll
         for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nb);
         const float alpha = BRINK ? alphaBuf[n] : 0.0f;
+        if (HOOK && hk->rec19) record_populations<St>(*hk, fh, p.N, n);
         if (HOOK) run_hooks<LES>(*hk, fh, n, x, alpha, eps, p);
         collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);
         // A separate copy of the offset keeps the stores on the scalar-base
@@ -559,6 +568,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     STRH_PULL(7) STRH_PULL(8) STRH_PULL(9) STRH_PULL(10) STRH_PULL(11) STRH_PULL(12)
     STRH_PULL(13) STRH_PULL(14) STRH_PULL(15) STRH_PULL(16) STRH_PULL(17) STRH_PULL(18)
 #undef STRH_PULL
+    if (HOOK && hk->rec19) record_populations<St>(*hk, fh, p.N, n);
     if (WALLS && sM != 0u) {   // wall links: moving-wall corrections to apply, force terms
 #define STRH_WALLIN(I)                                                                \
         {                                                                             \
@@ -1000,13 +1010,18 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
 // where dJ/dw+- = sum_j mu_j dOmega_j/dw+- (scaled by 1 - B on partial cells,
 // which add the B(tw(w+)) term). Porosity and LES exclude each other, so F is
 // the body force and J = p + F/2.
-template <bool WALLS, bool NT>
+// The recording holds the populations in the storage format as loaded, before
+// the odd pass's moving-wall corrections; the kernel re-applies them from the
+// step's parameters sp with the forward's arithmetic.
+template <class Conv, bool WALLS, bool NT>
 __global__ void __launch_bounds__(256)
-adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ lamN,
+adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __restrict__ lamN,
                    float* __restrict__ lam, const uint8_t* __restrict__ flags,
-                   const uint32_t* __restrict__ link, double* __restrict__ gUin,
-                   const float* __restrict__ epsBuf, const int32_t* __restrict__ ntIdx,
-                   double* __restrict__ gEps, AdjStep q) {
+                   const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
+                   double* __restrict__ gUin, const float* __restrict__ epsBuf,
+                   const int32_t* __restrict__ ntIdx, double* __restrict__ gEps, AdjStep q,
+                   StepParams sp) {
+    using T = typename Conv::T;
     const uint32_t n = blockIdx.x * 256u + threadIdx.x;
     if (n >= q.N) return;
     uint32_t sM = 0u;   // lattices without walls hold fluid cells only
@@ -1026,7 +1041,40 @@ adjoint_les_kernel(const float* __restrict__ rec19, const float* __restrict__ la
 
     float g[19];
 #pragma unroll
-    for (int i = 0; i < 19; i++) g[i] = ld_off(rec19 + (size_t)i * q.N, n4);
+    for (int i = 0; i < 19; i++)
+        g[i] = Conv::ld(ld_off(rec19 + (size_t)i * q.N, n * (uint32_t)sizeof(T)));
+    if (WALLS && sp.parity == 1 && sM != 0u) {
+        const uint32_t dm = sM & lidm[n];   // moving-wall / outflow links (Kernels.metal:292-314)
+        if (dm != 0u) {
+            const int nx = (int)q.nx, ny = (int)q.ny, nz = (int)q.nz, plane = nx * ny;
+            Offsets o;
+            o.xm = (x == 0) ? nx - 1 : -1;
+            o.xp = (x == nx - 1) ? 1 - nx : 1;
+            o.ym = (y == 0) ? (ny - 1) * nx : -nx;
+            o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
+            o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
+            o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
+#define STRH_REDEFER(I)                                                                       \
+            if ((dm >> (kOpp[I] - 1)) & 1u) {                                                 \
+                const int sy = y - kCy[I] < 0 ? ny - 1 : (y - kCy[I] >= ny ? 0 : y - kCy[I]); \
+                const uint32_t sc = (uint32_t)((int)n + o.at<-kCx[I], -kCy[I], -kCz[I]>());   \
+                const uint8_t wf = flags[sc];                                                 \
+                if (wf == kOutflow) {                                                         \
+                    g[I] = -g[I];                                                             \
+                } else {                                                                      \
+                    float uwx, uwy, uwz;                                                      \
+                    wall_vel(wf, sy, sp, uwx, uwy, uwz);                                      \
+                    const float cu = (float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz; \
+                    g[I] += 6.0f * wgt<I>() * cu;                                             \
+                }                                                                             \
+            }
+            STRH_REDEFER(1) STRH_REDEFER(2) STRH_REDEFER(3) STRH_REDEFER(4) STRH_REDEFER(5)
+            STRH_REDEFER(6) STRH_REDEFER(7) STRH_REDEFER(8) STRH_REDEFER(9) STRH_REDEFER(10)
+            STRH_REDEFER(11) STRH_REDEFER(12) STRH_REDEFER(13) STRH_REDEFER(14) STRH_REDEFER(15)
+            STRH_REDEFER(16) STRH_REDEFER(17) STRH_REDEFER(18)
+#undef STRH_REDEFER
+        }
+    }
     // moments of g: rho - 1, p and the second moments sum c_a c_b g
     float rm1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
     float sxx = 0.0f, syy = 0.0f, szz = 0.0f, sxy = 0.0f, sxz = 0.0f, syz = 0.0f;
@@ -1674,18 +1722,25 @@ hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float4* rec,
                                float* lam, const uint8_t* flags, const uint32_t* link,
                                const float* alpha, double* gAlpha, double* gUin,
                                const float* eps, const int32_t* ntIdx, const float* recNt,
-                               double* gEps, const float* rec19, hipStream_t s) {
+                               double* gEps, const void* rec19, const uint32_t* lidm,
+                               const StepParams& sp, hipStream_t s) {
     AdjStep q = q0;
     q.blocks = (q.N + 255u) / 256u;
     const dim3 g(q.blocks), b(256);
     if (rec19) {
-#define STRH_ADJL(W, NTV) hipLaunchKernelGGL((adjoint_les_kernel<W, NTV>), g, b, 0, s, rec19, lamNext, lam, \
-                                             flags, link, gUin, eps, ntIdx, gEps, q)
-        if (ntIdx) {
-            if (walls) STRH_ADJL(true, true); else STRH_ADJL(false, true);
-        } else {
-            if (walls) STRH_ADJL(true, false); else STRH_ADJL(false, false);
+#define STRH_ADJL(C, W, NTV) hipLaunchKernelGGL((adjoint_les_kernel<C, W, NTV>), g, b, 0, s,              \
+                                                (const typename C::T*)rec19, lamNext, lam, flags, link,   \
+                                                lidm, gUin, eps, ntIdx, gEps, q, sp)
+#define STRH_ADJC(C)                                                                                  \
+        if (ntIdx) { if (walls) STRH_ADJL(C, true, true); else STRH_ADJL(C, false, true); }           \
+        else { if (walls) STRH_ADJL(C, true, false); else STRH_ADJL(C, false, false); }
+        switch (q.precision) {
+            case kFP32: STRH_ADJC(ConvF32); break;
+            case kFP16S: STRH_ADJC(ConvF16); break;
+            case kBF16: STRH_ADJC(ConvBF16); break;
+            default: return hipErrorInvalidValue;
         }
+#undef STRH_ADJC
 #undef STRH_ADJL
         return hipGetLastError();
     }

@@ -1247,11 +1247,11 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
 // float4) when given, objective accumulated into jacc when given.
 // Partial-cell lattices also record the 19 populations of their M cells with
 // eps > 0 into recNt (count * 19 * M floats) when given.
-// LES lattices record all 19 populations of every cell into rec19 (count *
-// 19 * N floats) instead.
+// LES lattices record all 19 populations of every cell into rec19 instead
+// (count * 19 * N values in the storage format).
 int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_t k0, int64_t count,
                 const strh_objective* o, const AdjPlan& P, float4* rec, double* jacc,
-                const int32_t* ntIdx = nullptr, float* recNt = nullptr, float* rec19 = nullptr) {
+                const int32_t* ntIdx = nullptr, float* recNt = nullptr, uint8_t* rec19 = nullptr) {
     const uint32_t M = (uint32_t)P.ntCells.size();
     for (int64_t s = 0; s < count; s++) {
         const int64_t k = k0 + s;
@@ -1271,7 +1271,7 @@ int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_
         hk.recNt = (recNt && ntIdx) ? recNt + (size_t)s * 19 * M : nullptr;
         hk.M = M;
         hk.wf = win ? P.wf : 0.0;
-        hk.rec19 = rec19 ? rec19 + (size_t)s * 19 * L->N : nullptr;
+        hk.rec19 = rec19 ? rec19 + (size_t)s * 19 * L->N * L->B : nullptr;
         HIP_TRY(launch_step_hooked(sl, hk, L->ctx->stream), "hooked step launch");
     }
     return STRH_OK;
@@ -1353,7 +1353,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         // populations under LES); K steps of it plus ceil(T/K) - 1 checkpoints.
         // When no K fits, the one that needs least (the allocation then
         // reports the shortfall).
-        const double perStep = les ? (double)N * 76 : (double)N * 16 + (double)M * 76;
+        const double perStep = les ? (double)N * 19 * L->B : (double)N * 16 + (double)M * 76;
         auto need = [&](int64_t k) { return (double)k * perStep + (double)((T + k - 1) / k - 1) * (double)ddf; };
         int64_t best = 1;
         K = 0;
@@ -1375,7 +1375,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     AdjWork W;
     double *jacc, *gA, *gU;
     float4* rec = nullptr;
-    float* rec19 = nullptr;
+    uint8_t* rec19 = nullptr;
     float *lam0, *lam1, *zeroAlpha = nullptr;
     uint8_t *ck, *work;
     if ((rc = W.get(&jacc, N * 8, "objective accumulator"))) return rc;
@@ -1384,7 +1384,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     if ((rc = W.get(&lam0, 19 * S * 4, "adjoint state"))) return rc;
     if ((rc = W.get(&lam1, 19 * S * 4, "adjoint state"))) return rc;
     if (les) {
-        if ((rc = W.get(&rec19, (size_t)K * 19 * N * 4, "recorded populations"))) return rc;
+        if ((rc = W.get(&rec19, (size_t)K * 19 * N * L->B, "recorded populations"))) return rc;
     } else {
         if ((rc = W.get(&rec, (size_t)K * N * 16, "recorded moments"))) return rc;
     }
@@ -1467,10 +1467,12 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
             q.spongeX0 = sp.spongeX0; q.spongeInvW = sp.spongeInvW; q.spongeTau = sp.spongeTau;
             q.cSmago = sp.cSmago;
             q.srt = sp.omegaMinus == sp.omega ? 1 : 0;
+            q.precision = L->precision;
             HIP_TRY(launch_adjoint_step(q, L->walls, rec ? rec + (size_t)(k - k0) * N : nullptr, lamN, lamT,
                                         L->flags, L->link, alpha, gA, gU, L->useEps ? L->eps : nullptr,
                                         ntIdx, recNt ? recNt + (size_t)(k - k0) * 19 * M : nullptr, gE,
-                                        rec19 ? rec19 + (size_t)(k - k0) * 19 * N : nullptr, st),
+                                        rec19 ? rec19 + (size_t)(k - k0) * 19 * N * L->B : nullptr, L->lidm,
+                                        sp, st),
                     "adjoint step launch");
             std::swap(lamN, lamT);
         }

@@ -293,3 +293,21 @@ def test_les_recording_forward_keeps_the_state_and_interval(gpu, precision):
     assert a.state_digest == b.state_digest == c.state_digest
     assert J1 == J2 and u1 == u2 and np.array_equal(i1["grad_eps"], i2["grad_eps"])
     assert i1["checkpoint_every"] == 4 and i2["checkpoint_every"] == 13
+
+
+@pytest.mark.parametrize("precision", ["fp16s", "bf16"])
+def test_vjp_les_16bit_storage_near_fp64_oracle(gpu, precision):
+    """16-bit lattices record the populations in their storage format and the
+    reverse step re-applies the inflow corrections of the odd pass: J and the
+    gradients stay within the storage precision of the FP64 oracle (FP16 about
+    1e-3, BF16 about 1e-2 of the largest component after 13 steps)."""
+    g, r, eps = _les_pair(gpu, precision, 3.0 / 16.0, True)
+    J, _, gu, info = g.vjp(**LES_OBJ, checkpoint_every=5, want_eps=True)
+    Jr, _, ger, gur = r.vjp(**LES_OBJ, with_eps=True)
+    ge = info["grad_eps"].reshape(-1)
+    tol = 5e-3 if precision == "fp16s" else 5e-2
+    assert abs(J - Jr) <= tol * abs(Jr), (J, Jr)
+    assert abs(gu - gur) <= tol * abs(gur), (gu, gur)
+    err = np.max(np.abs(ge - ger)) / np.max(np.abs(ger))
+    assert err <= tol, err
+    assert np.all(ge[eps.reshape(-1) == 0] == 0.0)
