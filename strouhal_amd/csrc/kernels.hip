@@ -630,6 +630,18 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     int nip = ni;
     asm("" : "+v"(nip));
     auto nbp = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB; };
+    // (the coordinates and the six axis offsets are recomputed from it too)
+    const uint32_t tp = fdiv((uint32_t)nip, p.divx);
+    const int xp = nip - (int)(tp * (uint32_t)nx);
+    const uint32_t zp = fdiv(tp, p.divy);
+    const int yp = (int)(tp - zp * (uint32_t)ny);
+    Offsets op;
+    op.xm = (xp == 0) ? nx - 1 : -1;
+    op.xp = (xp == nx - 1) ? 1 - nx : 1;
+    op.ym = (yp == 0) ? (ny - 1) * nx : -nx;
+    op.yp = (yp == ny - 1) ? -(ny - 1) * nx : nx;
+    op.zm = ((int)zp == 0) ? (nz - 1) * plane : -plane;
+    op.zp = ((int)zp == nz - 1) ? -(nz - 1) * plane : plane;
     // ... and the slot bases from a pinned copy of f, so the pull's 19 base
     // pointers (38 SGPRs) are not held across the collision either
     T* fpin = f;
@@ -650,8 +662,8 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                 float corr = 0.0f;                                                    \
                 float sign = 1.0f;                                                    \
                 if ((lM >> dstBit) & 1u) {                                            \
-                    const int dy = y + kCy[I] < 0 ? ny - 1 : (y + kCy[I] >= ny ? 0 : y + kCy[I]); \
-                    const uint32_t d = (uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>()); \
+                    const int dy = yp + kCy[I] < 0 ? ny - 1 : (yp + kCy[I] >= ny ? 0 : yp + kCy[I]); \
+                    const uint32_t d = (uint32_t)(nip + op.at<kCx[I], kCy[I], kCz[I]>()); \
                     const uint8_t wf = flags[d];                                      \
                     if (wf == kOutflow) {                                             \
                         sign = -1.0f;                                                 \
@@ -680,7 +692,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     {                                                                                 \
         constexpr int dstBit = I - 1;                                                 \
         constexpr int lo = I < kOpp[I] ? I : kOpp[I];                                 \
-        uint32_t off = nbp(o.at<kCx[I], kCy[I], kCz[I]>()) + (I == lo ? 0u : WB);     \
+        uint32_t off = nbp(op.at<kCx[I], kCy[I], kCz[I]>()) + (I == lo ? 0u : WB);     \
         if (WALLS && ((sM >> dstBit) & 1u)) off = nbs + (kOpp[I] == lo ? 0u : WB);    \
         St::wr(fs[lo], off, fh[I]);                                                   \
     }
