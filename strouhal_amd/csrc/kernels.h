@@ -25,6 +25,7 @@ struct StepLaunch {
     bool les;        // cSmago > 0
     bool has_force;  // body force non-zero
     uint32_t rows_bs;  // 256 / 512: odd pass uses LDS row staging; 0: direct
+    bool large;        // blocked image of 4 GiB or more (row-staged pass with row bases)
     const float* alpha;  // Brinkman porosity per cell, or null
     StepParams p;
 };

@@ -500,8 +500,17 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         if (NT && p.useEps) eps = epsBuf[n];
         const uint32_t t = fdiv(n, p.divx);
         const int x = (int)(n - t * (uint32_t)p.nx);
+        // A wave's 64 cells lie in one block of the layout: a uniform block
+        // base and a 32-bit offset inside the block reach any image size.
+        {
+            const uint32_t bk = __builtin_amdgcn_readfirstlane(n >> p.lay.lw);
+            T* fb = f + (size_t)bk * p.lay.bstride;
 #pragma unroll
-        for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nb);
+            for (int i = 0; i < 19; i++) fs[i] = fb + (size_t)i * p.lay.W;
+        }
+        const uint32_t nbi = (n & p.lay.mask) * (uint32_t)TB;
+#pragma unroll
+        for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nbi);
         const float alpha = BRINK ? alphaBuf[n] : 0.0f;
         if (HOOK && hk->rec19) record_populations<St>(*hk, fh, p.N, n);
         if (HOOK) run_hooks<LES>(*hk, fh, n, x, alpha, eps, p);
@@ -509,7 +518,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         // A separate copy of the offset keeps the stores on the scalar-base
         // form too (otherwise the shared base+offset sum is kept as 19
         // 64-bit VGPR addresses across the collision).
-        uint32_t nbs = nb;
+        uint32_t nbs = nbi;
         asm("" : "+v"(nbs));
         St::wr(fs[0], nbs, fh[0]);
 #pragma unroll
@@ -1270,7 +1279,7 @@ adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __re
 // (the AA property), so no other workgroup reads or writes it.
 template <int I> __device__ __forceinline__ constexpr int xslot() { return I <= 2 ? I - 1 : I - 5; }
 
-template <class St, int BS, bool NT, bool LES, bool FORCE>
+template <class St, int BS, bool NT, bool LES, bool FORCE, bool LARGE>
 __global__ void __launch_bounds__(BS)
 step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
               const float* __restrict__ epsBuf, StepParams p) {
@@ -1278,16 +1287,18 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
     __shared__ T row[10][BS];
     const uint32_t t = threadIdx.x;
     const uint32_t n = blockIdx.x * BS + t;   // N % BS == 0
-    T* fs[19];
-#pragma unroll
-    for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * p.lay.W;
     constexpr int TB = (int)sizeof(T);
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t xmask = (uint32_t)nx - 1u;
     const uint32_t x = n & xmask;
     const uint32_t rowbase = t - x;
-    const uint32_t r = fdiv(n, p.divx);
-    const uint32_t z = fdiv(r, p.divy);
+    // LARGE (images of 4 GiB and more): a wave lies inside one x-row (nx >= 64),
+    // so the row and its neighbour rows' first elements are uniform and every
+    // access is a 64-bit row base plus the lane's 32-bit x offset. Otherwise
+    // each access is a slot base plus the cell's 32-bit offset, which holds
+    // fewer registers.
+    const uint32_t r = LARGE ? __builtin_amdgcn_readfirstlane(fdiv(n, p.divx)) : fdiv(n, p.divx);
+    const uint32_t z = LARGE ? __builtin_amdgcn_readfirstlane(fdiv(r, p.divy)) : fdiv(r, p.divy);
     const int y = (int)(r - z * (uint32_t)ny);
     const int plane = nx * ny;
     Offsets o;
@@ -1296,15 +1307,28 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
     o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
     o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
     o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
-    const uint32_t nb = cell_elem(p.lay, n) * (uint32_t)TB;
-    // byte offset of the cell at n + d (same x, neighbouring row) from the slot bases
-    auto nbo = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)((int)n + d)) * (uint32_t)TB; };
+    const uint32_t n0 = r * (uint32_t)nx;
+    // slot i of the row n0 + d (rows never straddle a block of the layout)
+    auto rowp = [&](int i, int d) -> T* {
+        return f + (size_t)i * p.lay.W + cell_elem(p.lay, (uint32_t)((int)n0 + d));
+    };
+    const uint32_t xo = x * (uint32_t)TB;
+    // the same access for images below 4 GiB; the store copy is pinned so the
+    // push recomputes its offsets rather than keeping the pull's live
+    auto rd = [&](int i, int d) -> float {
+        if constexpr (LARGE) return St::rd(rowp(i, d), xo);
+        else return St::rd(f + (size_t)i * p.lay.W, cell_elem(p.lay, (uint32_t)((int)n + d)) * (uint32_t)TB);
+    };
+    auto rd_raw = [&](int i, int d) -> T {
+        if constexpr (LARGE) return St::rd_raw(rowp(i, d), xo);
+        else return St::rd_raw(f + (size_t)i * p.lay.W, cell_elem(p.lay, (uint32_t)((int)n + d)) * (uint32_t)TB);
+    };
 
     float fh[19];
-    fh[0] = St::rd(fs[0], nb);
+    fh[0] = rd(0, 0);
 #define STRH_RPULL(I)                                                                        \
-    if (kCx[I] == 0) fh[I] = St::rd(fs[kOpp[I]], nbo(o.at<0, -kCy[I], -kCz[I]>()));          \
-    else row[xslot<I>()][t] = St::rd_raw(fs[kOpp[I]], nbo(o.at<0, -kCy[I], -kCz[I]>()));
+    if (kCx[I] == 0) fh[I] = rd(kOpp[I], o.at<0, -kCy[I], -kCz[I]>());                       \
+    else row[xslot<I>()][t] = rd_raw(kOpp[I], o.at<0, -kCy[I], -kCz[I]>());
     STRH_RPULL(1) STRH_RPULL(2) STRH_RPULL(3) STRH_RPULL(4) STRH_RPULL(5) STRH_RPULL(6)
     STRH_RPULL(7) STRH_RPULL(8) STRH_RPULL(9) STRH_RPULL(10) STRH_RPULL(11) STRH_RPULL(12)
     STRH_RPULL(13) STRH_RPULL(14) STRH_RPULL(15) STRH_RPULL(16) STRH_RPULL(17) STRH_RPULL(18)
@@ -1322,14 +1346,19 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
     collide<LES, FORCE, NT>(fh, p, (int)x, eps, ntx, nty, ntz);
 
     __syncthreads();   // every shifted read of `row` is done
-    uint32_t nbs = nb;
-    asm("" : "+v"(nbs));
-    int nip = (int)n;   // pinned: recompute the row offsets instead of keeping them live
+    int nip = (int)n;   // pinned: the push recomputes the row offsets
     asm("" : "+v"(nip));
-    auto nbp = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB; };
-    St::wr(fs[0], nbs, fh[0]);
+    auto wr = [&](int i, int d, float v) {
+        if constexpr (LARGE) St::wr(rowp(i, d), xo, v);
+        else St::wr(f + (size_t)i * p.lay.W, cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB, v);
+    };
+    auto wr_raw = [&](int i, int d, T v) {
+        if constexpr (LARGE) St::wr_raw(rowp(i, d), xo, v);
+        else St::wr_raw(f + (size_t)i * p.lay.W, cell_elem(p.lay, (uint32_t)(nip + d)) * (uint32_t)TB, v);
+    };
+    wr(0, 0, fh[0]);
 #define STRH_RPUSH(I)                                                                        \
-    if (kCx[I] == 0) St::wr(fs[I], nbp(o.at<0, kCy[I], kCz[I]>()), fh[I]);                  \
+    if (kCx[I] == 0) wr(I, o.at<0, kCy[I], kCz[I]>(), fh[I]);                                \
     else row[xslot<I>()][rowbase + ((x + (uint32_t)kCx[I]) & xmask)] = St::st(fh[I]);
     STRH_RPUSH(1) STRH_RPUSH(2) STRH_RPUSH(3) STRH_RPUSH(4) STRH_RPUSH(5) STRH_RPUSH(6)
     STRH_RPUSH(7) STRH_RPUSH(8) STRH_RPUSH(9) STRH_RPUSH(10) STRH_RPUSH(11) STRH_RPUSH(12)
@@ -1337,7 +1366,7 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
 #undef STRH_RPUSH
     __syncthreads();
 #define STRH_RSTORE(I) \
-    if (kCx[I] != 0) St::wr_raw(fs[I], nbp(o.at<0, kCy[I], kCz[I]>()), row[xslot<I>()][t]);
+    if (kCx[I] != 0) wr_raw(I, o.at<0, kCy[I], kCz[I]>(), row[xslot<I>()][t]);
     STRH_RSTORE(1) STRH_RSTORE(2) STRH_RSTORE(7) STRH_RSTORE(8) STRH_RSTORE(9) STRH_RSTORE(10)
     STRH_RSTORE(11) STRH_RSTORE(12) STRH_RSTORE(13) STRH_RSTORE(14)
 #undef STRH_RSTORE
@@ -1636,15 +1665,16 @@ static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
     if (PARITY == 1 && !WALLS && L.rows_bs != 0 && !L.alpha) {
         using St = Store<Conv, kPolAligned>;
         StepParams p = L.p;
+#define STRH_ROWS(BSV, LG) hipLaunchKernelGGL((step_odd_rows<St, BSV, NT, LES, FORCE, LG>), dim3(p.blocks), \
+                                           dim3(BSV), 0, s, (T*)L.f, (float4*)L.force, L.eps, p)
         if (L.rows_bs == 512) {
             p.blocks = p.N / 512u;
-            hipLaunchKernelGGL((step_odd_rows<St, 512, NT, LES, FORCE>), dim3(p.blocks), dim3(512),
-                               0, s, (T*)L.f, (float4*)L.force, L.eps, p);
+            if (L.large) STRH_ROWS(512, true); else STRH_ROWS(512, false);
         } else {
             p.blocks = p.N / 256u;
-            hipLaunchKernelGGL((step_odd_rows<St, 256, NT, LES, FORCE>), dim3(p.blocks), dim3(256),
-                               0, s, (T*)L.f, (float4*)L.force, L.eps, p);
+            if (L.large) STRH_ROWS(256, true); else STRH_ROWS(256, false);
         }
+#undef STRH_ROWS
         return hipGetLastError();
     }
     using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;

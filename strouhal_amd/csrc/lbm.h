@@ -57,12 +57,16 @@ struct DdfLayout {
     uint32_t blocks;   // number of blocks
 };
 
-inline DdfLayout make_ddf_layout(uint64_t N, uint32_t bytes) {
+// large: the image may exceed 4 GiB. Only kernels that address the DDFs from
+// per-workgroup block or per-row bases (the even pass and the row-staged odd
+// pass of wall-free lattices) reach such an image; the direct odd pass's
+// 32-bit offsets from the slot bases do not.
+inline DdfLayout make_ddf_layout(uint64_t N, uint32_t bytes, bool large = false) {
     DdfLayout l;
     const uint32_t lw = bytes == 4 ? 13u : 14u;
     const uint64_t W = 1ull << lw;
     const uint64_t nb = (N + W - 1) / W;
-    if (N > W && nb * 19 * W * bytes < (1ull << 32)) {
+    if (N > W && (large || nb * 19 * W * bytes < (1ull << 32))) {
         l.W = (uint32_t)W;
         l.lw = lw;
         l.mask = (uint32_t)W - 1u;

@@ -102,6 +102,11 @@ namespace {
 
 size_t ddf_bytes(const strh_lattice* L) { return (size_t)ddf_elements(L->lay) * L->B; }
 
+// A blocked image of 4 GiB or more: only the even pass and the row-staged odd
+// pass reach it (per-workgroup block / per-row bases); porosity, batches and
+// the adjoint, which run the direct odd pass, refuse such lattices.
+bool large_image(const strh_lattice* L) { return L->lay.blocks > 1 && ddf_bytes(L) >= (1ull << 32); }
+
 // Copy cells [c0, c0 + count) of DDF slot `slot` between the device image
 // and the contiguous buffer `lin` (count elements), in at most three copies:
 // the rest of a partly covered first block, one pitched copy of the whole
@@ -201,6 +206,7 @@ StepLaunch make_launch(const strh_lattice* L, int64_t t, bool writeForce) {
     s.les = L->cSmago > 0.0f;
     s.has_force = !(L->bodyForce[0] == 0.0f && L->bodyForce[1] == 0.0f && L->bodyForce[2] == 0.0f);
     s.rows_bs = L->walls ? 0u : rows_block_size((uint32_t)L->nx, L->N);
+    s.large = large_image(L);
     s.alpha = L->alpha;
     s.p = make_params(L, t, writeForce);
     return s;
@@ -365,7 +371,10 @@ int strh_lattice_create(strh_ctx* c, const strh_lattice_desc* d, const uint8_t* 
     L->N = (uint32_t)N;
     L->S = (uint32_t)((N + 63u) & ~63ull);
     L->B = d->precision == STRH_FP32 ? 4 : 2;
-    L->lay = make_ddf_layout(N, (uint32_t)L->B);
+    bool anyWall = false;
+    for (uint64_t n = 0; n < N && !anyWall; n++) anyWall = flags[n] != STRH_FLUID;
+    // blocked beyond 4 GiB only where every pass addresses from block / row bases
+    L->lay = make_ddf_layout(N, (uint32_t)L->B, !anyWall && rows_block_size((uint32_t)d->nx, N) != 0);
     L->omega = d->omega;
     L->omegaMinus = d->omega_minus < 0.0f ? d->omega : d->omega_minus;
     for (int k = 0; k < 3; k++) { L->lid[k] = d->lid[k]; L->bodyForce[k] = d->force[k]; }
@@ -387,8 +396,6 @@ int strh_lattice_create(strh_ctx* c, const strh_lattice_desc* d, const uint8_t* 
     if ((e = hipMemsetAsync(L->f, 0, fbytes, c->stream)) != hipSuccess) return bail(e, "hipMemset(f)");
     if ((e = hipMalloc(&L->flags, N)) != hipSuccess) return bail(e, "hipMalloc(flags)");
     if ((e = hipMemcpy(L->flags, flags, N, hipMemcpyHostToDevice)) != hipSuccess) return bail(e, "copy flags");
-    bool anyWall = false;
-    for (uint64_t n = 0; n < N && !anyWall; n++) anyWall = flags[n] != STRH_FLUID;
     L->walls = anyWall;
     if (anyWall) {
         std::vector<uint32_t> link, lidm;
@@ -495,6 +502,9 @@ int strh_set_porosity(strh_lattice* L, const float* alpha) {
         L->alpha = nullptr;
         return STRH_OK;
     }
+    if (large_image(L))
+        return fail(STRH_ERR_UNSUPPORTED, "porosity needs the direct odd pass, which addresses images "
+                                          "below 4 GiB; this lattice's image is %zu bytes", ddf_bytes(L));
     for (int64_t n = 0; n < L->N; n++)
         if (!(alpha[n] >= 0.0f) || !std::isfinite(alpha[n]))
             return fail(STRH_ERR_INVALID, "porosity at cell %lld is not a finite value >= 0",
@@ -1004,6 +1014,8 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
         if (!lats[i] || lats[i]->ctx != c) return fail(STRH_ERR_INVALID, "lattice %d is not on this context", i);
         if (lats[i]->precision != lats[0]->precision) return fail(STRH_ERR_INVALID, "lattices must share one precision");
         if (lats[i]->alpha) return fail(STRH_ERR_INVALID, "lattice %d has porosity: batches run plain lattices", i);
+        if (large_image(lats[i]))
+            return fail(STRH_ERR_UNSUPPORTED, "lattice %d: batches address images below 4 GiB", i);
         for (int j = 0; j < i; j++)
             if (lats[j] == lats[i]) return fail(STRH_ERR_INVALID, "lattice %d appears twice", i);
     }
@@ -1214,6 +1226,8 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
     if (L->alpha && (L->useEps || L->cSmago > 0.0f))
         return fail(STRH_ERR_UNSUPPORTED, "the adjoint does not combine Brinkman porosity with partial "
                                           "cells or LES");
+    if (large_image(L))
+        return fail(STRH_ERR_UNSUPPORTED, "the adjoint's recording step addresses images below 4 GiB");
     P.T = o->spinup + o->avg;
     std::vector<uint8_t> fl(L->N);
     HIP_TRY(hipMemcpy(fl.data(), L->flags, L->N, hipMemcpyDeviceToHost), "copy flags");
