@@ -511,3 +511,33 @@ def test_error_paths(gpu):
     sim2 = Simulation(gpu, "fp32", nx=8, ny=8, nz=8, omega=1.0, wants_forces=True)
     with pytest.raises(StrouhalError):
         sim2.probe_force((0, 9), (0, 3))
+
+
+def test_blocked_image_beyond_4gib(gpu):
+    """A wall-free FP32 lattice whose blocked DDF image exceeds 4 GiB
+    (512 x 512 x 216: 4.30 GB). The even pass addresses it from the wave's
+    block base, the row-staged odd pass from row bases; two steps from the
+    Taylor-Green field equal the FP32 oracle's bit for bit. Porosity and
+    batches, which run the direct odd pass, refuse the lattice."""
+    import ctypes
+    from oracle.oracle import OracleSim
+    from strouhal_amd import Simulation, StrouhalError
+    from strouhal_amd._lib import lib
+    from strouhal_amd.batch import Batch
+    nx, ny, nz = 512, 512, 216
+    g = Simulation(gpu, "fp32", nx=nx, ny=ny, nz=nz, omega=1.9, omega_minus=1.2)
+    sz = ctypes.c_size_t()
+    lib().strh_ddf_bytes(g._h, ctypes.byref(sz))
+    assert sz.value >= 1 << 32
+    g.init_field(2, 0.05)
+    r = OracleSim(nx, ny, nz, g.omega, g.omega_minus, arith="f32", storage="f32", threads=16)
+    r.upload_ddf(g.download_ddf())
+    g.run(2)
+    r.run(2)
+    assert g.state_digest == r.state_digest
+    with pytest.raises(StrouhalError) as e:
+        g.set_porosity(np.zeros(g.cells, dtype=np.float32))
+    assert e.value.code == 8
+    with pytest.raises(StrouhalError):
+        Batch([g])
+    g.close()
