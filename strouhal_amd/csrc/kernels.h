@@ -57,8 +57,8 @@ struct BatchLat {
     double* series;          // 3 doubles per sample
 };
 
-// Hooks of the recording forward step of the adjoint (walls, Guo forcing and
-// Brinkman porosity; no partial cells, LES or sponge).
+// Hooks of the recording forward step of the adjoint (walls, Guo forcing,
+// Brinkman porosity, partial cells, LES, sponge).
 struct AdjHooks {
     float4* rec;             // [N] pre-collision moments (rho - 1, p_x, p_y, p_z), or null
     double* jacc;            // per-cell objective accumulator, or null

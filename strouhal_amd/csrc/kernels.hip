@@ -3,14 +3,14 @@
 // The hot path is `step_kernel`: one AA-pattern time step, collide and stream
 // fused, in place (the reference's `step` + `collide`,
 // Sources/StrouhalCore/Kernels.metal:105-355). Design for CDNA4:
-//  * SoA DDF slots f[i*S + n] with the slot stride S padded to 64 cells, so
-//    every wave's load of one slot is one run of 64 consecutive values
-//    (256 B fp32 / 128 B fp16) — the even pass is perfectly coalesced; the
-//    odd pass shifts x by +-1 cell, which the L2 absorbs because one
-//    workgroup covers whole x-runs.
-//  * One thread per cell, 256-thread workgroups, block ids remapped so
-//    consecutive cell ranges land on the same XCD (each of the 8 XCDs has its
-//    own L2 and the hardware deals workgroups round-robin over them).
+//  * Blocked SoA DDF layout (DdfLayout in lbm.h): slot i of a block of W cells
+//    is one contiguous run, so every wave's load of one slot is 64
+//    consecutive values (256 B fp32 / 128 B fp16) and the even pass is
+//    perfectly coalesced. The odd pass shifts x by +-1 cell: wall-free
+//    lattices stage the shift through LDS by whole x-rows, wall lattices read
+//    the shifted runs directly.
+//  * One thread per cell, 256-thread workgroups in plain block order, so all
+//    XCDs stream neighbouring parts of the lattice at any moment.
 //  * Specialised per (storage, parity, walls, NT, LES, forcing): the all-fluid
 //    periodic bulk reads and writes exactly 19 + 19 DDFs per cell and
 //    nothing else (152 B fp32 / 76 B fp16 per cell).
@@ -725,9 +725,9 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                                                         alphaBuf);
 }
 
-// Forward step of the differentiable configuration (Guo forcing, Brinkman
-// porosity) with the adjoint hooks. The arithmetic is that of step_kernel:
-// with alpha = 0 and no body force the DDFs are the plain update's bits.
+// Forward step with the adjoint hooks (Guo forcing with Brinkman porosity, or
+// partial cells and / or LES). The arithmetic is that of step_kernel: with
+// alpha = 0 and no body force the DDFs are the plain update's bits.
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool BRINK>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
@@ -1590,8 +1590,7 @@ __device__ __forceinline__ bool is_force_step(const BatchLat& B, int64_t t) {
 // a body force (otherwise each lattice takes its own forcing path).
 template <class Conv, int BS, int PAR, bool LES, bool FORCED>
 __global__ void __launch_bounds__(BS)
-step_batch(const BatchLat* __restrict__ lats, int L, uint32_t totalBlocks,
-           const int64_t* __restrict__ base, int k) {
+step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k) {
     const uint32_t b = blockIdx.x;
     const int l = batch_find(lats, L, b);
     const BatchLat& B = lats[l];
@@ -1908,7 +1907,7 @@ template <class Conv, int BS, int PAR>
 static void launch_batch_par(bool les, bool forced, const BatchLat* lats, int L, uint32_t totalBlocks,
                              const int64_t* base, int k, hipStream_t s) {
 #define STRH_BATCH(LESV, FV) hipLaunchKernelGGL((step_batch<Conv, BS, PAR, LESV, FV>), dim3(totalBlocks), \
-                                                dim3(BS), 0, s, lats, L, totalBlocks, base, k)
+                                                dim3(BS), 0, s, lats, L, base, k)
     if (les) { if (forced) STRH_BATCH(true, true); else STRH_BATCH(true, false); }
     else { if (forced) STRH_BATCH(false, true); else STRH_BATCH(false, false); }
 #undef STRH_BATCH
@@ -1919,7 +1918,7 @@ static void launch_batch_conv(int parity, bool les, bool forced, const BatchLat*
     if (parity == 0) launch_batch_par<Conv, BS, 0>(les, forced, lats, L, totalBlocks, base, k, s);
     else if (parity == 1) launch_batch_par<Conv, BS, 1>(les, forced, lats, L, totalBlocks, base, k, s);
     else hipLaunchKernelGGL((step_batch<Conv, BS, -1, true, true>), dim3(totalBlocks), dim3(BS), 0, s,
-                            lats, L, totalBlocks, base, k);
+                            lats, L, base, k);
 }
 template <int BS>
 static hipError_t launch_batch_bs(int precision, int parity, bool les, bool forced, const BatchLat* lats,

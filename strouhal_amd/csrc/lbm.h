@@ -104,7 +104,7 @@ struct StepParams {
     float cF[9];
     float threeCF[9];
     FastDiv divx, divy;  // n -> (x, y, z)
-    uint32_t blocks;     // grid size, for the XCD-aware block remap
+    uint32_t blocks;     // grid size of the launch
 };
 
 }  // namespace strh

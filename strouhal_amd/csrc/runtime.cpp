@@ -1387,11 +1387,11 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     hipStream_t st = c->stream;
 
     AdjWork W;
-    double *jacc, *gA, *gU;
+    double *jacc = nullptr, *gA = nullptr, *gU = nullptr;
     float4* rec = nullptr;
     uint8_t* rec19 = nullptr;
-    float *lam0, *lam1, *zeroAlpha = nullptr;
-    uint8_t *ck, *work;
+    float *lam0 = nullptr, *lam1 = nullptr, *zeroAlpha = nullptr;
+    uint8_t *ck = nullptr, *work = nullptr;
     if ((rc = W.get(&jacc, N * 8, "objective accumulator"))) return rc;
     if ((rc = W.get(&gA, N * 8, "porosity gradient"))) return rc;
     if ((rc = W.get(&gU, N * 8, "inlet gradient"))) return rc;
