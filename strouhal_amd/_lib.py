@@ -30,8 +30,8 @@ EXPORTS = [
     "strh_download_force", "strh_save_state", "strh_load_state", "strh_snapshot", "strh_restore", "strh_sha256",
     "strh_sha256_force_portable", "strh_batch_create", "strh_batch_destroy",
     "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
-    "strh_batch_set_block_size", "strh_batch_autotune", "strh_batch_set_graph_steps",
-    "strh_batch_gpu_seconds", "strh_objective_eval", "strh_vjp",
+    "strh_batch_set_block_size", "strh_batch_autotune", "strh_batch_set_tiles", "strh_batch_tiles",
+    "strh_batch_set_graph_steps", "strh_batch_gpu_seconds", "strh_objective_eval", "strh_vjp",
 ]
 
 
@@ -135,6 +135,8 @@ def lib():
         "strh_batch_probe_series": [vp, i32, vp, i64, ctypes.POINTER(i64)],
         "strh_batch_set_block_size": [vp, i32],
         "strh_batch_autotune": [vp, ctypes.POINTER(i32)],
+        "strh_batch_set_tiles": [vp, i32],
+        "strh_batch_tiles": [vp, ctypes.POINTER(i32), ctypes.POINTER(i64)],
         "strh_batch_set_graph_steps": [vp, i32],
         "strh_batch_gpu_seconds": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_objective_eval": [vp, ctypes.POINTER(Objective), ctypes.POINTER(ctypes.c_double)],

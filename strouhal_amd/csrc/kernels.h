@@ -42,6 +42,9 @@ struct BatchLat {
     const float* eps;
     uint32_t block0;         // first workgroup of this lattice in the batch grid
     uint32_t nblocks;
+    void* f2;                // second DDF image of the tiled launches (same layout), or null
+    uint32_t tile0;          // first workgroup of this lattice in a tiled launch
+    uint32_t tilesX;         // tiles along x
     int64_t step0;           // step count at the start of the batch run
     int64_t tEnd;            // absolute step at which this lattice stops
     float uinTarget;
@@ -114,7 +117,14 @@ hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uin
 // lattice has a Smagorinsky constant; forced: some lattice has a body force.
 hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, const BatchLat* lats,
                              int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
-hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, hipStream_t s);
+// Probes the force step among steps k .. k + span - 1 (at most one per lattice).
+hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s);
+// Tiled launch of K steps (k .. k + K - 1) for lattices one cell deep; mode 1: K = 4,
+// mode 2: K = 2. swap 0 reads f and writes f2, swap 1 the reverse.
+hipError_t launch_batch_tiles(int precision, int mode, const BatchLat* lats, int L, uint32_t tileBlocks,
+                              const int64_t* base, int k, int swap, hipStream_t s);
+// Tile shape of a mode: width and height with the halo, and K.
+void batch_tile_shape(int precision, int mode, int* tw, int* th, int* K);
 hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
 hipError_t launch_set_base(int64_t* base, int64_t v, hipStream_t s);
 void box_reduce_plan(uint32_t M, uint32_t max_partials, uint32_t* Pb, uint32_t* chunk);

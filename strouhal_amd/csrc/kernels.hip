@@ -63,19 +63,23 @@ struct ConvBF16 {
 // addressing form. POL bit 0: nontemporal loads, bit 1: nontemporal stores
 // (streaming data touched once per pass; keeps it from displacing lines
 // that still have a reader in the same pass).
+// The DDFs are always in global memory: the accesses go through global
+// address-space pointers, so kernels that read their base pointers from a
+// descriptor in memory (the batch) get global instead of FLAT instructions.
+#define GLOBAL __attribute__((address_space(1)))
 template <class Conv, int POL> struct Store : Conv {
     using T = typename Conv::T;
     __device__ static float rd(const T* base, uint32_t off) {
-        const T* p = (const T*)((const char*)base + off);
+        const GLOBAL T* p = (const GLOBAL T*)((const char*)base + off);
         return Conv::ld((POL & 1) ? __builtin_nontemporal_load(p) : *p);
     }
     __device__ static void wr(T* base, uint32_t off, float v) { wr_raw(base, off, Conv::st(v)); }
     __device__ static T rd_raw(const T* base, uint32_t off) {
-        const T* p = (const T*)((const char*)base + off);
+        const GLOBAL T* p = (const GLOBAL T*)((const char*)base + off);
         return (POL & 1) ? __builtin_nontemporal_load(p) : *p;
     }
     __device__ static void wr_raw(T* base, uint32_t off, T t) {
-        T* p = (T*)((char*)base + off);
+        GLOBAL T* p = (GLOBAL T*)((char*)base + off);
         if (POL & 2) __builtin_nontemporal_store(t, p); else *p = t;
     }
 };
@@ -476,14 +480,20 @@ __device__ __forceinline__ void record_populations(const AdjHooks& hk, const flo
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
           bool HOOK = false>
 __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
-                                          const uint8_t* __restrict__ flags,
-                                          const uint32_t* __restrict__ link,
-                                          const uint32_t* __restrict__ lidm,
+                                          const uint8_t* __restrict__ flags_,
+                                          const uint32_t* __restrict__ link_,
+                                          const uint32_t* __restrict__ lidm_,
                                           float4* __restrict__ force,
-                                          const float* __restrict__ epsBuf,
+                                          const float* __restrict__ epsBuf_,
                                           const StepParams& p, uint32_t n,
-                                          const float* __restrict__ alphaBuf = nullptr,
+                                          const float* __restrict__ alphaBuf_ = nullptr,
                                           const AdjHooks* hk = nullptr) {
+    // side arrays through global pointers (see Store)
+    const GLOBAL uint8_t* __restrict__ flags = (const GLOBAL uint8_t*)flags_;
+    const GLOBAL uint32_t* __restrict__ link = (const GLOBAL uint32_t*)link_;
+    const GLOBAL uint32_t* __restrict__ lidm = (const GLOBAL uint32_t*)lidm_;
+    const GLOBAL float* __restrict__ epsBuf = (const GLOBAL float*)epsBuf_;
+    const GLOBAL float* __restrict__ alphaBuf = (const GLOBAL float*)alphaBuf_;
     using T = typename St::T;
     T* fs[19];
 #pragma unroll
@@ -1619,20 +1629,23 @@ step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
     }
 }
 
-// After batch step k: for every lattice whose step t was a force step, sum the
+// After batch steps k .. k + span - 1: for every lattice with a force step t
+// among them (probe periods are at least span, so at most one), sum the
 // per-cell momentum exchange over its probe box into its time series. Block
 // (l, c) sums component c of lattice l with exactly the chunking of the
 // single-lattice probe (reduce_stage1 over Pb chunks, then reduce_stage2), so
 // the FP64 values are the same bits.
 __global__ void __launch_bounds__(256)
-probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k) {
+probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k, int span) {
     __shared__ double lds[256];
     __shared__ double part[kBatchMaxChunks];
     const int l = blockIdx.x / 3, c = blockIdx.x % 3;
     if (l >= L) return;
     const BatchLat& B = lats[l];
-    const int64_t t = B.step0 + *base + k;
-    if (t >= B.tEnd || !is_force_step(B, t)) return;
+    int64_t t = B.step0 + *base + k;
+    const int64_t tLast = t + span;
+    while (t < tLast && t < B.tEnd && !is_force_step(B, t)) t++;
+    if (t >= tLast || t >= B.tEnd) return;
     const BoxForceElem e{B.force, (uint32_t)B.p.nx, (uint32_t)B.p.ny, B.bx0, B.by0, B.fbw, B.fbh, c};
     for (uint32_t pb = 0; pb < B.Pb; pb++) {
         const uint32_t lo = pb * B.chunk;
@@ -1647,6 +1660,296 @@ probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base
     for (uint32_t q = threadIdx.x; q < B.Pb; q += 256u) acc += part[q];
     const double total = block_tree_sum(acc, lds);
     if (threadIdx.x == 0) B.series[3 * ((t - B.fT0) / B.fPeriod) + c] = total;
+}
+
+// ---- temporally blocked batch step for lattices one cell deep ------------
+// A lattice with nz = 1 advances K steps per launch. A workgroup owns an
+// IW x IH patch of cells (IW = TW - 2K, IH = TH - 2K). It loads the patch with
+// a K-cell halo, a TW x TH tile wrapped periodically like the step's
+// neighbours, into LDS in the storage format, runs the K AA steps there and
+// writes the patch back. Step j updates the tile cells at distance >= j from
+// the tile edge: an even (in-place) step needs nothing around a cell, an odd
+// step pulls from and pushes to the cell's 8 in-plane neighbours, so after
+// step j the cells at distance >= j + 1 hold exactly the values the
+// lattice-wide pass gives them, and after K steps the patch does. Each cell's
+// arithmetic is the one-step batch kernel's (walls, moving walls, partial
+// cells, sponge, momentum exchange), and every value passes through the
+// storage format between steps as it does in HBM, so the result is the same
+// bits. Halo cells of one tile are patch cells of another, so a launch reads
+// one image and writes the other (`swap` picks the direction); the host pairs
+// the launches so the state is back in `f` after each group.
+__device__ __forceinline__ int batch_find_tile(const BatchLat* lats, int L, uint32_t b) {
+    int l = 0;
+    while (l + 1 < L && b >= lats[l + 1].tile0) l++;
+    return l;
+}
+
+// Step t of lattice B with the ramped boundary speeds and the probe flag.
+__device__ __forceinline__ void batch_params(StepParams& p, const BatchLat& B, int64_t t) {
+    const float r = t < B.rampSteps ? B.ramp[t] : 1.0f;      // Sim.swift:258-266
+    p.uin = B.uinTarget * r;
+    p.ulidX = B.lid[0] * r;
+    p.ulidY = B.lid[1] * r;
+    p.ulidZ = B.lid[2] * r;
+    p.writeForce = is_force_step(B, t) ? 1 : 0;
+}
+
+// The per-cell side arrays of a tiled lattice, as global-memory pointers.
+struct TileSide {
+    const GLOBAL uint8_t* flags;
+    const GLOBAL uint32_t* link;
+    const GLOBAL uint32_t* lidm;
+    const GLOBAL float* eps;
+    float4* force;           // written on probe steps only
+};
+
+// The even pass of one tile cell (step_cell, PARITY 0): own slots in, swapped
+// slots out.
+template <class Conv, int C>
+__device__ __forceinline__ void tile_even(typename Conv::T* __restrict__ lds, const TileSide& sd,
+                                          const StepParams& p, uint32_t n, int cl, int gx, bool own) {
+    if (sd.link[n] & kNotFluidBit) return;      // bit 31: the cell is not fluid
+    const float eps = p.useEps ? sd.eps[n] : 0.0f;
+    float fh[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) fh[i] = Conv::ld(lds[i * C + cl]);
+    float ntx = 0.0f, nty = 0.0f, ntz = 0.0f;
+    collide<false, false, true>(fh, p, gx, eps, ntx, nty, ntz);
+    lds[cl] = Conv::st(fh[0]);
+#pragma unroll
+    for (int i = 1; i < 19; i++) lds[kOpp[i] * C + cl] = Conv::st(fh[i]);
+    if (own && p.writeForce != 0 && eps > 0.0f) sd.force[n] = make_float4(ntx, nty, ntz, 0.0f);
+}
+
+// The odd pass of one tile cell (step_cell, PARITY 1): pull from the in-plane
+// neighbours (own slot I across a wall link), the moving-wall corrections,
+// collide, push (own opposite slot across a wall link), momentum exchange.
+// Every access stays inside the tile for cells at distance >= 1 from its edge.
+template <class Conv, int TW, int C>
+__device__ __forceinline__ void tile_odd(typename Conv::T* __restrict__ lds, const TileSide& sd,
+                                         const StepParams& p, uint32_t n, int cl, int gx, int gy, bool own) {
+    const uint32_t lw = sd.link[n];
+    if (lw & kNotFluidBit) return;
+    const uint32_t sM = lw & 0x3ffffu;
+    const uint32_t lM = sM != 0u ? sd.lidm[n] : 0u;
+    const float eps = p.useEps ? sd.eps[n] : 0.0f;
+    const int nx = p.nx, ny = p.ny;
+    float fh[19];
+    fh[0] = Conv::ld(lds[cl]);
+#define STRH_TPULL(I)                                                                 \
+    {                                                                                 \
+        constexpr int srcBit = kOpp[I] - 1;                                           \
+        const int a = ((sM >> srcBit) & 1u) ? I * C + cl                              \
+                                             : kOpp[I] * C + cl - kCx[I] - kCy[I] * TW; \
+        fh[I] = Conv::ld(lds[a]);                                                     \
+    }
+    STRH_TPULL(1) STRH_TPULL(2) STRH_TPULL(3) STRH_TPULL(4) STRH_TPULL(5) STRH_TPULL(6)
+    STRH_TPULL(7) STRH_TPULL(8) STRH_TPULL(9) STRH_TPULL(10) STRH_TPULL(11) STRH_TPULL(12)
+    STRH_TPULL(13) STRH_TPULL(14) STRH_TPULL(15) STRH_TPULL(16) STRH_TPULL(17) STRH_TPULL(18)
+#undef STRH_TPULL
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    if (sM != 0u) {
+        uint32_t deferred = 0u;
+#define STRH_TWALLIN(I)                                                               \
+        {                                                                             \
+            constexpr int srcBit = kOpp[I] - 1;                                       \
+            if ((sM >> srcBit) & 1u) {                                                \
+                if ((lM >> srcBit) & 1u) deferred |= 1u << I;                         \
+                if (p.writeForce != 0) {                                              \
+                    ax = acc<-kCx[I]>(ax, fh[I]);                                     \
+                    ay = acc<-kCy[I]>(ay, fh[I]);                                     \
+                    az = acc<-kCz[I]>(az, fh[I]);                                     \
+                }                                                                     \
+            }                                                                         \
+        }
+        STRH_TWALLIN(1) STRH_TWALLIN(2) STRH_TWALLIN(3) STRH_TWALLIN(4) STRH_TWALLIN(5) STRH_TWALLIN(6)
+        STRH_TWALLIN(7) STRH_TWALLIN(8) STRH_TWALLIN(9) STRH_TWALLIN(10) STRH_TWALLIN(11) STRH_TWALLIN(12)
+        STRH_TWALLIN(13) STRH_TWALLIN(14) STRH_TWALLIN(15) STRH_TWALLIN(16) STRH_TWALLIN(17) STRH_TWALLIN(18)
+#undef STRH_TWALLIN
+        if (deferred != 0u) {
+            Offsets o;   // nz = 1: the z neighbour is the cell itself
+            o.xm = (gx == 0) ? nx - 1 : -1;
+            o.xp = (gx == nx - 1) ? 1 - nx : 1;
+            o.ym = (gy == 0) ? (ny - 1) * nx : -nx;
+            o.yp = (gy == ny - 1) ? -(ny - 1) * nx : nx;
+            o.zm = 0;
+            o.zp = 0;
+            const int ni = (int)n;
+#define STRH_TDEFER(I)                                                                \
+            if ((deferred >> I) & 1u) {                                               \
+                const int sy = gy - kCy[I] < 0 ? ny - 1 : (gy - kCy[I] >= ny ? 0 : gy - kCy[I]); \
+                const uint8_t wf = sd.flags[(uint32_t)(ni + o.at<-kCx[I], -kCy[I], -kCz[I]>())]; \
+                if (wf == kOutflow) {                                                 \
+                    fh[I] = -fh[I];                                                   \
+                } else {                                                              \
+                    float uwx, uwy, uwz;                                              \
+                    wall_vel(wf, sy, p, uwx, uwy, uwz);                               \
+                    const float cu = (float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz; \
+                    const float corr = 6.0f * wgt<I>() * cu;                          \
+                    fh[I] += corr;                                                    \
+                    if (p.writeForce != 0) {                                          \
+                        ax = acc<-kCx[I]>(ax, corr);                                  \
+                        ay = acc<-kCy[I]>(ay, corr);                                  \
+                        az = acc<-kCz[I]>(az, corr);                                  \
+                    }                                                                 \
+                }                                                                     \
+            }
+            STRH_TDEFER(1) STRH_TDEFER(2) STRH_TDEFER(3) STRH_TDEFER(4) STRH_TDEFER(5) STRH_TDEFER(6)
+            STRH_TDEFER(7) STRH_TDEFER(8) STRH_TDEFER(9) STRH_TDEFER(10) STRH_TDEFER(11) STRH_TDEFER(12)
+            STRH_TDEFER(13) STRH_TDEFER(14) STRH_TDEFER(15) STRH_TDEFER(16) STRH_TDEFER(17) STRH_TDEFER(18)
+#undef STRH_TDEFER
+        }
+    }
+    float ntx = 0.0f, nty = 0.0f, ntz = 0.0f;
+    collide<false, false, true>(fh, p, gx, eps, ntx, nty, ntz);
+    ax += ntx;
+    ay += nty;
+    az += ntz;
+    if (sM != 0u) {
+        Offsets o;
+        o.xm = (gx == 0) ? nx - 1 : -1;
+        o.xp = (gx == nx - 1) ? 1 - nx : 1;
+        o.ym = (gy == 0) ? (ny - 1) * nx : -nx;
+        o.yp = (gy == ny - 1) ? -(ny - 1) * nx : nx;
+        o.zm = 0;
+        o.zp = 0;
+        const int ni = (int)n;
+#define STRH_TWALLOUT(I)                                                              \
+        {                                                                             \
+            constexpr int dstBit = I - 1;                                             \
+            if ((sM >> dstBit) & 1u) {                                                \
+                float corr = 0.0f;                                                    \
+                float sign = 1.0f;                                                    \
+                if ((lM >> dstBit) & 1u) {                                            \
+                    const int dy = gy + kCy[I] < 0 ? ny - 1 : (gy + kCy[I] >= ny ? 0 : gy + kCy[I]); \
+                    const uint8_t wf = sd.flags[(uint32_t)(ni + o.at<kCx[I], kCy[I], kCz[I]>())]; \
+                    if (wf == kOutflow) {                                             \
+                        sign = -1.0f;                                                 \
+                    } else {                                                          \
+                        float uwx, uwy, uwz;                                          \
+                        wall_vel(wf, dy, p, uwx, uwy, uwz);                           \
+                        corr = 6.0f * wgt<I>()                                        \
+                             * ((float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz); \
+                    }                                                                 \
+                }                                                                     \
+                if (p.writeForce != 0) {                                              \
+                    const float v = sign * fh[I] + 2.0f * wgt<I>();                   \
+                    ax = acc<kCx[I]>(ax, v);                                          \
+                    ay = acc<kCy[I]>(ay, v);                                          \
+                    az = acc<kCz[I]>(az, v);                                          \
+                }                                                                     \
+                fh[I] = sign * fh[I] - corr;                                          \
+            }                                                                         \
+        }
+        STRH_TWALLOUT(1) STRH_TWALLOUT(2) STRH_TWALLOUT(3) STRH_TWALLOUT(4) STRH_TWALLOUT(5) STRH_TWALLOUT(6)
+        STRH_TWALLOUT(7) STRH_TWALLOUT(8) STRH_TWALLOUT(9) STRH_TWALLOUT(10) STRH_TWALLOUT(11) STRH_TWALLOUT(12)
+        STRH_TWALLOUT(13) STRH_TWALLOUT(14) STRH_TWALLOUT(15) STRH_TWALLOUT(16) STRH_TWALLOUT(17) STRH_TWALLOUT(18)
+#undef STRH_TWALLOUT
+    }
+    // The push addresses are the pull addresses of the opposite directions;
+    // a pinned copy of the cell index makes them recomputed here instead of
+    // kept across the collision.
+    int cp = cl;
+    asm("" : "+v"(cp));
+    lds[cp] = Conv::st(fh[0]);
+#define STRH_TPUSH(I)                                                                 \
+    {                                                                                 \
+        constexpr int dstBit = I - 1;                                                 \
+        const int a = ((sM >> dstBit) & 1u) ? kOpp[I] * C + cp                        \
+                                             : I * C + cp + kCx[I] + kCy[I] * TW;     \
+        lds[a] = Conv::st(fh[I]);                                                     \
+    }
+    STRH_TPUSH(1) STRH_TPUSH(2) STRH_TPUSH(3) STRH_TPUSH(4) STRH_TPUSH(5) STRH_TPUSH(6)
+    STRH_TPUSH(7) STRH_TPUSH(8) STRH_TPUSH(9) STRH_TPUSH(10) STRH_TPUSH(11) STRH_TPUSH(12)
+    STRH_TPUSH(13) STRH_TPUSH(14) STRH_TPUSH(15) STRH_TPUSH(16) STRH_TPUSH(17) STRH_TPUSH(18)
+#undef STRH_TPUSH
+    if (own && p.writeForce != 0 && (sM != 0u || eps > 0.0f)) sd.force[n] = make_float4(ax, ay, az, 0.0f);
+}
+
+// Step J of a tile launch: the cells at distance >= J from the tile edge,
+// row-major over that region, BS at a time.
+template <class Conv, int TW, int TH, int K, int BS, int J>
+__device__ __forceinline__ void tile_step(typename Conv::T* __restrict__ lds, const BatchLat& B,
+                                          const TileSide& sd, int64_t t0, int x0, int y0) {
+    constexpr int C = TW * TH;
+    constexpr int RW = TW - 2 * J, RH = TH - 2 * J;
+    StepParams p = B.p;
+    batch_params(p, B, t0 + J);
+    const int nx = p.nx, ny = p.ny;
+#pragma unroll 1
+    for (int c = threadIdx.x; c < RW * RH; c += BS) {
+        const int yl = c / RW + J, xl = c % RW + J;
+        const int cl = yl * TW + xl;
+        int gx = x0 + xl, gy = y0 + yl;
+        // a patch cell (distance >= K) inside the lattice: its force sample is this tile's
+        const bool own = xl >= K && xl < TW - K && yl >= K && yl < TH - K && gx < nx && gy < ny;
+        gx += gx < 0 ? nx : 0;
+        gx -= gx >= nx ? nx : 0;
+        gy += gy < 0 ? ny : 0;
+        gy -= gy >= ny ? ny : 0;
+        const uint32_t n = (uint32_t)(gy * nx + gx);
+        if ((J & 1) == 0) tile_even<Conv, C>(lds, sd, p, n, cl, gx, own);
+        else tile_odd<Conv, TW, C>(lds, sd, p, n, cl, gx, gy, own);
+    }
+    __syncthreads();
+    if constexpr (J + 1 < K) tile_step<Conv, TW, TH, K, BS, J + 1>(lds, B, sd, t0, x0, y0);
+}
+
+// The host launches tiles only for whole K-step blocks inside a batch segment
+// (every active lattice at an even step, K even, no LES, no body force,
+// nz = 1, nx >= TW, ny >= TH, probe periods >= K), and reads the state from
+// `f` after an even number of launches.
+template <class Conv, int TW, int TH, int K, int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4)))
+step_tiles(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k, int swap) {
+    using T = typename Conv::T;
+    constexpr int C = TW * TH;
+    constexpr int IW = TW - 2 * K, IH = TH - 2 * K;
+    static_assert(K % 2 == 0 && IW > 0 && IH > 0, "tile shape");
+    __shared__ T lds[19 * C];
+    const uint32_t b = blockIdx.x;
+    const BatchLat& B = lats[batch_find_tile(lats, L, b)];
+    const int64_t t0 = B.step0 + *base + k;
+    if (t0 >= B.tEnd) return;
+    const uint32_t tb = b - B.tile0;
+    const uint32_t ty = tb / B.tilesX, tx = tb - ty * B.tilesX;
+    const int x0 = (int)(tx * IW) - K, y0 = (int)(ty * IH) - K;
+    const int nx = B.p.nx, ny = B.p.ny;
+    const DdfLayout lay = B.p.lay;
+    const TileSide sd{(const GLOBAL uint8_t*)B.flags, (const GLOBAL uint32_t*)B.link,
+                      (const GLOBAL uint32_t*)B.lidm, (const GLOBAL float*)B.eps, B.force};
+    const GLOBAL T* __restrict__ src = (const GLOBAL T*)(swap ? B.f2 : B.f);
+    GLOBAL T* __restrict__ dst = (GLOBAL T*)(swap ? B.f : B.f2);
+#pragma unroll 1
+    for (int c = threadIdx.x; c < C; c += BS) {
+        const int yl = c / TW, xl = c % TW;
+        int gx = x0 + xl, gy = y0 + yl;
+        gx += gx < 0 ? nx : 0;
+        gx -= gx >= nx ? nx : 0;
+        gy += gy < 0 ? ny : 0;
+        gy -= gy >= ny ? ny : 0;
+        const uint32_t e = cell_elem(lay, (uint32_t)(gy * nx + gx));
+        T v[19];
+#pragma unroll
+        for (int i = 0; i < 19; i++) v[i] = src[e + (uint32_t)i * lay.W];
+#pragma unroll
+        for (int i = 0; i < 19; i++) lds[i * C + c] = v[i];
+    }
+    __syncthreads();
+    tile_step<Conv, TW, TH, K, BS, 0>(lds, B, sd, t0, x0, y0);
+#pragma unroll 1
+    for (int c = threadIdx.x; c < IW * IH; c += BS) {
+        const int yi = c / IW, xi = c % IW;
+        const int gx = (int)(tx * IW) + xi, gy = (int)(ty * IH) + yi;
+        if (gx >= nx || gy >= ny) continue;
+        const int cl = (yi + K) * TW + xi + K;
+        const uint32_t e = cell_elem(lay, (uint32_t)(gy * nx + gx));
+        T v[19];
+#pragma unroll
+        for (int i = 0; i < 19; i++) v[i] = lds[i * C + cl];
+#pragma unroll
+        for (int i = 0; i < 19; i++) dst[e + (uint32_t)i * lay.W] = v[i];
+    }
 }
 
 __global__ void advance_base(int64_t* base, int64_t by) {
@@ -1942,8 +2245,41 @@ hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool f
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, hipStream_t s) {
-    hipLaunchKernelGGL(probe_batch, dim3(3 * L), dim3(256), 0, s, lats, L, base, k);
+hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s) {
+    hipLaunchKernelGGL(probe_batch, dim3(3 * L), dim3(256), 0, s, lats, L, base, k, span);
+    return hipGetLastError();
+}
+
+// Tiles of 76 KiB of LDS (two workgroups per CU): 32 x 32 cells in FP32,
+// 64 x 32 in the 16-bit formats.
+constexpr int kTileBS = 512;
+
+void batch_tile_shape(int precision, int mode, int* tw, int* th, int* K) {
+    *tw = precision == kFP32 ? 32 : 64;
+    *th = 32;
+    *K = mode == 2 ? 2 : 4;
+}
+
+template <class Conv, int TW>
+static void launch_tiles_t(int mode, const BatchLat* lats, int L, uint32_t tileBlocks, const int64_t* base,
+                           int k, int swap, hipStream_t s) {
+    if (mode == 2)
+        hipLaunchKernelGGL((step_tiles<Conv, TW, 32, 2, kTileBS>), dim3(tileBlocks), dim3(kTileBS), 0, s,
+                           lats, L, base, k, swap);
+    else
+        hipLaunchKernelGGL((step_tiles<Conv, TW, 32, 4, kTileBS>), dim3(tileBlocks), dim3(kTileBS), 0, s,
+                           lats, L, base, k, swap);
+}
+
+hipError_t launch_batch_tiles(int precision, int mode, const BatchLat* lats, int L, uint32_t tileBlocks,
+                              const int64_t* base, int k, int swap, hipStream_t s) {
+    if (mode != 1 && mode != 2) return hipErrorInvalidValue;
+    switch (precision) {
+        case kFP32: launch_tiles_t<ConvF32, 32>(mode, lats, L, tileBlocks, base, k, swap, s); break;
+        case kFP16S: launch_tiles_t<ConvF16, 64>(mode, lats, L, tileBlocks, base, k, swap, s); break;
+        case kBF16: launch_tiles_t<ConvBF16, 64>(mode, lats, L, tileBlocks, base, k, swap, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

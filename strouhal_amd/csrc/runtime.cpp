@@ -834,8 +834,11 @@ struct strh_batch {
     std::vector<int64_t> rampLen;
     std::vector<double*> series;         // per lattice, device
     std::vector<int64_t> taken;          // samples taken so far
+    std::vector<void*> shadow;           // per lattice, second image of the tiled launches (device)
     size_t devCap = 0;                   // descriptors `dev` can hold
     int bs = 256;
+    int tiles = 0;                       // 0: one launch per step; 1 / 2: tiled launches of 4 / 2 steps
+    int64_t tiledSteps = 0;              // lock-step steps run by tiled launches (strh_batch_run)
     int graphSteps = 64;
     double gpuSeconds = 0.0;
 };
@@ -883,18 +886,46 @@ struct Segment {
     int parity0;             // parity of every active lattice at k0, or -1 if they differ
     bool les;                // some active lattice has a Smagorinsky constant
     bool forced;             // some active lattice has a body force
+    int tileK;               // steps per tiled launch, or 0: every step is its own launch
+    uint32_t tileBlocks;     // workgroups of a tiled launch
 };
 
-// Launches steps k = 0 .. count-1 from base; parity0 is the common parity at
-// k = 0 (or -1), so step k runs the pass (parity0 + k) & 1 only.
-int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int parity0) {
+// Whether lattice i can run the tiled launches of `mode` (step_tiles): one
+// cell deep, at least one tile across, no LES, no body force, and at most one
+// force probe per launch.
+bool batch_tile_ok(const strh_batch* B, size_t i, const BatchLat& d, int mode) {
+    const strh_lattice* L = B->lats[i];
+    int tw, th, K;
+    batch_tile_shape(B->precision, mode, &tw, &th, &K);
+    return L->nz == 1 && L->nx >= tw && L->ny >= th && !(L->cSmago > 0.0f) && d.hasForce == 0 &&
+           (d.fCount <= 0 || d.fPeriod >= K);
+}
+
+// Launches steps k = kOff .. kOff+count-1 from base; parity0 is the common
+// parity at k = kOff (or -1), so step k runs the pass (parity0 + k - kOff) & 1 only.
+int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int parity0, int kOff = 0) {
     hipStream_t s = B->ctx->stream;
-    for (int k = 0; k < count; k++) {
-        const int par = parity0 < 0 ? -1 : (parity0 + k) & 1;
+    for (int j = 0; j < count; j++) {
+        const int k = kOff + j;
+        const int par = parity0 < 0 ? -1 : (parity0 + j) & 1;
         HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
         // probe steps are odd (strh_batch_set_probe): an even pass needs no probe
-        if (par != 0) HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, s), "batch probe launch");
+        if (par != 0) HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, 1, s), "batch probe launch");
+    }
+    return STRH_OK;
+}
+
+// Launches 2 * pairs tiled launches of sg.tileK steps from k = 0 (every
+// lattice at an even step there), reading f and writing the second image,
+// then back: the state is in f again afterwards.
+int batch_launch_tiles(strh_batch* B, const Segment& sg, int pairs) {
+    hipStream_t s = B->ctx->stream;
+    for (int m = 0; m < 2 * pairs; m++) {
+        const int k = m * sg.tileK;
+        HIP_TRY(launch_batch_tiles(B->precision, B->tiles, sg.dev, sg.L, sg.tileBlocks, B->base, k, m & 1, s),
+                "batch tile launch");
+        HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, sg.tileK, s), "batch probe launch");
     }
     return STRH_OK;
 }
@@ -905,7 +936,8 @@ int batch_capture(strh_batch* B, const Segment& sg, hipGraphExec_t* out) {
     // replays advance base by graphSteps: fixed parities need an even length
     const int par0 = (B->graphSteps % 2 == 0) ? sg.parity0 : -1;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-    int rc = batch_launch_steps(B, sg, B->bs, B->graphSteps, par0);
+    int rc = sg.tileK > 0 ? batch_launch_tiles(B, sg, B->graphSteps / (2 * sg.tileK))
+                          : batch_launch_steps(B, sg, B->bs, B->graphSteps, par0);
     hipError_t e2 = launch_advance_base(B->base, B->graphSteps, s);
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
@@ -921,8 +953,12 @@ int batch_capture(strh_batch* B, const Segment& sg, hipGraphExec_t* out) {
 // finish: each segment launches only the still-active lattices (compacted
 // descriptors and grid), as whole captured graphs of graphSteps steps plus
 // single launches for the remainder. Kernels read t = step0 + base + k, base
-// being set at each segment start and advanced by every graph replay.
-int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probes, double* seconds) {
+// being set at each segment start and advanced by every graph replay. A
+// segment whose lattices all take tiled launches runs one single step first
+// if they are at an odd step, then pairs of tiled launches (in the graphs when
+// graphSteps is a multiple of a pair), then single steps for what is left.
+int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probes, double* seconds,
+                  int64_t* tiled = nullptr) {
     hipStream_t s = B->ctx->stream;
     const int n = (int)B->lats.size();
     std::vector<int64_t> ends;
@@ -940,6 +976,10 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         sg.parity0 = -2;
         sg.les = false;
         sg.forced = false;
+        sg.tileK = 0;
+        sg.tileBlocks = 0;
+        bool tiled = B->tiles > 0;
+        const size_t first = up.size();
         for (int i = 0; i < n; i++) {
             if (steps[i] <= k0) continue;
             BatchLat d = B->host[i];
@@ -950,9 +990,30 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             d.block0 = sg.blocks;
             d.nblocks = (B->lats[i]->N + (uint32_t)B->bs - 1) / (uint32_t)B->bs;
             if (!probes) d.fCount = 0;
+            tiled = tiled && batch_tile_ok(B, (size_t)i, d, B->tiles);
             sg.blocks += d.nblocks;
             up.push_back(d);
             sg.L++;
+        }
+        if (tiled && sg.parity0 >= 0) {
+            int tw, th, K;
+            batch_tile_shape(B->precision, B->tiles, &tw, &th, &K);
+            const int64_t lead = sg.parity0;   // one single step to reach an even step
+            if (e - k0 - lead >= 2 * K) {
+                sg.tileK = K;
+                size_t j = first;
+                for (int i = 0; i < n; i++) {
+                    if (steps[i] <= k0) continue;
+                    strh_lattice* L = B->lats[i];
+                    if (!B->shadow[i]) HIP_TRY(hipMalloc(&B->shadow[i], ddf_bytes(L)), "hipMalloc(tile image)");
+                    BatchLat& d = up[j++];
+                    d.f2 = B->shadow[i];
+                    d.tilesX = (uint32_t)((L->nx + tw - 2 * K - 1) / (tw - 2 * K));
+                    const uint32_t tilesY = (uint32_t)((L->ny + th - 2 * K - 1) / (th - 2 * K));
+                    d.tile0 = sg.tileBlocks;
+                    sg.tileBlocks += d.tilesX * tilesY;
+                }
+            }
         }
         segs.push_back(sg);
         k0 = e;
@@ -975,6 +1036,36 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         HIP_TRY(launch_set_base(B->base, sg.k0, s), "set base");
         const int64_t K = sg.k1 - sg.k0;
         int64_t done = 0;
+        if (sg.tileK > 0) {
+            if (sg.parity0 == 1) {
+                int rc = batch_launch_steps(B, sg, B->bs, 1, 1);
+                if (rc) return rc;
+                HIP_TRY(launch_advance_base(B->base, 1, s), "advance base");
+                done = 1;
+            }
+            const int pair = 2 * sg.tileK;
+            if (useGraph && B->graphSteps > 0 && B->graphSteps % pair == 0 && K - done >= B->graphSteps) {
+                hipGraphExec_t g = nullptr;
+                int rc = batch_capture(B, sg, &g);
+                if (rc) return rc;
+                while (K - done >= B->graphSteps) {
+                    hipError_t e = hipGraphLaunch(g, s);
+                    if (e != hipSuccess) { hipGraphExecDestroy(g); return hip_fail(e, "graph launch"); }
+                    done += B->graphSteps;
+                    if (tiled) *tiled += B->graphSteps;
+                }
+                hipError_t e = hipStreamSynchronize(s);
+                hipGraphExecDestroy(g);
+                if (e != hipSuccess) return hip_fail(e, "graph replays");
+            }
+            const int pairs = (int)((K - done) / pair);
+            if (tiled) *tiled += (int64_t)pairs * pair;
+            int rc = batch_launch_tiles(B, sg, pairs);
+            if (rc) return rc;
+            const int rest = (int)(K - done - (int64_t)pairs * pair);
+            if (rest > 0 && (rc = batch_launch_steps(B, sg, B->bs, rest, 0, pairs * pair))) return rc;
+            continue;
+        }
         if (useGraph && B->graphSteps > 0 && K >= B->graphSteps) {
             hipGraphExec_t g = nullptr;
             int rc = batch_capture(B, sg, &g);
@@ -1030,6 +1121,7 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
     B->rampLen.assign(count, 0);
     B->series.assign(count, nullptr);
     B->taken.assign(count, 0);
+    B->shadow.assign(count, nullptr);
     auto bail = [&](hipError_t e, const char* what) {
         strh_batch_destroy(B);
         return hip_fail(e, what);
@@ -1054,6 +1146,7 @@ int strh_batch_destroy(strh_batch* B) {
     hipSetDevice(B->ctx->device);
     for (float* r : B->ramp) if (r) hipFree(r);
     for (double* d : B->series) if (d) hipFree(d);
+    for (void* p : B->shadow) if (p) hipFree(p);
     if (B->dev) hipFree(B->dev);
     if (B->base) hipFree(B->base);
     delete B;
@@ -1103,7 +1196,7 @@ int strh_batch_run(strh_batch* B, const int64_t* steps) {
     if (K == 0) return STRH_OK;
     if ((rc = batch_prepare(B, steps))) return rc;
     double sec = 0.0;
-    if ((rc = batch_execute(B, steps, true, true, &sec))) return rc;
+    if ((rc = batch_execute(B, steps, true, true, &sec, &B->tiledSteps))) return rc;
     B->gpuSeconds += sec;
     for (size_t i = 0; i < B->lats.size(); i++) {
         strh_lattice* L = B->lats[i];
@@ -1137,6 +1230,19 @@ int strh_batch_set_block_size(strh_batch* B, int bs) {
     return STRH_OK;
 }
 
+int strh_batch_set_tiles(strh_batch* B, int mode) {
+    if (!B || mode < 0 || mode > 2) return fail(STRH_ERR_INVALID, "tile mode must be 0, 1 or 2");
+    B->tiles = mode;
+    return STRH_OK;
+}
+
+int strh_batch_tiles(strh_batch* B, int* mode, int64_t* tiled_steps) {
+    if (!B || !mode) return fail(STRH_ERR_INVALID, "null argument");
+    *mode = B->tiles;
+    if (tiled_steps) *tiled_steps = B->tiledSteps;
+    return STRH_OK;
+}
+
 int strh_batch_set_graph_steps(strh_batch* B, int steps) {
     if (!B || steps < 0 || steps > 4096) return fail(STRH_ERR_INVALID, "graph steps must be in [0, 4096]");
     B->graphSteps = steps;
@@ -1149,9 +1255,11 @@ int strh_batch_gpu_seconds(strh_batch* B, double* seconds) {
     return STRH_OK;
 }
 
-// Times 16 steps of the batch at each workgroup size, without probes, and
+// Times 16 steps of the batch at each workgroup size with one launch per
+// step, then each tiled mode at the fastest size, without probes, and
 // restores every lattice's DDFs afterwards (the state is copied aside on the
-// device). Keeps the fastest; ties go to the smaller size.
+// device). Keeps the fastest; ties go to the smaller size and to fewer steps
+// per launch.
 int strh_batch_autotune(strh_batch* B, int* chosen) {
     if (!B) return fail(STRH_ERR_INVALID, "null batch");
     int rc = set_device(B->ctx);
@@ -1165,24 +1273,38 @@ int strh_batch_autotune(strh_batch* B, int* chosen) {
             release(); return fail(STRH_ERR_DEVICE, "autotune copy");
         }
     }
-    const int keep = B->bs;
+    const int keep = B->bs, keepTiles = B->tiles;
     std::vector<int64_t> steps(B->lats.size(), 16);
+    std::vector<int64_t> warm(B->lats.size(), 8);
     double best = 1e30;
-    int bestBs = keep;
+    int bestBs = keep, bestTiles = 0;
+    auto time_it = [&](double* sec) {
+        int r = batch_prepare(B, warm.data());
+        if (!r) r = batch_execute(B, warm.data(), false, false, sec);
+        if (!r) r = batch_prepare(B, steps.data());
+        if (!r) r = batch_execute(B, steps.data(), false, false, sec);
+        return r;
+    };
+    B->tiles = 0;
     for (int bs : {128, 256, 512}) {
         B->bs = bs;
-        if ((rc = batch_prepare(B, steps.data()))) break;
         double sec = 0.0;
-        std::vector<int64_t> warm(B->lats.size(), 4);
-        if ((rc = batch_execute(B, warm.data(), false, false, &sec))) break;
-        if ((rc = batch_execute(B, steps.data(), false, false, &sec))) break;
+        if ((rc = time_it(&sec))) break;
         if (sec < best * 0.98) { best = sec; bestBs = bs; }
+    }
+    B->bs = bestBs;
+    for (int mode = 2; mode >= 1 && !rc; mode--) {
+        B->tiles = mode;
+        double sec = 0.0;
+        if ((rc = time_it(&sec))) break;
+        if (sec < best * 0.98) { best = sec; bestTiles = mode; }
     }
     for (size_t i = 0; i < B->lats.size(); i++)
         hipMemcpy(B->lats[i]->f, saved[i], ddf_bytes(B->lats[i]), hipMemcpyDeviceToDevice);
     release();
-    if (rc) { B->bs = keep; return rc; }
+    if (rc) { B->bs = keep; B->tiles = keepTiles; return rc; }
     B->bs = bestBs;
+    B->tiles = bestTiles;
     if (chosen) *chosen = bestBs;
     return STRH_OK;
 }

@@ -146,12 +146,15 @@ def run_sequential(variants):
     return series, time.perf_counter() - t0
 
 
-def run_batched(variants, autotune=True, graph_steps=64):
+def run_batched(variants, autotune=True, graph_steps=64, tiles=None):
+    """tiles: None keeps the batch's (or the autotuner's) tile mode, else 0 / 1 / 2."""
     b = Batch([v.case.sim for v in variants])
     for i, v in enumerate(variants):
         first, period, count = probe_schedule(v)
         b.set_probe(i, first, period, count, v.case.box_x, v.case.box_y)
     bs = b.autotune() if autotune else 256
+    if tiles is not None:
+        b.set_tiles(tiles)
     b.set_graph_steps(graph_steps)
     t0 = time.perf_counter()
     b.run([v.total_steps for v in variants])
@@ -170,6 +173,8 @@ def main():
     ap.add_argument("--mach", type=float, default=MA_DEFAULT)
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--csv", default="", help="directory for one step,C_D,C_L file per rung")
+    ap.add_argument("--tiles", type=int, default=-1,
+                    help="batch tile mode 0 / 1 / 2 (default: the autotuner's choice)")
     args = ap.parse_args()
     res = [int(r) for r in args.resolutions.split(",")]
     base_umax = 1.5 * args.mach / math.sqrt(3.0)      # Ma of U_mean = umax * 2/3 * sqrt(3)
@@ -179,9 +184,10 @@ def main():
     vb = build(gpu, args.precision, res, args.scale, base_umax)
     cells = sum(v.case.sim.cells for v in vb)
     updates = sum(v.case.sim.cells * v.total_steps for v in vb)
-    sb, wb, bs, _ = run_batched(vb)
+    sb, wb, bs, bt = run_batched(vb, tiles=None if args.tiles < 0 else args.tiles)
     report.update({"lattices": len(vb), "cells": cells, "cell_updates": updates,
-                   "batched_s": wb, "batched_mlups": updates / wb / 1e6, "block_size": bs})
+                   "batched_s": wb, "batched_mlups": updates / wb / 1e6, "block_size": bs,
+                   "tile_mode": bt.tiles, "tiled_steps": bt.tiled_steps})
     rungs = []
     for v, s in zip(vb, sb):
         cd, cl, t = coefficients(v, s)

@@ -78,3 +78,81 @@ def test_autotune_keeps_state(gpu):
     assert bs in (128, 256, 512)
     assert [s.state_digest for s in sims] == before
     assert all(s.steps_done == 0 for s in sims)
+
+
+def _planar(gpu, precision):
+    """Lattices one cell deep of each kind the tiled launches cover: periodic
+    wall-free, a lid-driven cavity (moving wall, ramp) and the vortex street
+    (partial cells, ramped parabolic inflow, outflow, sponge, force probes).
+    Sizes are not multiples of the tile patches."""
+    from strouhal_amd import Simulation
+    from strouhal_amd.cases import cavity, vortex_street
+    a = Simulation(gpu, precision, nx=100, ny=40, nz=1, omega=1.7)
+    a.init_field(1, 0.05)
+    b = cavity(gpu, precision, n=66, re=1e3, ramp_steps=30)
+    street = vortex_street(gpu, precision, D=8, uin_max=0.1)
+    return [a, b, street.sim], (street.box_x, street.box_y)
+
+
+def _run_planar(gpu, precision, mode, graph):
+    sims, box = _planar(gpu, precision)
+    bt = Batch(sims)
+    bt.set_tiles(mode)
+    bt.set_graph_steps(graph)
+    bt.set_probe(2, 7, 6, 12, *box)      # odd steps 7, 13, ..., 73
+    bt.run([36, 50, 44])      # all even at the start; segments of 36, 8 and 6 steps
+    bt.run([3, 0, 17])        # the street alone from an odd step: one single step first
+    bt.run([22, 9, 0])        # mixed parities (single steps), then one lattice even
+    bt.run([40, 24, 16])      # all odd: one single step, then tiles
+    return ([s.state_digest for s in sims], [s.steps_done for s in sims], bt.probe_series(2),
+            bt.tiled_steps)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16s", "bf16"])
+@pytest.mark.parametrize("mode,graph", [(1, 16), (2, 16), (1, 0), (2, 12), (1, 64)])
+def test_tiled_batch_matches_one_launch_per_step(gpu, precision, mode, graph):
+    d0, n0, s0, t0 = _run_planar(gpu, precision, 0, graph)
+    d1, n1, s1, t1 = _run_planar(gpu, precision, mode, graph)
+    assert t0 == 0
+    assert t1 >= 64, "the tiled launches did not run"
+    assert n1 == n0
+    assert d1 == d0
+    assert s1.shape == s0.shape == (12, 3)
+    assert s0[-1, 0] != 0.0          # the ramped flow has reached the cylinder
+    assert np.array_equal(s1, s0)
+
+
+def test_tiled_batch_matches_solo_runs(gpu):
+    sims, _ = _planar(gpu, "fp32")
+    solo, _ = _planar(gpu, "fp32")
+    bt = Batch(sims)
+    bt.set_tiles(1)
+    bt.run([40, 56, 72])
+    assert bt.tiled_steps >= 56
+    for s, k in zip(solo, [40, 56, 72]):
+        s.run(k)
+    for x, y in zip(sims, solo):
+        assert x.state_digest == y.state_digest
+
+
+def test_tiles_skip_lattices_they_do_not_cover(gpu):
+    # nz > 1 in the batch: every segment runs one launch per step
+    sims = _mixed(gpu, "fp32")
+    bt = Batch(sims)
+    bt.set_tiles(1)
+    bt.run([16, 16, 16])
+    assert bt.tiled_steps == 0
+
+
+def test_autotune_over_tile_modes_keeps_state(gpu):
+    sims, _ = _planar(gpu, "fp16s")
+    solo, _ = _planar(gpu, "fp16s")
+    bt = Batch(sims)
+    before = [s.state_digest for s in sims]
+    bs = bt.autotune()
+    assert bs in (128, 256, 512) and bt.tiles in (0, 1, 2)
+    assert [s.state_digest for s in sims] == before
+    bt.run([24, 24, 24])
+    for s in solo:
+        s.run(24)
+    assert [s.state_digest for s in sims] == [s.state_digest for s in solo]
