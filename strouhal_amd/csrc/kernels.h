@@ -38,8 +38,9 @@ struct BatchLat {
     const uint8_t* flags;
     const uint32_t* link;
     const uint32_t* lidm;
-    float4* force;
+    float4* force;           // force buffers: a ring of fRing buffers of N cells when probed
     const float* eps;
+    int32_t fRing;           // force buffers in the ring (sample j writes buffer j % fRing)
     uint32_t block0;         // first workgroup of this lattice in the batch grid
     uint32_t nblocks;
     void* f2;                // second DDF image of the tiled launches (same layout), or null
@@ -117,7 +118,8 @@ hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uin
 // lattice has a Smagorinsky constant; forced: some lattice has a body force.
 hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, const BatchLat* lats,
                              int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
-// Probes the force step among steps k .. k + span - 1 (at most one per lattice).
+// Probes every force step among steps k .. k + span - 1 (span <= kProbeSpan).
+constexpr int kProbeSpan = 64;
 hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s);
 // Tiled launch of K steps (k .. k + K - 1) for lattices one cell deep; mode 1: K = 4,
 // mode 2: K = 2. swap 0 reads f and writes f2, swap 1 the reverse.

@@ -1593,6 +1593,14 @@ __device__ __forceinline__ bool is_force_step(const BatchLat& B, int64_t t) {
     return d % B.fPeriod == 0 && d / B.fPeriod < B.fCount;
 }
 
+// The force buffer sample j = (t - fT0) / fPeriod is written to: buffer
+// j % fRing of the ring, so the samples of up to kProbeSpan steps are reduced
+// by one probe launch.
+__device__ __forceinline__ float4* force_slot(const BatchLat& B, int64_t t) {
+    if (B.fCount <= 0 || t < B.fT0) return B.force;
+    return B.force + (size_t)(((t - B.fT0) / B.fPeriod) % B.fRing) * B.p.N;
+}
+
 // PAR = 0 / 1: the host guarantees that every active lattice is at that
 // parity in this launch (equal start parities, even graph lengths), so only
 // that pass is compiled in; PAR = -1 branches per lattice. LES = false when no
@@ -1616,22 +1624,23 @@ step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
     p.ulidY = B.lid[1] * r;
     p.ulidZ = B.lid[2] * r;
     p.writeForce = is_force_step(B, t) ? 1 : 0;
+    float4* fo = p.writeForce ? force_slot(B, t) : B.force;
     using T = typename Conv::T;
     T* f = (T*)B.f;
     if (PAR != 1 && p.parity == 0) {
         using St = Store<Conv, kPolAligned>;
-        if (FORCED && B.hasForce) step_cell<St, 0, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
-        else step_cell<St, 0, true, true, LES, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        if (FORCED && B.hasForce) step_cell<St, 0, true, true, LES, true>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
+        else step_cell<St, 0, true, true, LES, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
     } else if (PAR != 0) {
         using St = Store<Conv, kPolShifted>;
-        if (FORCED && B.hasForce) step_cell<St, 1, true, true, LES, true>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
-        else step_cell<St, 1, true, true, LES, false>(f, B.flags, B.link, B.lidm, B.force, B.eps, p, n);
+        if (FORCED && B.hasForce) step_cell<St, 1, true, true, LES, true>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
+        else step_cell<St, 1, true, true, LES, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
     }
 }
 
-// After batch steps k .. k + span - 1: for every lattice with a force step t
-// among them (probe periods are at least span, so at most one), sum the
-// per-cell momentum exchange over its probe box into its time series. Block
+// After batch steps k .. k + span - 1 (span <= kProbeSpan): for every lattice
+// and every force step t among them, sum the per-cell momentum exchange of
+// that step's ring buffer over the probe box into the time series. Block
 // (l, c) sums component c of lattice l with exactly the chunking of the
 // single-lattice probe (reduce_stage1 over Pb chunks, then reduce_stage2), so
 // the FP64 values are the same bits.
@@ -1642,24 +1651,31 @@ probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base
     const int l = blockIdx.x / 3, c = blockIdx.x % 3;
     if (l >= L) return;
     const BatchLat& B = lats[l];
-    int64_t t = B.step0 + *base + k;
-    const int64_t tLast = t + span;
-    while (t < tLast && t < B.tEnd && !is_force_step(B, t)) t++;
-    if (t >= tLast || t >= B.tEnd) return;
-    const BoxForceElem e{B.force, (uint32_t)B.p.nx, (uint32_t)B.p.ny, B.bx0, B.by0, B.fbw, B.fbh, c};
-    for (uint32_t pb = 0; pb < B.Pb; pb++) {
-        const uint32_t lo = pb * B.chunk;
-        const uint32_t hi = (B.M - lo) > B.chunk ? lo + B.chunk : B.M;
+    if (B.fCount <= 0) return;
+    const int64_t t0 = B.step0 + *base + k;
+    int64_t tLast = t0 + span;
+    const int64_t tCap = B.fT0 + B.fCount * B.fPeriod;        // one past the last sample's step
+    tLast = tLast < B.tEnd ? tLast : B.tEnd;
+    tLast = tLast < tCap ? tLast : tCap;
+    // first sample step at or after t0
+    int64_t t = t0 <= B.fT0 ? B.fT0 : B.fT0 + ((t0 - B.fT0 + B.fPeriod - 1) / B.fPeriod) * B.fPeriod;
+    for (; t < tLast; t += B.fPeriod) {
+        const BoxForceElem e{force_slot(B, t), (uint32_t)B.p.nx, (uint32_t)B.p.ny, B.bx0, B.by0, B.fbw, B.fbh, c};
+        for (uint32_t pb = 0; pb < B.Pb; pb++) {
+            const uint32_t lo = pb * B.chunk;
+            const uint32_t hi = (B.M - lo) > B.chunk ? lo + B.chunk : B.M;
+            double acc = 0.0;
+            for (uint32_t q = lo + threadIdx.x; q < hi; q += 256u) acc += e(0, q);
+            const double sum = block_tree_sum(acc, lds);
+            if (threadIdx.x == 0) part[pb] = sum;
+            __syncthreads();
+        }
         double acc = 0.0;
-        for (uint32_t q = lo + threadIdx.x; q < hi; q += 256u) acc += e(0, q);
-        const double sum = block_tree_sum(acc, lds);
-        if (threadIdx.x == 0) part[pb] = sum;
+        for (uint32_t q = threadIdx.x; q < B.Pb; q += 256u) acc += part[q];
+        const double total = block_tree_sum(acc, lds);
+        if (threadIdx.x == 0) B.series[3 * ((t - B.fT0) / B.fPeriod) + c] = total;
         __syncthreads();
     }
-    double acc = 0.0;
-    for (uint32_t q = threadIdx.x; q < B.Pb; q += 256u) acc += part[q];
-    const double total = block_tree_sum(acc, lds);
-    if (threadIdx.x == 0) B.series[3 * ((t - B.fT0) / B.fPeriod) + c] = total;
 }
 
 // ---- temporally blocked batch step for lattices one cell deep ------------
@@ -1700,14 +1716,14 @@ struct TileSide {
     const GLOBAL uint32_t* link;
     const GLOBAL uint32_t* lidm;
     const GLOBAL float* eps;
-    float4* force;           // written on probe steps only
 };
 
 // The even pass of one tile cell (step_cell, PARITY 0): own slots in, swapped
 // slots out.
 template <class Conv, int C>
 __device__ __forceinline__ void tile_even(typename Conv::T* __restrict__ lds, const TileSide& sd,
-                                          const StepParams& p, uint32_t n, int cl, int gx, bool own) {
+                                          float4* fo, const StepParams& p, uint32_t n, int cl, int gx,
+                                          bool own) {
     if (sd.link[n] & kNotFluidBit) return;      // bit 31: the cell is not fluid
     const float eps = p.useEps ? sd.eps[n] : 0.0f;
     float fh[19];
@@ -1718,7 +1734,7 @@ __device__ __forceinline__ void tile_even(typename Conv::T* __restrict__ lds, co
     lds[cl] = Conv::st(fh[0]);
 #pragma unroll
     for (int i = 1; i < 19; i++) lds[kOpp[i] * C + cl] = Conv::st(fh[i]);
-    if (own && p.writeForce != 0 && eps > 0.0f) sd.force[n] = make_float4(ntx, nty, ntz, 0.0f);
+    if (own && p.writeForce != 0 && eps > 0.0f) fo[n] = make_float4(ntx, nty, ntz, 0.0f);
 }
 
 // The odd pass of one tile cell (step_cell, PARITY 1): pull from the in-plane
@@ -1727,7 +1743,8 @@ __device__ __forceinline__ void tile_even(typename Conv::T* __restrict__ lds, co
 // Every access stays inside the tile for cells at distance >= 1 from its edge.
 template <class Conv, int TW, int C>
 __device__ __forceinline__ void tile_odd(typename Conv::T* __restrict__ lds, const TileSide& sd,
-                                         const StepParams& p, uint32_t n, int cl, int gx, int gy, bool own) {
+                                         float4* fo, const StepParams& p, uint32_t n, int cl, int gx, int gy,
+                                         bool own) {
     const uint32_t lw = sd.link[n];
     if (lw & kNotFluidBit) return;
     const uint32_t sM = lw & 0x3ffffu;
@@ -1863,7 +1880,7 @@ __device__ __forceinline__ void tile_odd(typename Conv::T* __restrict__ lds, con
     STRH_TPUSH(7) STRH_TPUSH(8) STRH_TPUSH(9) STRH_TPUSH(10) STRH_TPUSH(11) STRH_TPUSH(12)
     STRH_TPUSH(13) STRH_TPUSH(14) STRH_TPUSH(15) STRH_TPUSH(16) STRH_TPUSH(17) STRH_TPUSH(18)
 #undef STRH_TPUSH
-    if (own && p.writeForce != 0 && (sM != 0u || eps > 0.0f)) sd.force[n] = make_float4(ax, ay, az, 0.0f);
+    if (own && p.writeForce != 0 && (sM != 0u || eps > 0.0f)) fo[n] = make_float4(ax, ay, az, 0.0f);
 }
 
 // Step J of a tile launch: the cells at distance >= J from the tile edge,
@@ -1875,6 +1892,7 @@ __device__ __forceinline__ void tile_step(typename Conv::T* __restrict__ lds, co
     constexpr int RW = TW - 2 * J, RH = TH - 2 * J;
     StepParams p = B.p;
     batch_params(p, B, t0 + J);
+    float4* fo = p.writeForce ? force_slot(B, t0 + J) : B.force;
     const int nx = p.nx, ny = p.ny;
 #pragma unroll 1
     for (int c = threadIdx.x; c < RW * RH; c += BS) {
@@ -1888,8 +1906,8 @@ __device__ __forceinline__ void tile_step(typename Conv::T* __restrict__ lds, co
         gy += gy < 0 ? ny : 0;
         gy -= gy >= ny ? ny : 0;
         const uint32_t n = (uint32_t)(gy * nx + gx);
-        if ((J & 1) == 0) tile_even<Conv, C>(lds, sd, p, n, cl, gx, own);
-        else tile_odd<Conv, TW, C>(lds, sd, p, n, cl, gx, gy, own);
+        if ((J & 1) == 0) tile_even<Conv, C>(lds, sd, fo, p, n, cl, gx, own);
+        else tile_odd<Conv, TW, C>(lds, sd, fo, p, n, cl, gx, gy, own);
     }
     __syncthreads();
     if constexpr (J + 1 < K) tile_step<Conv, TW, TH, K, BS, J + 1>(lds, B, sd, t0, x0, y0);
@@ -1917,7 +1935,7 @@ step_tiles(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
     const int nx = B.p.nx, ny = B.p.ny;
     const DdfLayout lay = B.p.lay;
     const TileSide sd{(const GLOBAL uint8_t*)B.flags, (const GLOBAL uint32_t*)B.link,
-                      (const GLOBAL uint32_t*)B.lidm, (const GLOBAL float*)B.eps, B.force};
+                      (const GLOBAL uint32_t*)B.lidm, (const GLOBAL float*)B.eps};
     const GLOBAL T* __restrict__ src = (const GLOBAL T*)(swap ? B.f2 : B.f);
     GLOBAL T* __restrict__ dst = (GLOBAL T*)(swap ? B.f : B.f2);
 #pragma unroll 1

@@ -835,6 +835,8 @@ struct strh_batch {
     std::vector<double*> series;         // per lattice, device
     std::vector<int64_t> taken;          // samples taken so far
     std::vector<void*> shadow;           // per lattice, second image of the tiled launches (device)
+    std::vector<void*> fring;            // per probed lattice, ring of force buffers (device)
+    std::vector<int> fringLen;           // buffers in that ring
     size_t devCap = 0;                   // descriptors `dev` can hold
     int bs = 256;
     int tiles = 0;                       // 0: one launch per step; 1 / 2: tiled launches of 4 / 2 steps
@@ -853,7 +855,9 @@ int batch_prepare(strh_batch* B, const int64_t* steps) {
         BatchLat& d = B->host[i];
         d.p = make_params(L, L->stepsDone, false);
         d.f = L->f; d.flags = L->flags; d.link = L->link; d.lidm = L->lidm;
-        d.force = (float4*)L->force; d.eps = L->eps;
+        d.force = B->fring[i] ? (float4*)B->fring[i] : (float4*)L->force;
+        d.fRing = B->fring[i] ? B->fringLen[i] : 1;
+        d.eps = L->eps;
         d.step0 = L->stepsDone;
         d.tEnd = L->stepsDone + (steps ? steps[i] : 0);
         d.uinTarget = L->uinTarget;
@@ -888,6 +892,7 @@ struct Segment {
     bool forced;             // some active lattice has a body force
     int tileK;               // steps per tiled launch, or 0: every step is its own launch
     uint32_t tileBlocks;     // workgroups of a tiled launch
+    bool probes;             // some active lattice has force samples
 };
 
 // Whether lattice i can run the tiled launches of `mode` (step_tiles): one
@@ -903,29 +908,40 @@ bool batch_tile_ok(const strh_batch* B, size_t i, const BatchLat& d, int mode) {
 
 // Launches steps k = kOff .. kOff+count-1 from base; parity0 is the common
 // parity at k = kOff (or -1), so step k runs the pass (parity0 + k - kOff) & 1 only.
+// The force samples of every kProbeSpan steps (and of the last few) are
+// reduced by one probe launch; each lattice's ring holds that many samples.
 int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int parity0, int kOff = 0) {
     hipStream_t s = B->ctx->stream;
+    int k0 = kOff;
     for (int j = 0; j < count; j++) {
         const int k = kOff + j;
         const int par = parity0 < 0 ? -1 : (parity0 + j) & 1;
         HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
-        // probe steps are odd (strh_batch_set_probe): an even pass needs no probe
-        if (par != 0) HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, 1, s), "batch probe launch");
+        if (sg.probes && (k + 1 - k0 == kProbeSpan || j == count - 1)) {
+            HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k0, k + 1 - k0, s), "batch probe launch");
+            k0 = k + 1;
+        }
     }
     return STRH_OK;
 }
 
 // Launches 2 * pairs tiled launches of sg.tileK steps from k = 0 (every
 // lattice at an even step there), reading f and writing the second image,
-// then back: the state is in f again afterwards.
+// then back: the state is in f again afterwards. Probes as above (kProbeSpan
+// is a multiple of both step counts).
 int batch_launch_tiles(strh_batch* B, const Segment& sg, int pairs) {
     hipStream_t s = B->ctx->stream;
+    int k0 = 0;
     for (int m = 0; m < 2 * pairs; m++) {
         const int k = m * sg.tileK;
         HIP_TRY(launch_batch_tiles(B->precision, B->tiles, sg.dev, sg.L, sg.tileBlocks, B->base, k, m & 1, s),
                 "batch tile launch");
-        HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k, sg.tileK, s), "batch probe launch");
+        const int k1 = k + sg.tileK;
+        if (sg.probes && (k1 - k0 == kProbeSpan || m == 2 * pairs - 1)) {
+            HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k0, k1 - k0, s), "batch probe launch");
+            k0 = k1;
+        }
     }
     return STRH_OK;
 }
@@ -978,6 +994,7 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         sg.forced = false;
         sg.tileK = 0;
         sg.tileBlocks = 0;
+        sg.probes = false;
         bool tiled = B->tiles > 0;
         const size_t first = up.size();
         for (int i = 0; i < n; i++) {
@@ -990,6 +1007,7 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             d.block0 = sg.blocks;
             d.nblocks = (B->lats[i]->N + (uint32_t)B->bs - 1) / (uint32_t)B->bs;
             if (!probes) d.fCount = 0;
+            sg.probes = sg.probes || d.fCount > 0;
             tiled = tiled && batch_tile_ok(B, (size_t)i, d, B->tiles);
             sg.blocks += d.nblocks;
             up.push_back(d);
@@ -1122,6 +1140,8 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
     B->series.assign(count, nullptr);
     B->taken.assign(count, 0);
     B->shadow.assign(count, nullptr);
+    B->fring.assign(count, nullptr);
+    B->fringLen.assign(count, 0);
     auto bail = [&](hipError_t e, const char* what) {
         strh_batch_destroy(B);
         return hip_fail(e, what);
@@ -1147,6 +1167,7 @@ int strh_batch_destroy(strh_batch* B) {
     for (float* r : B->ramp) if (r) hipFree(r);
     for (double* d : B->series) if (d) hipFree(d);
     for (void* p : B->shadow) if (p) hipFree(p);
+    for (void* p : B->fring) if (p) hipFree(p);
     if (B->dev) hipFree(B->dev);
     if (B->base) hipFree(B->base);
     delete B;
@@ -1174,13 +1195,23 @@ int strh_batch_set_probe(strh_batch* B, int index, int64_t first, int64_t period
     d.fT0 = first; d.fPeriod = period; d.fCount = count;
     if (B->series[index]) hipFree(B->series[index]);
     B->series[index] = nullptr;
+    if (B->fring[index]) hipFree(B->fring[index]);
+    B->fring[index] = nullptr;
+    B->fringLen[index] = 0;
     if (count > 0) HIP_TRY(hipMalloc(&B->series[index], (size_t)count * 3 * sizeof(double)), "hipMalloc(series)");
     d.series = B->series[index];
     B->taken[index] = 0;
-    // Cells outside the momentum-exchange set are never written on force
-    // steps; one zero fill here stands for the per-probe clear of Sim.swift:361.
-    HIP_TRY(launch_fill_zero(L->force, (size_t)L->N * 16, B->ctx->stream), "zero force");
-    HIP_TRY(hipStreamSynchronize(B->ctx->stream), "zero force");
+    if (count > 0) {
+        // One force buffer per sample of a probe launch's kProbeSpan steps, plus one.
+        const int ring = (int)((kProbeSpan + period - 1) / period) + 1;
+        const size_t bytes = (size_t)ring * L->N * 16;
+        HIP_TRY(hipMalloc(&B->fring[index], bytes), "hipMalloc(force ring)");
+        B->fringLen[index] = ring;
+        // Cells outside the momentum-exchange set are never written on force
+        // steps; one zero fill here stands for the per-probe clear of Sim.swift:361.
+        HIP_TRY(launch_fill_zero(B->fring[index], bytes, B->ctx->stream), "zero force");
+        HIP_TRY(hipStreamSynchronize(B->ctx->stream), "zero force");
+    }
     return STRH_OK;
 }
 
