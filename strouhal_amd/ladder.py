@@ -163,6 +163,30 @@ def run_batched(variants, autotune=True, graph_steps=64, tiles=None):
     return series, wall, bs, b
 
 
+def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64):
+    """Microseconds per lock-step step of every rung together and of the finest
+    rung alone, for each batch mode (one launch per step, 4- and 2-step tiles),
+    from the rungs' initial state."""
+    out = {}
+    for sel in ("all", "finest"):
+        for mode in (0, 1, 2):
+            sims = [v.case.sim for v in build(gpu, precision, res, 0.01, base_umax)]
+            if sel == "finest":
+                sims = sims[-1:]
+            cells = sum(s.cells for s in sims)
+            b = Batch(sims)
+            b.set_tiles(mode)
+            b.set_graph_steps(graph_steps)
+            b.run([16] * len(sims))
+            g0 = b.gpu_seconds
+            b.run([steps] * len(sims))
+            dt = b.gpu_seconds - g0
+            out[f"{sel}_tiles{mode}"] = {"us_per_step": dt / steps * 1e6, "glups": cells * steps / dt / 1e9,
+                                         "tiled_steps": b.tiled_steps}
+            b.close()
+    return out
+
+
 def main():
     import argparse
     from strouhal_amd import default_gpu
@@ -175,10 +199,20 @@ def main():
     ap.add_argument("--csv", default="", help="directory for one step,C_D,C_L file per rung")
     ap.add_argument("--tiles", type=int, default=-1,
                     help="batch tile mode 0 / 1 / 2 (default: the autotuner's choice)")
+    ap.add_argument("--step-rate", type=int, default=0,
+                    help="instead of the schedule: time this many steps of all rungs and of the "
+                         "finest rung alone in each tile mode")
+    ap.add_argument("--graph-steps", type=int, default=64)
     args = ap.parse_args()
     res = [int(r) for r in args.resolutions.split(",")]
     base_umax = 1.5 * args.mach / math.sqrt(3.0)      # Ma of U_mean = umax * 2/3 * sqrt(3)
     gpu = default_gpu()
+    if args.step_rate > 0:
+        print(json.dumps({"workload": f"DFG 2D-2 street ladder D={res}, {args.step_rate} steps per mode",
+                          "precision": args.precision, "graph_steps": args.graph_steps,
+                          **step_rates(gpu, args.precision, res, base_umax, args.step_rate,
+                                       args.graph_steps)}), flush=True)
+        return
     report = {"workload": f"DFG 2D-2 street ladder D={res} x (Ma {args.mach}, half-Mach), "
                           f"schedule x{args.scale}", "precision": args.precision}
     vb = build(gpu, args.precision, res, args.scale, base_umax)
