@@ -198,9 +198,8 @@ STRH_API int strh_batch_autotune(strh_batch* batch, int* block_size);
 /* Temporal blocking for lattices one cell deep (nz = 1): mode 0 (the default)
  * launches every step; mode 1 / 2 advance 4 / 2 steps per launch in LDS tiles
  * (32 x 32 cells FP32, 64 x 32 FP16 / BF16, halo included) wherever every
- * active lattice has nx and ny of at least a tile, no LES, no body force and
- * probe periods of at least the step count. Results do not depend on the
- * mode. The tiled launches keep a second DDF image per lattice. */
+ * active lattice has nx and ny of at least a tile, no LES and no body force.
+ * Results do not depend on the mode. The tiled launches keep a second DDF image per lattice. */
 STRH_API int strh_batch_set_tiles(strh_batch* batch, int mode);
 /* The mode, and how many of the lock-step steps of strh_batch_run so far
  * were run by tiled launches (may be NULL). */

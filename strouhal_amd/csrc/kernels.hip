@@ -1915,8 +1915,8 @@ __device__ __forceinline__ void tile_step(typename Conv::T* __restrict__ lds, co
 
 // The host launches tiles only for whole K-step blocks inside a batch segment
 // (every active lattice at an even step, K even, no LES, no body force,
-// nz = 1, nx >= TW, ny >= TH, probe periods >= K), and reads the state from
-// `f` after an even number of launches.
+// nz = 1, nx >= TW, ny >= TH), and reads the state from `f` after an even
+// number of launches.
 template <class Conv, int TW, int TH, int K, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4)))
 step_tiles(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k, int swap) {

@@ -896,14 +896,12 @@ struct Segment {
 };
 
 // Whether lattice i can run the tiled launches of `mode` (step_tiles): one
-// cell deep, at least one tile across, no LES, no body force, and at most one
-// force probe per launch.
+// cell deep, at least one tile across, no LES, no body force.
 bool batch_tile_ok(const strh_batch* B, size_t i, const BatchLat& d, int mode) {
     const strh_lattice* L = B->lats[i];
     int tw, th, K;
     batch_tile_shape(B->precision, mode, &tw, &th, &K);
-    return L->nz == 1 && L->nx >= tw && L->ny >= th && !(L->cSmago > 0.0f) && d.hasForce == 0 &&
-           (d.fCount <= 0 || d.fPeriod >= K);
+    return L->nz == 1 && L->nx >= tw && L->ny >= th && !(L->cSmago > 0.0f) && d.hasForce == 0;
 }
 
 // Launches steps k = kOff .. kOff+count-1 from base; parity0 is the common

@@ -156,3 +156,22 @@ def test_autotune_over_tile_modes_keeps_state(gpu):
     for s in solo:
         s.run(24)
     assert [s.state_digest for s in sims] == [s.state_digest for s in solo]
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_tiled_batch_with_several_samples_per_launch(gpu, mode):
+    # probe period 2 < K: one tiled launch takes up to two samples, each in
+    # its own buffer of the force ring
+    def run(m):
+        sims, box = _planar(gpu, "fp32")
+        bt = Batch(sims)
+        bt.set_tiles(m)
+        bt.set_graph_steps(16)
+        bt.set_probe(2, 5, 2, 40, *box)          # odd steps 5, 7, ..., 83
+        bt.run([48, 48, 88])
+        return [s.state_digest for s in sims], bt.probe_series(2), bt.tiled_steps
+    d0, s0, _ = run(0)
+    d1, s1, t1 = run(mode)
+    assert t1 >= 80
+    assert d1 == d0
+    assert s1.shape == (40, 3) and np.array_equal(s1, s0)
