@@ -175,3 +175,26 @@ def test_tiled_batch_with_several_samples_per_launch(gpu, mode):
     assert t1 >= 80
     assert d1 == d0
     assert s1.shape == (40, 3) and np.array_equal(s1, s0)
+
+
+@pytest.mark.parametrize("precision,nx,ny", [("fp32", 32, 32), ("fp32", 33, 45), ("fp16s", 64, 32),
+                                              ("bf16", 70, 33)])
+def test_tiled_batch_on_lattices_of_about_one_tile(gpu, precision, nx, ny):
+    # tiles wrap onto themselves: wrapped duplicates evolve like the cells
+    # they stand for and only the owned patch cells are written back
+    from strouhal_amd import Simulation
+
+    def make():
+        a = Simulation(gpu, precision, nx=nx, ny=ny, nz=1, omega=1.6)
+        a.init_field(1, 0.05)
+        b = Simulation(gpu, precision, nx=nx, ny=ny, nz=1, omega=1.5, lid=(0.05, 0.0, 0.0),
+                       flags=lambda x, y, z: np.where(y == 0, 1, np.where(y == ny - 1, 2, 0)))
+        return [a, b]
+    tiled, solo = make(), make()
+    bt = Batch(tiled)
+    bt.set_tiles(1)
+    bt.run([40, 24])
+    assert bt.tiled_steps >= 24
+    for s, k in zip(solo, [40, 24]):
+        s.run(k)
+    assert [s.state_digest for s in tiled] == [s.state_digest for s in solo]
