@@ -1628,7 +1628,10 @@ step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
     using T = typename Conv::T;
     T* f = (T*)B.f;
     if (PAR != 1 && p.parity == 0) {
-        using St = Store<Conv, kPolAligned>;
+        // Batched lattices are small enough to stay in the last-level cache
+        // between passes: the default policy (measured against nontemporal
+        // loads / stores on the street ladder).
+        using St = Store<Conv, kPolShifted>;
         if (FORCED && B.hasForce) step_cell<St, 0, true, true, LES, true>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
         else step_cell<St, 0, true, true, LES, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
     } else if (PAR != 0) {
