@@ -185,3 +185,39 @@ def test_qoi_csv_format(tmp_path):
     p = tmp_path / "q.csv"
     write_qoi_csv(str(p), [100.0, 202.0], [3.2, 3.25], [-0.1234567, 0.5])
     assert p.read_text() == "step,C_D,C_L\n100,3.200000,-0.123457\n202,3.250000,0.500000\n"
+
+
+def _top_level_args(text: str, start: int) -> int:
+    """Number of comma-separated arguments of the call whose '(' is at start."""
+    depth, n, i, empty = 0, 1, start, True
+    while True:
+        ch = text[i]
+        if ch in "([{":
+            depth += 1
+        elif ch in ")]}":
+            depth -= 1
+            if depth == 0:
+                return 0 if empty else n
+        elif ch == "," and depth == 1:
+            n += 1
+        elif not ch.isspace() and depth >= 1 and i > start:
+            empty = False
+        i += 1
+
+
+def test_swift_facade_calls_only_declared_entry_points():
+    # No Swift toolchain here: check the facade against the header instead
+    # (names and argument counts of every C call it makes).
+    hdr = open(os.path.join(ROOT, "include", "strouhal_hip.h")).read()
+    arity = {}
+    for m in re.finditer(r"STRH_API\s+[\w\s\*]+?\b(strh_\w+)\s*\(", hdr):
+        void = re.match(r"\(\s*void\s*\)", hdr[m.end() - 1:])
+        arity[m.group(1)] = 0 if void else _top_level_args(hdr, m.end() - 1)
+    src = open(os.path.join(ROOT, "swift", "Sources", "StrouhalCore", "Sim.swift")).read()
+    types = set(re.findall(r"\}\s*(strh_\w+)\s*;", hdr)) | set(re.findall(r"typedef\s+struct\s+\w+\s+(strh_\w+)\s*;", hdr))
+    calls = [m for m in re.finditer(r"\b(strh_\w+)\s*\(", src) if m.group(1) not in types]
+    assert len(calls) >= 20
+    for m in calls:
+        name = m.group(1)
+        assert name in arity, name
+        assert _top_level_args(src, m.end() - 1) == arity[name], name
