@@ -198,3 +198,24 @@ def test_tiled_batch_on_lattices_of_about_one_tile(gpu, precision, nx, ny):
     for s, k in zip(solo, [40, 24]):
         s.run(k)
     assert [s.state_digest for s in tiled] == [s.state_digest for s in solo]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_probe_series_do_not_depend_on_how_runs_are_split(gpu, mode):
+    # the force ring is reduced once per 64 steps and at the end of each run:
+    # splitting a run at odd places (spans that start mid-ring, runs that end
+    # mid-span) must give the same samples and states as one long run
+    def run(splits):
+        sims, box = _planar(gpu, "fp32")
+        bt = Batch(sims)
+        bt.set_tiles(mode)
+        bt.set_graph_steps(16)
+        bt.set_probe(2, 3, 4, 40, *box)          # odd steps 3, 7, ..., 159
+        for k in splits:
+            bt.run([k, k, k])
+        return [s.state_digest for s in sims], bt.probe_series(2)
+    d0, s0 = run([160])
+    d1, s1 = run([13, 29, 40, 6, 72])
+    assert d1 == d0
+    assert s0.shape == s1.shape == (40, 3)
+    assert np.array_equal(s1, s0)
