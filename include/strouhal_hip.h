@@ -115,6 +115,11 @@ STRH_API int strh_init_field(strh_lattice* lat, uint32_t mode, float amplitude);
 
 /* ---- stepping: Simulation.run (Sim.swift:288-331) ------------------------ */
 STRH_API int strh_run(strh_lattice* lat, int64_t steps);
+/* strh_run with a HIP event pair around every step launch (no reference
+ * counterpart; a measurement aid): ms[k] receives the duration of step k in
+ * milliseconds (`steps` floats, steps <= 2^20). Advances stepsDone and the
+ * GPU seconds like strh_run. */
+STRH_API int strh_run_timed(strh_lattice* lat, int64_t steps, float* ms);
 STRH_API int strh_steps_done(strh_lattice* lat, int64_t* steps);
 /* Accumulated GPU time of run(), Sim.swift:176-178. */
 STRH_API int strh_gpu_seconds(strh_lattice* lat, double* seconds);
@@ -191,19 +196,10 @@ STRH_API int strh_batch_run(strh_batch* batch, const int64_t* steps);
 STRH_API int strh_batch_probe_series(strh_batch* batch, int index, double* out,
                                      int64_t capacity, int64_t* taken);
 /* Workgroup size of the batch step kernel (128, 256 or 512); results do not
- * depend on it. autotune times each on the batch, then the tiled modes below
- * with the fastest size, and keeps the fastest (state restored after). */
+ * depend on it. autotune times each on the batch and keeps the fastest (state
+ * restored after). */
 STRH_API int strh_batch_set_block_size(strh_batch* batch, int block_size);
 STRH_API int strh_batch_autotune(strh_batch* batch, int* block_size);
-/* Temporal blocking for lattices one cell deep (nz = 1): mode 0 (the default)
- * launches every step; mode 1 / 2 advance 4 / 2 steps per launch in LDS tiles
- * (32 x 32 cells FP32, 64 x 32 FP16 / BF16, halo included) wherever every
- * active lattice has nx and ny of at least a tile, no LES and no body force.
- * Results do not depend on the mode. The tiled launches keep a second DDF image per lattice. */
-STRH_API int strh_batch_set_tiles(strh_batch* batch, int mode);
-/* The mode, and how many of the lock-step steps of strh_batch_run so far
- * were run by tiled launches (may be NULL). */
-STRH_API int strh_batch_tiles(strh_batch* batch, int* mode, int64_t* tiled_steps);
 /* Steps per captured HIP graph (0: plain launches). Default 64. */
 STRH_API int strh_batch_set_graph_steps(strh_batch* batch, int steps);
 STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);

@@ -24,13 +24,14 @@ EXPORTS = [
     "strh_lattice_create", "strh_lattice_destroy", "strh_lattice_cells", "strh_ddf_bytes",
     "strh_set_omega", "strh_set_lid_velocity", "strh_set_uin", "strh_set_ramp_steps",
     "strh_set_force", "strh_set_c_smago", "strh_set_inflow_uniform", "strh_set_sponge",
-    "strh_set_solid_fractions", "strh_set_porosity", "strh_init_field", "strh_run", "strh_steps_done",
+    "strh_set_solid_fractions", "strh_set_porosity", "strh_init_field", "strh_run", "strh_run_timed",
+    "strh_steps_done",
     "strh_gpu_seconds", "strh_probe_moments", "strh_probe_force", "strh_state_digest",
     "strh_mass_sum", "strh_kinetic_energy", "strh_dissipation", "strh_download_ddf", "strh_upload_ddf",
     "strh_download_force", "strh_save_state", "strh_load_state", "strh_snapshot", "strh_restore", "strh_sha256",
     "strh_sha256_force_portable", "strh_batch_create", "strh_batch_destroy",
     "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
-    "strh_batch_set_block_size", "strh_batch_autotune", "strh_batch_set_tiles", "strh_batch_tiles",
+    "strh_batch_set_block_size", "strh_batch_autotune",
     "strh_batch_set_graph_steps", "strh_batch_gpu_seconds", "strh_objective_eval", "strh_vjp",
 ]
 
@@ -111,6 +112,7 @@ def lib():
         "strh_set_porosity": [vp, vp],
         "strh_init_field": [vp, u32, f32],
         "strh_run": [vp, i64],
+        "strh_run_timed": [vp, i64, vp],
         "strh_steps_done": [vp, ctypes.POINTER(i64)],
         "strh_gpu_seconds": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_probe_moments": [vp, vp],
@@ -135,8 +137,6 @@ def lib():
         "strh_batch_probe_series": [vp, i32, vp, i64, ctypes.POINTER(i64)],
         "strh_batch_set_block_size": [vp, i32],
         "strh_batch_autotune": [vp, ctypes.POINTER(i32)],
-        "strh_batch_set_tiles": [vp, i32],
-        "strh_batch_tiles": [vp, ctypes.POINTER(i32), ctypes.POINTER(i64)],
         "strh_batch_set_graph_steps": [vp, i32],
         "strh_batch_gpu_seconds": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_objective_eval": [vp, ctypes.POINTER(Objective), ctypes.POINTER(ctypes.c_double)],

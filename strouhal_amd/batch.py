@@ -70,24 +70,6 @@ class Batch:
         check(lib().strh_batch_autotune(self._h, ctypes.byref(v)))
         return v.value
 
-    def set_tiles(self, mode: int):
-        """0: one launch per step; 1 / 2: 4 / 2 steps per launch in LDS tiles
-        (lattices one cell deep). Results do not depend on the mode."""
-        check(lib().strh_batch_set_tiles(self._h, int(mode)))
-
-    @property
-    def tiles(self) -> int:
-        v = ctypes.c_int()
-        check(lib().strh_batch_tiles(self._h, ctypes.byref(v), None))
-        return v.value
-
-    @property
-    def tiled_steps(self) -> int:
-        """Lock-step steps of run() so far that ran as tiled launches."""
-        v, n = ctypes.c_int(), ctypes.c_int64()
-        check(lib().strh_batch_tiles(self._h, ctypes.byref(v), ctypes.byref(n)))
-        return n.value
-
     def set_graph_steps(self, steps: int):
         check(lib().strh_batch_set_graph_steps(self._h, int(steps)))
 

@@ -1,7 +1,8 @@
 """Build the HIP solver library in-tree: strouhal_amd/libstrouhal_hip.so.
 
-Device code (kernels.hip) is compiled by hipcc for gfx950; the host runtime
-and SHA-256 are plain C++ (g++) linked against the HIP runtime. All code is
+Device code (the csrc/*.hip translation units) is compiled by hipcc for
+gfx950, in parallel; the host runtime and SHA-256 are plain C++ (g++) linked
+against the HIP runtime. All code is
 built with -ffp-contract=off (results round after every operation except the
 explicit fused multiply-adds) and packed-FP32 instructions disabled.
 """
@@ -11,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -24,6 +26,8 @@ DEVICE_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fvisibility=hidden", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# Device translation units (the storage formats of the step kernels each have one).
+DEVICE_UNITS = ["step_fp32", "step_fp16", "step_bf16", "adjoint", "batch", "observe"]
 HOST_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden",
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wextra", "-Wno-unused-result",
@@ -40,19 +44,26 @@ def _run(cmd, verbose):
     return r
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int | None = None) -> str:
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     os.makedirs(BUILD, exist_ok=True)
     inc = f"-I{os.path.join(ROOT, 'include')}"
-    kobj = os.path.join(BUILD, "kernels.o")
-    robj = os.path.join(BUILD, "runtime.o")
-    sobj = os.path.join(BUILD, "sha256.o")
-    _run([hipcc, *DEVICE_FLAGS, inc, "-c", os.path.join(CSRC, "kernels.hip"), "-o", kobj], verbose)
-    _run(["g++", *HOST_FLAGS, inc, "-c", os.path.join(CSRC, "runtime.cpp"), "-o", robj], verbose)
-    _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "sha256.cpp"), "-o", sobj], verbose)
+    cmds, objs = [], []
+    for u in DEVICE_UNITS:
+        o = os.path.join(BUILD, u + ".o")
+        cmds.append([hipcc, *DEVICE_FLAGS, inc, "-c", os.path.join(CSRC, u + ".hip"), "-o", o])
+        objs.append(o)
+    for u in ("runtime", "sha256"):
+        o = os.path.join(BUILD, u + ".o")
+        cmds.append(["g++", *HOST_FLAGS, inc, "-c", os.path.join(CSRC, u + ".cpp"), "-o", o])
+        objs.append(o)
+    if jobs is None:
+        jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", kobj, robj, sobj, "-o", tmp,
-          "-lpthread"], verbose)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-lpthread"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

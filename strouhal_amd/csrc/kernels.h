@@ -1,4 +1,5 @@
-// Host-side launch interface of the D3Q19 kernels (kernels.hip).
+// Host-side launch interface of the D3Q19 kernels (the .hip translation units
+// beside this header; device code they share is in device.h).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,9 +44,6 @@ struct BatchLat {
     int32_t fRing;           // force buffers in the ring (sample j writes buffer j % fRing)
     uint32_t block0;         // first workgroup of this lattice in the batch grid
     uint32_t nblocks;
-    void* f2;                // second DDF image of the tiled launches (same layout), or null
-    uint32_t tile0;          // first workgroup of this lattice in a tiled launch
-    uint32_t tilesX;         // tiles along x
     int64_t step0;           // step count at the start of the batch run
     int64_t tEnd;            // absolute step at which this lattice stops
     float uinTarget;
@@ -97,6 +95,10 @@ struct AdjStep {
 };
 
 hipError_t launch_step(const StepLaunch& L, hipStream_t s);
+// launch_step per storage format (one translation unit each)
+hipError_t launch_step_fp32(const StepLaunch& L, hipStream_t s);
+hipError_t launch_step_fp16(const StepLaunch& L, hipStream_t s);
+hipError_t launch_step_bf16(const StepLaunch& L, hipStream_t s);
 // Forward step with the adjoint hooks; L.alpha must be set (zeros for a
 // lattice without porosity) unless the lattice has solid fractions (L.nt) or
 // Smagorinsky LES (L.les), which exclude porosity.
@@ -121,12 +123,6 @@ hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool f
 // Probes every force step among steps k .. k + span - 1 (span <= kProbeSpan).
 constexpr int kProbeSpan = 64;
 hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s);
-// Tiled launch of K steps (k .. k + K - 1) for lattices one cell deep; mode 1: K = 4,
-// mode 2: K = 2. swap 0 reads f and writes f2, swap 1 the reverse.
-hipError_t launch_batch_tiles(int precision, int mode, const BatchLat* lats, int L, uint32_t tileBlocks,
-                              const int64_t* base, int k, int swap, hipStream_t s);
-// Tile shape of a mode: width and height with the halo, and K.
-void batch_tile_shape(int precision, int mode, int* tw, int* th, int* K);
 hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
 hipError_t launch_set_base(int64_t* base, int64_t v, hipStream_t s);
 void box_reduce_plan(uint32_t M, uint32_t max_partials, uint32_t* Pb, uint32_t* chunk);

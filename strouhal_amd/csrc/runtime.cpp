@@ -530,6 +530,37 @@ int strh_run(strh_lattice* L, int64_t steps) {
     return run_steps(L, steps, -1);
 }
 
+int strh_run_timed(strh_lattice* L, int64_t steps, float* ms) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!ms || steps < 0 || steps > (1 << 20)) return fail(STRH_ERR_INVALID, "bad arguments (steps in [0, 2^20])");
+    if (L->alpha && (L->useEps || L->cSmago > 0.0f))
+        return fail(STRH_ERR_INVALID, "porosity cannot be combined with solid fractions or LES");
+    if (steps == 0) return STRH_OK;
+    strh_ctx* c = L->ctx;
+    std::vector<hipEvent_t> ev((size_t)steps + 1, nullptr);
+    auto release = [&]() { for (hipEvent_t e : ev) if (e) hipEventDestroy(e); };
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { e = nullptr; release(); return fail(STRH_ERR_DEVICE, "hipEventCreate"); }
+    hipError_t e = hipEventRecord(ev[0], c->stream);
+    for (int64_t s = 0; s < steps && e == hipSuccess; s++) {
+        const int64_t t = L->stepsDone + s;
+        e = launch_step(make_launch(L, t, false), c->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[(size_t)s + 1], c->stream);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(ev[(size_t)steps]);
+    double total = 0.0;
+    for (int64_t s = 0; s < steps && e == hipSuccess; s++) {
+        e = hipEventElapsedTime(&ms[s], ev[(size_t)s], ev[(size_t)s + 1]);
+        total += ms[s];
+    }
+    release();
+    if (e != hipSuccess) return hip_fail(e, "timed steps");
+    L->gpuSeconds += total * 1e-3;
+    L->stepsDone += steps;
+    return STRH_OK;
+}
+
 int strh_steps_done(strh_lattice* L, int64_t* steps) {
     if (!L || !steps) return fail(STRH_ERR_INVALID, "null argument");
     *steps = L->stepsDone;
@@ -834,13 +865,10 @@ struct strh_batch {
     std::vector<int64_t> rampLen;
     std::vector<double*> series;         // per lattice, device
     std::vector<int64_t> taken;          // samples taken so far
-    std::vector<void*> shadow;           // per lattice, second image of the tiled launches (device)
     std::vector<void*> fring;            // per probed lattice, ring of force buffers (device)
     std::vector<int> fringLen;           // buffers in that ring
     size_t devCap = 0;                   // descriptors `dev` can hold
     int bs = 256;
-    int tiles = 0;                       // 0: one launch per step; 1 / 2: tiled launches of 4 / 2 steps
-    int64_t tiledSteps = 0;              // lock-step steps run by tiled launches (strh_batch_run)
     int graphSteps = 64;
     double gpuSeconds = 0.0;
 };
@@ -890,19 +918,8 @@ struct Segment {
     int parity0;             // parity of every active lattice at k0, or -1 if they differ
     bool les;                // some active lattice has a Smagorinsky constant
     bool forced;             // some active lattice has a body force
-    int tileK;               // steps per tiled launch, or 0: every step is its own launch
-    uint32_t tileBlocks;     // workgroups of a tiled launch
     bool probes;             // some active lattice has force samples
 };
-
-// Whether lattice i can run the tiled launches of `mode` (step_tiles): one
-// cell deep, at least one tile across, no LES, no body force.
-bool batch_tile_ok(const strh_batch* B, size_t i, const BatchLat& d, int mode) {
-    const strh_lattice* L = B->lats[i];
-    int tw, th, K;
-    batch_tile_shape(B->precision, mode, &tw, &th, &K);
-    return L->nz == 1 && L->nx >= tw && L->ny >= th && !(L->cSmago > 0.0f) && d.hasForce == 0;
-}
 
 // Launches steps k = kOff .. kOff+count-1 from base; parity0 is the common
 // parity at k = kOff (or -1), so step k runs the pass (parity0 + k - kOff) & 1 only.
@@ -924,34 +941,13 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
     return STRH_OK;
 }
 
-// Launches 2 * pairs tiled launches of sg.tileK steps from k = 0 (every
-// lattice at an even step there), reading f and writing the second image,
-// then back: the state is in f again afterwards. Probes as above (kProbeSpan
-// is a multiple of both step counts).
-int batch_launch_tiles(strh_batch* B, const Segment& sg, int pairs) {
-    hipStream_t s = B->ctx->stream;
-    int k0 = 0;
-    for (int m = 0; m < 2 * pairs; m++) {
-        const int k = m * sg.tileK;
-        HIP_TRY(launch_batch_tiles(B->precision, B->tiles, sg.dev, sg.L, sg.tileBlocks, B->base, k, m & 1, s),
-                "batch tile launch");
-        const int k1 = k + sg.tileK;
-        if (sg.probes && (k1 - k0 == kProbeSpan || m == 2 * pairs - 1)) {
-            HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k0, k1 - k0, s), "batch probe launch");
-            k0 = k1;
-        }
-    }
-    return STRH_OK;
-}
-
 int batch_capture(strh_batch* B, const Segment& sg, hipGraphExec_t* out) {
     hipStream_t s = B->ctx->stream;
     hipGraph_t g = nullptr;
     // replays advance base by graphSteps: fixed parities need an even length
     const int par0 = (B->graphSteps % 2 == 0) ? sg.parity0 : -1;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-    int rc = sg.tileK > 0 ? batch_launch_tiles(B, sg, B->graphSteps / (2 * sg.tileK))
-                          : batch_launch_steps(B, sg, B->bs, B->graphSteps, par0);
+    int rc = batch_launch_steps(B, sg, B->bs, B->graphSteps, par0);
     hipError_t e2 = launch_advance_base(B->base, B->graphSteps, s);
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) { if (g) hipGraphDestroy(g); return rc; }
@@ -967,12 +963,8 @@ int batch_capture(strh_batch* B, const Segment& sg, hipGraphExec_t* out) {
 // finish: each segment launches only the still-active lattices (compacted
 // descriptors and grid), as whole captured graphs of graphSteps steps plus
 // single launches for the remainder. Kernels read t = step0 + base + k, base
-// being set at each segment start and advanced by every graph replay. A
-// segment whose lattices all take tiled launches runs one single step first
-// if they are at an odd step, then pairs of tiled launches (in the graphs when
-// graphSteps is a multiple of a pair), then single steps for what is left.
-int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probes, double* seconds,
-                  int64_t* tiled = nullptr) {
+// being set at each segment start and advanced by every graph replay.
+int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probes, double* seconds) {
     hipStream_t s = B->ctx->stream;
     const int n = (int)B->lats.size();
     std::vector<int64_t> ends;
@@ -990,11 +982,7 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         sg.parity0 = -2;
         sg.les = false;
         sg.forced = false;
-        sg.tileK = 0;
-        sg.tileBlocks = 0;
         sg.probes = false;
-        bool tiled = B->tiles > 0;
-        const size_t first = up.size();
         for (int i = 0; i < n; i++) {
             if (steps[i] <= k0) continue;
             BatchLat d = B->host[i];
@@ -1006,30 +994,9 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             d.nblocks = (B->lats[i]->N + (uint32_t)B->bs - 1) / (uint32_t)B->bs;
             if (!probes) d.fCount = 0;
             sg.probes = sg.probes || d.fCount > 0;
-            tiled = tiled && batch_tile_ok(B, (size_t)i, d, B->tiles);
             sg.blocks += d.nblocks;
             up.push_back(d);
             sg.L++;
-        }
-        if (tiled && sg.parity0 >= 0) {
-            int tw, th, K;
-            batch_tile_shape(B->precision, B->tiles, &tw, &th, &K);
-            const int64_t lead = sg.parity0;   // one single step to reach an even step
-            if (e - k0 - lead >= 2 * K) {
-                sg.tileK = K;
-                size_t j = first;
-                for (int i = 0; i < n; i++) {
-                    if (steps[i] <= k0) continue;
-                    strh_lattice* L = B->lats[i];
-                    if (!B->shadow[i]) HIP_TRY(hipMalloc(&B->shadow[i], ddf_bytes(L)), "hipMalloc(tile image)");
-                    BatchLat& d = up[j++];
-                    d.f2 = B->shadow[i];
-                    d.tilesX = (uint32_t)((L->nx + tw - 2 * K - 1) / (tw - 2 * K));
-                    const uint32_t tilesY = (uint32_t)((L->ny + th - 2 * K - 1) / (th - 2 * K));
-                    d.tile0 = sg.tileBlocks;
-                    sg.tileBlocks += d.tilesX * tilesY;
-                }
-            }
         }
         segs.push_back(sg);
         k0 = e;
@@ -1052,36 +1019,6 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         HIP_TRY(launch_set_base(B->base, sg.k0, s), "set base");
         const int64_t K = sg.k1 - sg.k0;
         int64_t done = 0;
-        if (sg.tileK > 0) {
-            if (sg.parity0 == 1) {
-                int rc = batch_launch_steps(B, sg, B->bs, 1, 1);
-                if (rc) return rc;
-                HIP_TRY(launch_advance_base(B->base, 1, s), "advance base");
-                done = 1;
-            }
-            const int pair = 2 * sg.tileK;
-            if (useGraph && B->graphSteps > 0 && B->graphSteps % pair == 0 && K - done >= B->graphSteps) {
-                hipGraphExec_t g = nullptr;
-                int rc = batch_capture(B, sg, &g);
-                if (rc) return rc;
-                while (K - done >= B->graphSteps) {
-                    hipError_t e = hipGraphLaunch(g, s);
-                    if (e != hipSuccess) { hipGraphExecDestroy(g); return hip_fail(e, "graph launch"); }
-                    done += B->graphSteps;
-                    if (tiled) *tiled += B->graphSteps;
-                }
-                hipError_t e = hipStreamSynchronize(s);
-                hipGraphExecDestroy(g);
-                if (e != hipSuccess) return hip_fail(e, "graph replays");
-            }
-            const int pairs = (int)((K - done) / pair);
-            if (tiled) *tiled += (int64_t)pairs * pair;
-            int rc = batch_launch_tiles(B, sg, pairs);
-            if (rc) return rc;
-            const int rest = (int)(K - done - (int64_t)pairs * pair);
-            if (rest > 0 && (rc = batch_launch_steps(B, sg, B->bs, rest, 0, pairs * pair))) return rc;
-            continue;
-        }
         if (useGraph && B->graphSteps > 0 && K >= B->graphSteps) {
             hipGraphExec_t g = nullptr;
             int rc = batch_capture(B, sg, &g);
@@ -1137,7 +1074,6 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
     B->rampLen.assign(count, 0);
     B->series.assign(count, nullptr);
     B->taken.assign(count, 0);
-    B->shadow.assign(count, nullptr);
     B->fring.assign(count, nullptr);
     B->fringLen.assign(count, 0);
     auto bail = [&](hipError_t e, const char* what) {
@@ -1164,7 +1100,6 @@ int strh_batch_destroy(strh_batch* B) {
     hipSetDevice(B->ctx->device);
     for (float* r : B->ramp) if (r) hipFree(r);
     for (double* d : B->series) if (d) hipFree(d);
-    for (void* p : B->shadow) if (p) hipFree(p);
     for (void* p : B->fring) if (p) hipFree(p);
     if (B->dev) hipFree(B->dev);
     if (B->base) hipFree(B->base);
@@ -1225,7 +1160,7 @@ int strh_batch_run(strh_batch* B, const int64_t* steps) {
     if (K == 0) return STRH_OK;
     if ((rc = batch_prepare(B, steps))) return rc;
     double sec = 0.0;
-    if ((rc = batch_execute(B, steps, true, true, &sec, &B->tiledSteps))) return rc;
+    if ((rc = batch_execute(B, steps, true, true, &sec))) return rc;
     B->gpuSeconds += sec;
     for (size_t i = 0; i < B->lats.size(); i++) {
         strh_lattice* L = B->lats[i];
@@ -1259,19 +1194,6 @@ int strh_batch_set_block_size(strh_batch* B, int bs) {
     return STRH_OK;
 }
 
-int strh_batch_set_tiles(strh_batch* B, int mode) {
-    if (!B || mode < 0 || mode > 2) return fail(STRH_ERR_INVALID, "tile mode must be 0, 1 or 2");
-    B->tiles = mode;
-    return STRH_OK;
-}
-
-int strh_batch_tiles(strh_batch* B, int* mode, int64_t* tiled_steps) {
-    if (!B || !mode) return fail(STRH_ERR_INVALID, "null argument");
-    *mode = B->tiles;
-    if (tiled_steps) *tiled_steps = B->tiledSteps;
-    return STRH_OK;
-}
-
 int strh_batch_set_graph_steps(strh_batch* B, int steps) {
     if (!B || steps < 0 || steps > 4096) return fail(STRH_ERR_INVALID, "graph steps must be in [0, 4096]");
     B->graphSteps = steps;
@@ -1284,11 +1206,9 @@ int strh_batch_gpu_seconds(strh_batch* B, double* seconds) {
     return STRH_OK;
 }
 
-// Times 16 steps of the batch at each workgroup size with one launch per
-// step, then each tiled mode at the fastest size, without probes, and
+// Times 16 steps of the batch at each workgroup size without probes, and
 // restores every lattice's DDFs afterwards (the state is copied aside on the
-// device). Keeps the fastest; ties go to the smaller size and to fewer steps
-// per launch.
+// device). Keeps the fastest; ties go to the smaller size.
 int strh_batch_autotune(strh_batch* B, int* chosen) {
     if (!B) return fail(STRH_ERR_INVALID, "null batch");
     int rc = set_device(B->ctx);
@@ -1302,11 +1222,11 @@ int strh_batch_autotune(strh_batch* B, int* chosen) {
             release(); return fail(STRH_ERR_DEVICE, "autotune copy");
         }
     }
-    const int keep = B->bs, keepTiles = B->tiles;
+    const int keep = B->bs;
     std::vector<int64_t> steps(B->lats.size(), 16);
     std::vector<int64_t> warm(B->lats.size(), 8);
     double best = 1e30;
-    int bestBs = keep, bestTiles = 0;
+    int bestBs = keep;
     auto time_it = [&](double* sec) {
         int r = batch_prepare(B, warm.data());
         if (!r) r = batch_execute(B, warm.data(), false, false, sec);
@@ -1314,26 +1234,17 @@ int strh_batch_autotune(strh_batch* B, int* chosen) {
         if (!r) r = batch_execute(B, steps.data(), false, false, sec);
         return r;
     };
-    B->tiles = 0;
     for (int bs : {128, 256, 512}) {
         B->bs = bs;
         double sec = 0.0;
         if ((rc = time_it(&sec))) break;
         if (sec < best * 0.98) { best = sec; bestBs = bs; }
     }
-    B->bs = bestBs;
-    for (int mode = 2; mode >= 1 && !rc; mode--) {
-        B->tiles = mode;
-        double sec = 0.0;
-        if ((rc = time_it(&sec))) break;
-        if (sec < best * 0.98) { best = sec; bestTiles = mode; }
-    }
     for (size_t i = 0; i < B->lats.size(); i++)
         hipMemcpy(B->lats[i]->f, saved[i], ddf_bytes(B->lats[i]), hipMemcpyDeviceToDevice);
     release();
-    if (rc) { B->bs = keep; B->tiles = keepTiles; return rc; }
+    if (rc) { B->bs = keep; return rc; }
     B->bs = bestBs;
-    B->tiles = bestTiles;
     if (chosen) *chosen = bestBs;
     return STRH_OK;
 }

@@ -1,0 +1,69 @@
+// Launch of one AA pass of a single lattice: picks the kernel specialisation
+// (parity, walls, partial cells, LES, forcing, porosity, row staging) from the
+// lattice's contents. Each storage format is instantiated in its own
+// translation unit (step_fp32.hip, step_fp16.hip, step_bf16.hip).
+#pragma once
+
+#include "device.h"
+
+namespace strh {
+
+template <class Conv, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
+    using T = typename Conv::T;
+    if (PARITY == 1 && !WALLS && L.rows_bs != 0 && !L.alpha) {
+        using St = Store<Conv, kPolAligned>;
+        StepParams p = L.p;
+#define STRH_ROWS(BSV, LG) hipLaunchKernelGGL((step_odd_rows<St, BSV, NT, LES, FORCE, LG>), dim3(p.blocks), \
+                                           dim3(BSV), 0, s, (T*)L.f, (float4*)L.force, L.eps, p)
+        if (L.rows_bs == 512) {
+            p.blocks = p.N / 512u;
+            if (L.large) STRH_ROWS(512, true); else STRH_ROWS(512, false);
+        } else {
+            p.blocks = p.N / 256u;
+            if (L.large) STRH_ROWS(256, true); else STRH_ROWS(256, false);
+        }
+#undef STRH_ROWS
+        return hipGetLastError();
+    }
+    using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
+    StepParams p = L.p;
+    p.blocks = (p.N + kStepBS - 1) / kStepBS;
+    if (L.alpha) {   // Brinkman lattices: forcing path with per-cell force, no NT / LES
+        if (NT || LES) return hipErrorNotSupported;
+        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, false, false, true, true>), dim3(p.blocks),
+                           dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
+                           (float4*)L.force, L.eps, p, L.alpha);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(p.blocks),
+                       dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
+                       (float4*)L.force, L.eps, p, (const float*)nullptr);
+    return hipGetLastError();
+}
+
+template <class Conv, int PARITY, bool WALLS, bool NT, bool LES>
+static hipError_t pick_force(const StepLaunch& L, hipStream_t s) {
+    return L.has_force ? launch_step_t<Conv, PARITY, WALLS, NT, LES, true>(L, s)
+                       : launch_step_t<Conv, PARITY, WALLS, NT, LES, false>(L, s);
+}
+template <class Conv, int PARITY, bool WALLS, bool NT>
+static hipError_t pick_les(const StepLaunch& L, hipStream_t s) {
+    return L.les ? pick_force<Conv, PARITY, WALLS, NT, true>(L, s)
+                 : pick_force<Conv, PARITY, WALLS, NT, false>(L, s);
+}
+template <class Conv, int PARITY, bool WALLS>
+static hipError_t pick_nt(const StepLaunch& L, hipStream_t s) {
+    return L.nt ? pick_les<Conv, PARITY, WALLS, true>(L, s) : pick_les<Conv, PARITY, WALLS, false>(L, s);
+}
+template <class Conv, int PARITY>
+static hipError_t pick_walls(const StepLaunch& L, hipStream_t s) {
+    return L.walls ? pick_nt<Conv, PARITY, true>(L, s) : pick_nt<Conv, PARITY, false>(L, s);
+}
+template <class Conv>
+static hipError_t pick_parity(const StepLaunch& L, hipStream_t s) {
+    return L.p.parity == 0 ? pick_walls<Conv, 0>(L, s) : pick_walls<Conv, 1>(L, s);
+}
+
+
+}  // namespace strh

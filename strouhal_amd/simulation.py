@@ -326,6 +326,13 @@ class Simulation:
     def run(self, steps: int):
         check(lib().strh_run(self._h, int(steps)))
 
+    def run_timed(self, steps: int) -> np.ndarray:
+        """run(steps) with a HIP event pair around every step launch: the
+        duration of each step in milliseconds (float32, one per step)."""
+        ms = np.zeros(int(steps), dtype=np.float32)
+        check(lib().strh_run_timed(self._h, int(steps), ms.ctypes.data_as(ctypes.c_void_p)))
+        return ms
+
     @property
     def steps_done(self) -> int:
         v = ctypes.c_int64()

@@ -247,11 +247,4 @@ public final class Batch {
     }
     public func autotune() throws -> Int { var bs: Int32 = 0; try check(strh_batch_autotune(h, &bs)); return Int(bs) }
     public func setGraphSteps(_ n: Int) throws { try check(strh_batch_set_graph_steps(h, Int32(n))) }
-    /// 0: one launch per step; 1 / 2: 4 / 2 steps per launch in LDS tiles (nz = 1 lattices).
-    public func setTiles(_ mode: Int) throws { try check(strh_batch_set_tiles(h, Int32(mode))) }
-    public var tiledSteps: Int {
-        var m: Int32 = 0, n: Int64 = 0
-        _ = strh_batch_tiles(h, &m, &n)
-        return Int(n)
-    }
 }
