@@ -114,13 +114,15 @@ class StreetCase:
 
 def vortex_street(gpu=None, precision="fp32", D: int = 40, uin_max: float = 0.075,
                   length_d: int = 22, sponge_d: int = 3, sponge_tau: float = 1.0,
-                  re: float = 100.0, wants_forces: bool = True) -> StreetCase:
-    """DFG 2D-2 channel with an NT cylinder, sponge and cosine ramp (CaseSetups.swift:37-64)."""
+                  re: float = 100.0, wants_forces: bool = True,
+                  lam: float | None = 3.0 / 16.0) -> StreetCase:
+    """DFG 2D-2 channel with an NT cylinder, sponge and cosine ramp (CaseSetups.swift:37-64).
+    lam: TRT magic parameter (the reference's 3/16), None for SRT."""
     nx = length_d * D + 2
     ny = int(4.1 * D) + 2
     u_mean = float(np.float32(uin_max)) * 2.0 / 3.0
     nu = u_mean * D / re
-    wp, wm = trt_omegas(3.0 * nu + 0.5, 3.0 / 16.0)
+    wp, wm = trt_omegas(3.0 * nu + 0.5, lam)
     cx = 0.5 + 2.0 * D
     cy = 0.5 + 2.0 * D
     sim = Simulation(gpu, precision, nx=nx, ny=ny, nz=1, omega=wp, omega_minus=wm,

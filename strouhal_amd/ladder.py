@@ -65,9 +65,13 @@ class Variant:
 def street_variant(gpu, precision, D: int, mach_scale: float, base_umax: float,
                    scale_steps: float = 1.0) -> Variant:
     """CaseSetups.swift:177-201 with u_max = base_umax * mach_scale and the
-    step schedule scaled by `scale_steps` (1.0 = the reference's)."""
+    step schedule scaled by `scale_steps` (1.0 = the reference's). FP32 rungs
+    collide with TRT 3/16 (CaseSetups.swift:44-57); 16-bit rungs with SRT, as
+    the reference pairs FP16 storage (README.md:78-80: TRT's antisymmetric
+    mode amplifies the half-precision quantisation noise)."""
     u = base_umax * mach_scale
-    case = vortex_street(gpu, precision, D=D, uin_max=u, re=100.0)
+    lam = 3.0 / 16.0 if precision == "fp32" else None
+    case = vortex_street(gpu, precision, D=D, uin_max=u, re=100.0, lam=lam)
     scale = D / 40.0 / mach_scale
     transient = int(60_000 * scale * scale_steps) & ~1
     sample = int(120_000 * scale * scale_steps) & ~1
@@ -146,15 +150,12 @@ def run_sequential(variants):
     return series, time.perf_counter() - t0
 
 
-def run_batched(variants, autotune=True, graph_steps=64, tiles=None):
-    """tiles: None keeps the batch's (or the autotuner's) tile mode, else 0 / 1 / 2."""
+def run_batched(variants, autotune=True, graph_steps=64):
     b = Batch([v.case.sim for v in variants])
     for i, v in enumerate(variants):
         first, period, count = probe_schedule(v)
         b.set_probe(i, first, period, count, v.case.box_x, v.case.box_y)
     bs = b.autotune() if autotune else 256
-    if tiles is not None:
-        b.set_tiles(tiles)
     b.set_graph_steps(graph_steps)
     t0 = time.perf_counter()
     b.run([v.total_steps for v in variants])
@@ -165,25 +166,21 @@ def run_batched(variants, autotune=True, graph_steps=64, tiles=None):
 
 def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64):
     """Microseconds per lock-step step of every rung together and of the finest
-    rung alone, for each batch mode (one launch per step, 4- and 2-step tiles),
-    from the rungs' initial state."""
+    rung alone, from the rungs' initial state."""
     out = {}
     for sel in ("all", "finest"):
-        for mode in (0, 1, 2):
-            sims = [v.case.sim for v in build(gpu, precision, res, 0.01, base_umax)]
-            if sel == "finest":
-                sims = sims[-1:]
-            cells = sum(s.cells for s in sims)
-            b = Batch(sims)
-            b.set_tiles(mode)
-            b.set_graph_steps(graph_steps)
-            b.run([16] * len(sims))
-            g0 = b.gpu_seconds
-            b.run([steps] * len(sims))
-            dt = b.gpu_seconds - g0
-            out[f"{sel}_tiles{mode}"] = {"us_per_step": dt / steps * 1e6, "glups": cells * steps / dt / 1e9,
-                                         "tiled_steps": b.tiled_steps}
-            b.close()
+        sims = [v.case.sim for v in build(gpu, precision, res, 0.01, base_umax)]
+        if sel == "finest":
+            sims = sims[-1:]
+        cells = sum(s.cells for s in sims)
+        b = Batch(sims)
+        b.set_graph_steps(graph_steps)
+        b.run([16] * len(sims))
+        g0 = b.gpu_seconds
+        b.run([steps] * len(sims))
+        dt = b.gpu_seconds - g0
+        out[sel] = {"us_per_step": dt / steps * 1e6, "glups": cells * steps / dt / 1e9}
+        b.close()
     return out
 
 
@@ -197,11 +194,9 @@ def main():
     ap.add_argument("--mach", type=float, default=MA_DEFAULT)
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--csv", default="", help="directory for one step,C_D,C_L file per rung")
-    ap.add_argument("--tiles", type=int, default=-1,
-                    help="batch tile mode 0 / 1 / 2 (default: the autotuner's choice)")
     ap.add_argument("--step-rate", type=int, default=0,
                     help="instead of the schedule: time this many steps of all rungs and of the "
-                         "finest rung alone in each tile mode")
+                         "finest rung alone")
     ap.add_argument("--graph-steps", type=int, default=64)
     args = ap.parse_args()
     res = [int(r) for r in args.resolutions.split(",")]
@@ -218,10 +213,9 @@ def main():
     vb = build(gpu, args.precision, res, args.scale, base_umax)
     cells = sum(v.case.sim.cells for v in vb)
     updates = sum(v.case.sim.cells * v.total_steps for v in vb)
-    sb, wb, bs, bt = run_batched(vb, tiles=None if args.tiles < 0 else args.tiles)
+    sb, wb, bs, _ = run_batched(vb)
     report.update({"lattices": len(vb), "cells": cells, "cell_updates": updates,
-                   "batched_s": wb, "batched_mlups": updates / wb / 1e6, "block_size": bs,
-                   "tile_mode": bt.tiles, "tiled_steps": bt.tiled_steps})
+                   "batched_s": wb, "batched_mlups": updates / wb / 1e6, "block_size": bs})
     rungs = []
     for v, s in zip(vb, sb):
         cd, cl, t = coefficients(v, s)

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from strouhal_amd.batch import Batch
+from tests.helpers import assert_bit_exact
 from strouhal_amd.ladder import (build, cycle_stats, coefficients, probe_schedule, run_batched,
                                  run_sequential)
 
@@ -80,77 +81,122 @@ def test_autotune_keeps_state(gpu):
     assert all(s.steps_done == 0 for s in sims)
 
 
-def _planar(gpu, precision):
-    """Lattices one cell deep of each kind the tiled launches cover: periodic
-    wall-free, a lid-driven cavity (moving wall, ramp) and the vortex street
-    (partial cells, ramped parabolic inflow, outflow, sponge, force probes).
-    Sizes are not multiples of the tile patches."""
+def _planar_specs():
+    """Lattices one cell deep: a periodic wall-free lattice, a lid-driven
+    cavity (moving wall, ramp) and two vortex-street ladder rungs, D = 8 and
+    12 (NT cylinder, ramped parabolic inflow, outflow walls, sponge, force
+    probes). Each spec builds a GPU lattice and its FP32-oracle twin."""
+    from strouhal_amd.cases import cavity_flags, channel_flags, disk_solid_fractions
+    specs = [dict(nx=100, ny=40, omega=1.7, init=(1, 0.05)),
+             dict(nx=68, ny=68, omega=1.6, omega_minus=1.3, lid=(0.08, 0.0, 0.0), ramp=30,
+                  flags=cavity_flags(66))]
+    for D in (8, 12):
+        nx, ny = 22 * D + 2, int(4.1 * D) + 2
+        c = 0.5 + 2.0 * D
+        specs.append(dict(nx=nx, ny=ny, omega=1.85, omega_minus=1.4, uin=0.1, ramp=40,
+                          flags=channel_flags(nx, ny), eps=disk_solid_fractions(nx, ny, c, c, D / 2.0),
+                          sponge=(float(nx - 1 - 3 * D), float(3 * D), 1.0), forces=True,
+                          box=((int(c) - D // 2 - 3, int(c) + D // 2 + 3),
+                               (int(c) - D // 2 - 3, int(c) + D // 2 + 3))))
+    return specs
+
+
+def _make(gpu, precision, spec, oracle=False):
+    from oracle.oracle import OracleSim
     from strouhal_amd import Simulation
-    from strouhal_amd.cases import cavity, vortex_street
-    a = Simulation(gpu, precision, nx=100, ny=40, nz=1, omega=1.7)
-    a.init_field(1, 0.05)
-    b = cavity(gpu, precision, n=66, re=1e3, ramp_steps=30)
-    street = vortex_street(gpu, precision, D=8, uin_max=0.1)
-    return [a, b, street.sim], (street.box_x, street.box_y)
+    from tests.helpers import STORAGE
+    g = Simulation(gpu, precision, nx=spec["nx"], ny=spec["ny"], nz=1, omega=spec["omega"],
+                   omega_minus=spec.get("omega_minus"), lid=spec.get("lid", (0.0, 0.0, 0.0)),
+                   uin=spec.get("uin", 0.0), ramp_steps=spec.get("ramp", 0),
+                   wants_forces=spec.get("forces", False), flags=spec.get("flags"))
+    if "eps" in spec:
+        g.set_solid_fractions(spec["eps"])
+    if "sponge" in spec:
+        g.sponge = spec["sponge"]
+    if "init" in spec:
+        g.init_field(*spec["init"])
+    if not oracle:
+        return g
+    r = OracleSim(g.nx, g.ny, 1, g.omega, g.omega_minus, arith="f32", storage=STORAGE[precision],
+                  lid=g.lid_vel, uin=g.uin_target, ramp_steps=g.ramp_steps, wants_forces=g.wants_forces,
+                  flags=g.flags, threads=4)
+    if "eps" in spec:
+        r.set_solid_fractions(spec["eps"])
+    r.sponge = g.sponge
+    img = g.download_ddf()          # the GPU's initial field (its sin / cos)
+    r.upload_ddf(img if precision != "fp16s" else img.view(np.uint16))
+    return g, r
 
 
-def _run_planar(gpu, precision, mode, graph):
-    sims, box = _planar(gpu, precision)
-    bt = Batch(sims)
-    bt.set_tiles(mode)
-    bt.set_graph_steps(graph)
-    bt.set_probe(2, 7, 6, 12, *box)      # odd steps 7, 13, ..., 73
-    bt.run([36, 50, 44])      # all even at the start; segments of 36, 8 and 6 steps
-    bt.run([3, 0, 17])        # the street alone from an odd step: one single step first
-    bt.run([22, 9, 0])        # mixed parities (single steps), then one lattice even
-    bt.run([40, 24, 16])      # all odd: one single step, then tiles
-    return ([s.state_digest for s in sims], [s.steps_done for s in sims], bt.probe_series(2),
-            bt.tiled_steps)
+def _planar(gpu, precision):
+    specs = _planar_specs()
+    sims = [_make(gpu, precision, sp) for sp in specs]
+    return sims[:3], specs[2]["box"]
+
+
+# Batch runs of the oracle comparison: lock-step chunks (steps per lattice).
+# The first run starts every lattice at step 0 (segments where lattices finish);
+# the second starts the streets alone at odd steps; the third mixes parities
+# (a lattice at an even step beside one at an odd step).
+_RUNS = [[36, 50, 44, 30], [3, 0, 17, 9], [22, 9, 0, 14], [40, 24, 16, 31]]
+# Probe schedules (first odd step, even period, count) of the two streets.
+_PROBES = {2: (7, 6, 12), 3: (5, 2, 30)}
+
+
+def _oracle_series(r, box, first, period, count, total):
+    """The oracle run of `total` steps with Ladder.measure's probes: a force
+    sample on odd step t is probe_force from step t - 1 (zero the field, two
+    steps, force on the second, FP64 box sum)."""
+    out = []
+    for j in range(count):
+        t = first + j * period
+        if t + 1 > total:
+            break
+        r.run(t - 1 - r.steps_done)
+        out.append(r.probe_force(*box))
+    r.run(total - r.steps_done)
+    return np.array(out).reshape(-1, 3)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16s", "bf16"])
-@pytest.mark.parametrize("mode,graph", [(1, 16), (2, 16), (1, 0), (2, 12), (1, 64)])
-def test_tiled_batch_matches_one_launch_per_step(gpu, precision, mode, graph):
-    d0, n0, s0, t0 = _run_planar(gpu, precision, 0, graph)
-    d1, n1, s1, t1 = _run_planar(gpu, precision, mode, graph)
-    assert t0 == 0
-    assert t1 >= 64, "the tiled launches did not run"
-    assert n1 == n0
-    assert d1 == d0
-    assert s1.shape == s0.shape == (12, 3)
-    assert s0[-1, 0] != 0.0          # the ramped flow has reached the cylinder
-    assert np.array_equal(s1, s0)
-
-
-def test_tiled_batch_matches_solo_runs(gpu):
-    sims, _ = _planar(gpu, "fp32")
-    solo, _ = _planar(gpu, "fp32")
+@pytest.mark.parametrize("graph", [16, 0])
+def test_batch_matches_oracle(gpu, precision, graph):
+    """Every lattice of a batch (wall-free, cavity, two street rungs with
+    probes) against the FP32 oracle run alone: DDF images bit for bit, and
+    each FP64 force sample within 1e-13 of the oracle's probe_force."""
+    specs = _planar_specs()
+    pairs = [_make(gpu, precision, sp, oracle=True) for sp in specs]
+    sims = [g for g, _ in pairs]
     bt = Batch(sims)
-    bt.set_tiles(1)
-    bt.run([40, 56, 72])
-    assert bt.tiled_steps >= 56
-    for s, k in zip(solo, [40, 56, 72]):
-        s.run(k)
-    for x, y in zip(sims, solo):
-        assert x.state_digest == y.state_digest
+    bt.set_graph_steps(graph)
+    for i, (first, period, count) in _PROBES.items():
+        bt.set_probe(i, first, period, count, *specs[i]["box"])
+    for run in _RUNS:
+        bt.run(run)
+    totals = [sum(r[i] for r in _RUNS) for i in range(len(specs))]
+    for i, ((g, r), sp) in enumerate(zip(pairs, specs)):
+        assert g.steps_done == totals[i]
+        if i in _PROBES:
+            first, period, count = _PROBES[i]
+            ref = _oracle_series(r, sp["box"], first, period, count, totals[i])
+            got = bt.probe_series(i)
+            assert got.shape == ref.shape and len(ref) > 0, (i, got.shape, ref.shape)
+            assert np.all(np.isfinite(got))
+            scale = np.abs(ref).max() + 1e-300
+            assert np.all(np.abs(got - ref) <= 1e-13 * scale), (i, np.abs(got - ref).max(), scale)
+            assert ref[-1, 0] != 0.0            # the ramped flow has reached the cylinder
+        else:
+            r.run(totals[i])
+        assert r.steps_done == totals[i]
+        assert_bit_exact(g.download_ddf(), r.download_ddf(), f"lattice {i} DDFs after {totals[i]} steps")
 
 
-def test_tiles_skip_lattices_they_do_not_cover(gpu):
-    # nz > 1 in the batch: every segment runs one launch per step
-    sims = _mixed(gpu, "fp32")
-    bt = Batch(sims)
-    bt.set_tiles(1)
-    bt.run([16, 16, 16])
-    assert bt.tiled_steps == 0
-
-
-def test_autotune_over_tile_modes_keeps_state(gpu):
+def test_autotune_keeps_state_of_planar_lattices(gpu):
     sims, _ = _planar(gpu, "fp16s")
     solo, _ = _planar(gpu, "fp16s")
     bt = Batch(sims)
     before = [s.state_digest for s in sims]
-    bs = bt.autotune()
-    assert bs in (128, 256, 512) and bt.tiles in (0, 1, 2)
+    assert bt.autotune() in (128, 256, 512)
     assert [s.state_digest for s in sims] == before
     bt.run([24, 24, 24])
     for s in solo:
@@ -158,57 +204,13 @@ def test_autotune_over_tile_modes_keeps_state(gpu):
     assert [s.state_digest for s in sims] == [s.state_digest for s in solo]
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_tiled_batch_with_several_samples_per_launch(gpu, mode):
-    # probe period 2 < K: one tiled launch takes up to two samples, each in
-    # its own buffer of the force ring
-    def run(m):
-        sims, box = _planar(gpu, "fp32")
-        bt = Batch(sims)
-        bt.set_tiles(m)
-        bt.set_graph_steps(16)
-        bt.set_probe(2, 5, 2, 40, *box)          # odd steps 5, 7, ..., 83
-        bt.run([48, 48, 88])
-        return [s.state_digest for s in sims], bt.probe_series(2), bt.tiled_steps
-    d0, s0, _ = run(0)
-    d1, s1, t1 = run(mode)
-    assert t1 >= 80
-    assert d1 == d0
-    assert s1.shape == (40, 3) and np.array_equal(s1, s0)
-
-
-@pytest.mark.parametrize("precision,nx,ny", [("fp32", 32, 32), ("fp32", 33, 45), ("fp16s", 64, 32),
-                                              ("bf16", 70, 33)])
-def test_tiled_batch_on_lattices_of_about_one_tile(gpu, precision, nx, ny):
-    # tiles wrap onto themselves: wrapped duplicates evolve like the cells
-    # they stand for and only the owned patch cells are written back
-    from strouhal_amd import Simulation
-
-    def make():
-        a = Simulation(gpu, precision, nx=nx, ny=ny, nz=1, omega=1.6)
-        a.init_field(1, 0.05)
-        b = Simulation(gpu, precision, nx=nx, ny=ny, nz=1, omega=1.5, lid=(0.05, 0.0, 0.0),
-                       flags=lambda x, y, z: np.where(y == 0, 1, np.where(y == ny - 1, 2, 0)))
-        return [a, b]
-    tiled, solo = make(), make()
-    bt = Batch(tiled)
-    bt.set_tiles(1)
-    bt.run([40, 24])
-    assert bt.tiled_steps >= 24
-    for s, k in zip(solo, [40, 24]):
-        s.run(k)
-    assert [s.state_digest for s in tiled] == [s.state_digest for s in solo]
-
-
-@pytest.mark.parametrize("mode", [0, 1])
-def test_probe_series_do_not_depend_on_how_runs_are_split(gpu, mode):
+def test_probe_series_do_not_depend_on_how_runs_are_split(gpu):
     # the force ring is reduced once per 64 steps and at the end of each run:
     # splitting a run at odd places (spans that start mid-ring, runs that end
     # mid-span) must give the same samples and states as one long run
     def run(splits):
         sims, box = _planar(gpu, "fp32")
         bt = Batch(sims)
-        bt.set_tiles(mode)
         bt.set_graph_steps(16)
         bt.set_probe(2, 3, 4, 40, *box)          # odd steps 3, 7, ..., 159
         for k in splits:

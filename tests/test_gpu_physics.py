@@ -226,6 +226,44 @@ def test_tgv_256_energy_decay(gpu):
     assert np.max(np.abs(e16 - e32)) / e32[0] < 5e-3
 
 
+def test_tgv_512_energy_decay(gpu):
+    """Config 2 at its full 512^3 size: U0=0.05, Re=1600 (tau = 0.507639), TRT
+    3/16 with FP32 storage (10.2 GB of DDFs, the blocked image beyond 4 GiB)
+    and SRT with FP16 shifted storage, 0.5 convective times sampled every
+    0.1 t*: E*(0) = 1/8 within the initial-field discretisation error, E*
+    decays monotonically, the FP16 run tracks FP32 within 5e-3 of E*(0), and
+    the mean decay rate over 0.5 t* is within 5% of the initial field's
+    viscous dissipation, dE*/dt* = -2 (3 / Re) E*(0) = -3 / (4 Re) (k^2 = 3 in
+    units of L; the laminar stage of the vortex)."""
+    from strouhal_amd import Simulation, trt_omegas
+    from strouhal_amd.tgv import tgv_setup
+    n, u0 = 512, 0.05
+    L, tau, tc, stride = tgv_setup(n, u0)
+    assert abs(tau - 0.507639) < 1e-6
+    series = {}
+    for precision, lam in (("fp32", 3.0 / 16.0), ("fp16s", None)):
+        wp, wm = trt_omegas(tau, lam)
+        sim = Simulation(gpu, precision, nx=n, ny=n, nz=n, omega=wp, omega_minus=wm)
+        sim.init_field(2, u0)
+        e = [sim.kinetic_energy() / (2.0 * n ** 3 * u0 ** 2)]
+        for _ in range(5):
+            sim.run(stride)
+            e.append(sim.kinetic_energy() / (2.0 * n ** 3 * u0 ** 2))
+        assert sim.steps_done == 5 * stride
+        series[precision] = np.array(e)
+        sim.close()
+    e32, e16 = series["fp32"], series["fp16s"]
+    print("TGV 512^3 E*(t*) fp32:", np.round(e32, 7), "fp16:", np.round(e16, 7))
+    assert np.all(np.isfinite(e32)) and np.all(np.isfinite(e16))
+    assert abs(e32[0] - 0.125) < 1e-3
+    assert np.all(np.diff(e32) < 0) and np.all(np.diff(e16) < 0)
+    assert np.max(np.abs(e16 - e32)) / e32[0] < 5e-3
+    # early laminar decay of the TGV: dE*/dt* = -2 (3/Re) E*(0) at t* -> 0
+    t = 5 * stride / tc
+    rate = (e32[0] - e32[-1]) / t
+    assert rate == pytest.approx(2.0 * 3.0 / 1600.0 * e32[0], rel=0.05), rate
+
+
 @pytest.mark.parametrize("precision,lam", [("fp32", 3.0 / 16.0), ("fp16s", None), ("bf16", None)])
 def test_tgv64_energy_and_dissipation_vs_fp64_oracle(gpu, precision, lam):
     """Config 2's observables at 64^3 over one convective time, every 0.1 t*:

@@ -106,6 +106,39 @@ def test_lone_ddf_streaming(storage):
     assert np.count_nonzero(g) == 18
 
 
+@pytest.mark.parametrize("wall", [1, 4])
+def test_lone_ddf_bounces_off_a_wall_cell(wall):
+    """Halfway bounce-back at a SOLID cell and the dormant anti-bounce-back of an
+    OUTFLOW cell (Kernels.metal:299-300, 328-329: sign -1, no moving-wall term).
+    With omega = 0 the collision is the identity, so a lone DDF v in direction i
+    streams to A = c0 + (k - 1) c_i, meets the wall cell c0 + k c_i on step k,
+    comes back at A in direction opp(i), and after T = 8 steps sits at
+    A - (T - k) c_i: +v (SOLID) or -v (OUTFLOW), in slot opp(i), the only
+    non-zero slot. k = 3 bounces on an odd pass's pull, k = 4 on its push. Lid
+    and inlet speeds are non-zero, so a wrongly applied moving-wall term would
+    show."""
+    n, T = 16, 8
+    for i, k in ((i, k) for i in range(1, 19) for k in (3, 4)):
+        c = np.array([CX[i], CY[i], CZ[i]])
+        p0 = np.array([8, 8, 8])
+        w = (p0 + k * c) % n
+        fl = np.zeros((n, n, n), np.uint8)
+        fl[w[2], w[1], w[0]] = wall
+        s = OracleSim(n, n, n, 0.0, arith="f64", storage="f64", flags=fl, lid=(0.03, 0.02, 0.01),
+                      uin=0.05)
+        f = s.download_ddf()
+        v = 0.001 * i
+        c0 = (p0[2] * n + p0[1]) * n + p0[0]
+        f[i, c0] = v
+        s.upload_ddf(f)
+        s.run(T)
+        g = s.download_ddf()
+        e = (p0 + (k - 1) * c - (T - k) * c) % n      # A - (T - k) c_i
+        d = (e[2] * n + e[1]) * n + e[0]
+        assert g[OPP[i], d] == (v if wall == 1 else -v), (i, k, g[OPP[i], d])
+        assert np.count_nonzero(g) == 1, (i, k)
+
+
 def test_rest_fixed_points():
     s = OracleSim(32, 32, 32, 1.7, storage="f32")
     d0 = s.state_digest
@@ -348,6 +381,38 @@ def test_inflow_walls_inject_the_flux():
     for x in (1, nx // 2, nx - 2):
         flux = float((m[1:ny - 1, x, 0] * m[1:ny - 1, x, 3]).sum())
         assert flux == pytest.approx(injected, rel=2e-3)
+
+
+def test_parabolic_inflow_walls_inject_the_flux():
+    """The parabolic velocity wall (Kernels.metal:81-87): u_w(y) = max(4 uin
+    yd (H - yd) / H^2, 0), yd = y - 1/2, H = ny - 2, taken at the wall cell's
+    row. At steady state every column carries what the inlet links inject:
+    per fluid row y, links 1, 11, 13 pull from the inlet cell of row y (6 w c_x
+    u_w = u_w/3, u_w/6, u_w/6), link 7 from row y - 1 and link 9 from row y + 1
+    (u_w/6 each, where that row is an inlet cell, not a solid corner)."""
+    nx, ny, uin = 24, 10, 0.04
+    H = ny - 2
+    tau = 0.8
+    wp, wm = trt(tau, 3.0 / 16.0)
+    fl = np.zeros((1, ny, nx), np.uint8)
+    fl[0, :, 0] = fl[0, :, nx - 1] = 3
+    fl[0, 0, :] = fl[0, ny - 1, :] = 1
+    s = OracleSim(nx, ny, 1, wp, wm, arith="f64", storage="f64", uin=uin, flags=fl, threads=4)
+    s.run(30000)
+    m = s.probe_moments().reshape(ny, nx, 4)
+
+    def uw(y):
+        if y < 1 or y > ny - 2:
+            return 0.0                       # solid corner: plain bounce-back
+        yd = y - 0.5
+        return max(4.0 * uin * yd * (H - yd) / (H * H), 0.0)
+    injected = sum(uw(y) / 3.0 + uw(y) / 6.0 + uw(y) / 6.0 + uw(y - 1) / 6.0 + uw(y + 1) / 6.0
+                   for y in range(1, ny - 1))
+    for x in (1, nx // 2, nx - 2):
+        flux = float((m[1:ny - 1, x, 0] * m[1:ny - 1, x, 3]).sum())
+        assert flux == pytest.approx(injected, rel=2e-3)
+    # the profile is the parabola, not the uniform speed: 5.375 uin against 8 uin
+    assert injected < 0.75 * uin * H
 
 
 # ----------------------------------------------------------- accuracy gates --
