@@ -31,6 +31,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -46,6 +48,10 @@ def tgv_params(n: int, u0: float = 0.05, re: float = 1600.0):
 
 
 def measure(precision: str, lam, n: int, steps: int, warmup: int):
+    """One leg: `warmup` untimed steps, then exactly `steps` timed steps (wall
+    clock around strh_run, which synchronises at its end, and HIP events on the
+    library's stream), then the same number of steps again with an event pair
+    around every launch for the per-dispatch distribution."""
     from strouhal_amd import Simulation, trt_omegas
     tau = tgv_params(n)
     wp, wm = trt_omegas(tau, lam)
@@ -57,6 +63,9 @@ def measure(precision: str, lam, n: int, steps: int, warmup: int):
     sim.run(steps)
     wall = time.perf_counter() - t0
     gpu = sim.gpu_seconds - g0
+    per = sim.run_timed(max(steps, 2)).astype(float)
+    if sim.steps_done & 1:
+        sim.run(1)       # the energy probe reads the natural slot order
     ke = sim.kinetic_energy()
     cells = n ** 3
     sim.close()
@@ -68,31 +77,64 @@ def measure(precision: str, lam, n: int, steps: int, warmup: int):
         "mlups_gpu_events": cells * steps / gpu / 1e6,
         "ms_per_step": wall / steps * 1e3,
         "kernel_ms": gpu / steps * 1e3,
+        "dispatch_ms": {"steps": int(per.size), "min": float(per.min()), "median": float(np.median(per)),
+                        "max": float(per.max()), "mean": float(per.mean()),
+                        "first": float(per[0]), "last": float(per[-1])},
         "finite": bool(math.isfinite(ke)),
         "bytes_per_cell": 152 if precision == "fp32" else 76,
     }
 
 
-def cpu_baseline(n: int, sample_steps: int):
-    """FP32 oracle on the host cores: `sample_steps` steps of the same 256^3 TGV."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n: int):
+    """The oracle (a scalar C++ restatement of the same update) on the host
+    cores, in FP32 and FP64 arithmetic, with one thread and with the cores this
+    process may use (at most 16, the CPU share of one GPU): a bounded sample of
+    the same 256^3 TGV workload (1-2 untimed steps, then 2-6 timed)."""
     from oracle.oracle import OracleSim
     from strouhal_amd import trt_omegas
-    threads = min(16, os.cpu_count() or 1)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = max(1, min(16, avail))
     tau = tgv_params(n)
     wp, wm = trt_omegas(tau, 3.0 / 16.0)
-    sim = OracleSim(n, n, n, wp, wm, arith="f32", storage="f32", threads=threads)
-    sim.init_field(2, 0.05)
-    sim.run(2)   # warm (page in)
-    t0 = time.perf_counter()
-    sim.run(sample_steps)
-    dt = time.perf_counter() - t0
+    runs = {}
+    for arith in ("f32", "f64"):
+        sim = OracleSim(n, n, n, wp, wm, arith=arith, storage=arith, threads=threads)
+        sim.init_field(2, 0.05)
+        for t, (warm, timed) in ((threads, (2, 6)), (1, (1, 2))):
+            sim.set_threads(t)
+            sim.run(warm)
+            t0 = time.perf_counter()
+            sim.run(timed)
+            dt = time.perf_counter() - t0
+            runs[f"{arith}_{t}t"] = {"mlups": round(n ** 3 * timed / dt / 1e6, 2), "steps": timed,
+                                     "seconds": round(dt, 2)}
+        del sim
+    head = runs[f"f32_{threads}t"]
     return {
-        "value": n ** 3 * sample_steps / dt / 1e6,
+        "value": head["mlups"],
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP32 oracle, {n}^3 TGV TRT(3/16), {sample_steps} steps after 2 warm-up steps, "
-                  f"{threads} OpenMP threads, {dt:.1f} s",
+        "sample": f"FP32 oracle, {n}^3 TGV TRT(3/16), {head['steps']} steps after 2 warm-up steps, "
+                  f"{threads} OpenMP threads ({head['seconds']} s); FP64 and 1-thread runs in `runs`",
+        "runs": runs,
+        "nproc": os.cpu_count(),
+        "threads_available": avail,
+        "cpu_model": _cpu_model(),
     }
 
 
@@ -143,7 +185,6 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--cpu-sample-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
@@ -151,8 +192,10 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU z-slab sharding is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
-    steps = args.steps + (args.steps & 1)      # even: natural slot order at the end
-    warmup = args.warmup + (args.warmup & 1)
+    steps, warmup = args.steps, args.warmup
+    if steps < 1 or warmup < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
     n = args.n
 
     fp32 = measure("fp32", 3.0 / 16.0, n, steps, warmup)
@@ -171,7 +214,7 @@ def main():
     }
     fp16_achieved = fp16["bytes_per_cell"] * cells / (fp16["kernel_ms"] * 1e-3) / 1e9
     adj = None if args.no_adjoint else adjoint_ratio()
-    cpu = None if args.no_cpu_baseline else cpu_baseline(n, args.cpu_sample_steps)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(n)
     line = {
         "metric": "MLUPS + % HBM roofline, D3Q19 256³ FP32/FP16; adjoint/forward step-time ratio",
         "value": round(fp32["mlups"], 1),
