@@ -143,6 +143,17 @@ STRH_API int strh_mass_sum(strh_lattice* lat, double* out);
  * per-cell term in FP32 as Cases.swift:1242), reduced on the GPU in a fixed
  * order. Requires an even step count. */
 STRH_API int strh_kinetic_energy(strh_lattice* lat, double* out);
+/* strh_kinetic_energy without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The per-cell term is
+ * computed straight from the DDFs and reduced in the same fixed order, so the
+ * result is the same bits as strh_kinetic_energy. The reduction works in a
+ * static device array of 9,728 partial sums plus the result (77,856 bytes per
+ * GPU, held until the process ends); a lattice needing more partials is
+ * refused (STRH_ERR_INVALID; the plan caps at that many, so none does up to
+ * the 200M-cell lattice limit). Every call on one GPU shares that array: calls
+ * that overlap on one GPU (two contexts, two threads) race. */
+STRH_API int strh_kinetic_energy_noalloc(strh_lattice* lat, double* out);
 /* Viscous dissipation rate sum_fluid 2 nu S:S (FP64, fixed order), nu =
  * (1/omega - 1/2)/3, with the strain rate from the non-equilibrium momentum
  * flux, S = -3 Pi_neq / (2 rho tau). No reference counterpart: the reference

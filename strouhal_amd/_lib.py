@@ -27,7 +27,7 @@ EXPORTS = [
     "strh_set_solid_fractions", "strh_set_porosity", "strh_init_field", "strh_run", "strh_run_timed",
     "strh_steps_done",
     "strh_gpu_seconds", "strh_probe_moments", "strh_probe_force", "strh_state_digest",
-    "strh_mass_sum", "strh_kinetic_energy", "strh_dissipation", "strh_download_ddf", "strh_upload_ddf",
+    "strh_mass_sum", "strh_kinetic_energy", "strh_kinetic_energy_noalloc", "strh_dissipation", "strh_download_ddf", "strh_upload_ddf",
     "strh_download_force", "strh_save_state", "strh_load_state", "strh_snapshot", "strh_restore", "strh_sha256",
     "strh_sha256_force_portable", "strh_batch_create", "strh_batch_destroy",
     "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
@@ -120,6 +120,7 @@ def lib():
         "strh_state_digest": [vp, vp],
         "strh_mass_sum": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_kinetic_energy": [vp, ctypes.POINTER(ctypes.c_double)],
+        "strh_kinetic_energy_noalloc": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_dissipation": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_download_ddf": [vp, vp, sz],
         "strh_upload_ddf": [vp, vp, sz],

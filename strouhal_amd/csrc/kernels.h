@@ -141,6 +141,19 @@ hipError_t launch_box_force_sum(const float* force, uint32_t nx, uint32_t ny, ui
 hipError_t launch_dissipation_sum(int precision, const void* f, const uint8_t* flags, uint32_t N,
                                   DdfLayout lay, double tauP, double* partials, uint32_t max_partials,
                                   double* out, hipStream_t s);
+// Block partials of the fixed-order FP64 reductions (stage 1 writes at most
+// this many; box_reduce_plan caps the plan at it).
+constexpr uint32_t kReducePartials = 19 * 512;
+// The allocation-free kinetic energy's static device scratch: the partials and
+// the result (77,856 bytes).
+constexpr uint32_t kScratchDoubles = kReducePartials + 4;
+// Device address of that scratch on the current device.
+hipError_t scratch_address(double** out);
+// sum over cells of |u|^2 straight from the DDFs (no moments field); the same
+// bits as launch_moments followed by launch_kinetic_sum.
+hipError_t launch_kinetic_sum_cells(int precision, const void* f, const uint8_t* flags, uint32_t N,
+                                    DdfLayout lay, float fx, float fy, float fz, double* partials,
+                                    uint32_t max_partials, double* out, hipStream_t s);
 hipError_t launch_kinetic_sum(const float* moments, uint32_t N, double* partials,
                               uint32_t max_partials, double* out, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);

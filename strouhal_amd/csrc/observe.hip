@@ -5,13 +5,11 @@
 namespace strh {
 
 // ---- moments probe: Kernels.metal:359-380 (literal loop order) ----------
+// (ux, uy, uz, rho) of fluid cell n; shared by the probe and the fused
+// kinetic-energy reduction so both give the same bits.
 template <class St>
-__global__ void __launch_bounds__(256)
-moments_kernel(const typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
-               float4* __restrict__ out, uint32_t N, DdfLayout lay, float fx, float fy, float fz) {
-    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
-    if (n >= N) return;
-    if (flags[n] != kFluid) { out[n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); return; }
+__device__ __forceinline__ float4 cell_moments(const typename St::T* __restrict__ f, const DdfLayout& lay,
+                                               uint32_t n, float fx, float fy, float fz) {
     const size_t e = cell_elem(lay, n);
     float rhom1 = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
 #pragma unroll
@@ -23,8 +21,16 @@ moments_kernel(const typename St::T* __restrict__ f, const uint8_t* __restrict__
         pz += (float)kCz[i] * v;
     }
     const float rho = 1.0f + rhom1;
-    out[n] = make_float4((px + 0.5f * fx) / rho, (py + 0.5f * fy) / rho,
-                         (pz + 0.5f * fz) / rho, rho);
+    return make_float4((px + 0.5f * fx) / rho, (py + 0.5f * fy) / rho, (pz + 0.5f * fz) / rho, rho);
+}
+
+template <class St>
+__global__ void __launch_bounds__(256)
+moments_kernel(const typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
+               float4* __restrict__ out, uint32_t N, DdfLayout lay, float fx, float fy, float fz) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= N) return;
+    out[n] = flags[n] != kFluid ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : cell_moments<St>(f, lay, n, fx, fy, fz);
 }
 
 // ---- field initialiser: Kernels.metal:386-453 ----------------------------
@@ -116,6 +122,28 @@ struct KineticElem {
         return (double)(v.x * v.x + v.y * v.y + v.z * v.z);
     }
 };
+
+// The same per-cell term straight from the DDFs, without the moments field:
+// cell_moments then KineticElem's arithmetic, so the sum is the same bits.
+template <class St> struct KineticCellElem {
+    const typename St::T* f;
+    const uint8_t* flags;
+    DdfLayout lay;
+    float fx, fy, fz;
+    __device__ double operator()(uint32_t, uint32_t k) const {
+        if (flags[k] != kFluid) return 0.0;    // the probe's zero moments
+        const float4 v = cell_moments<St>(f, lay, k, fx, fy, fz);
+        return (double)(v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+};
+
+// Reduction scratch of the allocation-free kinetic energy
+// (strh_kinetic_energy_noalloc): kReducePartials block partials, then the
+// result. One copy per GPU, held until the process ends; the host reaches it
+// only through the address scratch_address() returns on that device. Every
+// partial a reduction reads is written by the same reduction's first stage,
+// so nothing has to start at zero; calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 // Viscous dissipation of one cell, 2 nu S:S, with the strain rate from the
 // non-equilibrium momentum flux of the natural-order state (Chapman-Enskog:
@@ -266,6 +294,21 @@ hipError_t launch_dissipation_sum(int precision, const void* f, const uint8_t* f
 hipError_t launch_kinetic_sum(const float* moments, uint32_t N, double* partials,
                               uint32_t max_partials, double* out, hipStream_t s) {
     return reduce(KineticElem{(const float4*)moments}, N, 1, partials, max_partials, out, s);
+}
+
+hipError_t scratch_address(double** out) {
+    return hipGetSymbolAddress((void**)out, HIP_SYMBOL(scratch));
+}
+
+hipError_t launch_kinetic_sum_cells(int precision, const void* f, const uint8_t* flags, uint32_t N,
+                                    DdfLayout lay, float fx, float fy, float fz, double* partials,
+                                    uint32_t max_partials, double* out, hipStream_t s) {
+    switch (precision) {
+        case kFP32: return reduce(KineticCellElem<StF32>{(const float*)f, flags, lay, fx, fy, fz}, N, 1, partials, max_partials, out, s);
+        case kFP16S: return reduce(KineticCellElem<StF16>{(const uint16_t*)f, flags, lay, fx, fy, fz}, N, 1, partials, max_partials, out, s);
+        case kBF16: return reduce(KineticCellElem<StBF16>{(const uint16_t*)f, flags, lay, fx, fy, fz}, N, 1, partials, max_partials, out, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {

@@ -50,7 +50,7 @@ int hip_fail(hipError_t e, const char* what) {
         if (_e != hipSuccess) return hip_fail(_e, what); \
     } while (0)
 
-constexpr uint32_t kMaxPartials = 19 * 512;
+constexpr uint32_t kMaxPartials = kReducePartials;
 // The largest lattice: 19*N must index the mass reduction in 32 bits and the
 // per-slot byte offsets of the step kernel must fit 32 bits.
 constexpr uint64_t kMaxCells = 200000000ull;
@@ -62,6 +62,10 @@ struct strh_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string name;
+    // device address of this device's static reduction scratch (observe.hip),
+    // or null with the reason in scratchError
+    double* scratch = nullptr;
+    std::string scratchError;
 };
 
 struct strh_lattice {
@@ -324,6 +328,13 @@ int strh_ctx_create(int device_ordinal, strh_ctx** out) {
     c->device = device_ordinal;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess) c->name = prop.gcnArchName;
+    // Each device holds its own copy of the static scratch at its own address:
+    // resolved here, with this device current, and kept with the context.
+    const hipError_t se = scratch_address(&c->scratch);
+    if (se != hipSuccess) {
+        c->scratch = nullptr;
+        c->scratchError = hipGetErrorString(se);
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -620,6 +631,27 @@ int strh_kinetic_energy(strh_lattice* L, double* out) {
     HIP_TRY(launch_kinetic_sum(L->moments, L->N, L->partials, kMaxPartials, L->result, L->ctx->stream),
             "kinetic reduction launch");
     HIP_TRY(hipMemcpyAsync(out, L->result, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream), "copy energy");
+    return sync(L, "kinetic energy");
+}
+
+int strh_kinetic_energy_noalloc(strh_lattice* L, double* out) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!out) return fail(STRH_ERR_INVALID, "null output");
+    if (L->stepsDone % 2 != 0) return fail(STRH_ERR_ODD_STEPS, "kinetic energy requires even step count");
+    double* scr = L->ctx->scratch;
+    if (!scr) return fail(STRH_ERR_DEVICE, "static scratch address: %s", L->ctx->scratchError.c_str());
+    uint32_t Pb = 0, chunk = 0;
+    box_reduce_plan(L->N, kReducePartials, &Pb, &chunk);
+    if (Pb > kReducePartials)
+        return fail(STRH_ERR_INVALID, "lattice of %u cells needs %u partial sums; the static scratch holds %u",
+                    L->N, Pb, kReducePartials);
+    HIP_TRY(launch_kinetic_sum_cells(L->precision, L->f, L->flags, L->N, L->lay, L->bodyForce[0],
+                                     L->bodyForce[1], L->bodyForce[2], scr, kReducePartials,
+                                     scr + kReducePartials, L->ctx->stream),
+            "kinetic reduction launch");
+    HIP_TRY(hipMemcpyAsync(out, scr + kReducePartials, sizeof(double), hipMemcpyDeviceToHost, L->ctx->stream),
+            "copy energy");
     return sync(L, "kinetic energy");
 }
 
