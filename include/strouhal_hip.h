@@ -198,7 +198,10 @@ STRH_API int strh_batch_destroy(strh_batch* batch);
  * first_step + j*period (j < count; first_step odd, period even), each the
  * FP64 box sum of probe_force over x in [x0,x1], y in [y0,y1], all z: the
  * schedule of Ladder.measure (Ladder.swift:56-66) with the probes kept on the
- * device. Needs wants_forces. */
+ * device. Needs wants_forces. A lattice with a schedule must not be stepped
+ * outside the batch between batch runs (strh_batch_run then fails with
+ * STRH_ERR_INVALID: the samples of those steps would be missing); setting the
+ * schedule again restarts it. */
 STRH_API int strh_batch_set_probe(strh_batch* batch, int index, int64_t first_step,
                                   int64_t period, int64_t count, int x0, int x1, int y0, int y1);
 /* Advance lattice i by steps[i] steps (all in lock-step from the first). */

@@ -897,6 +897,7 @@ struct strh_batch {
     std::vector<int64_t> rampLen;
     std::vector<double*> series;         // per lattice, device
     std::vector<int64_t> taken;          // samples taken so far
+    std::vector<int64_t> lastSteps;      // stepsDone of each lattice after its last batch run
     std::vector<void*> fring;            // per probed lattice, ring of force buffers (device)
     std::vector<int> fringLen;           // buffers in that ring
     size_t devCap = 0;                   // descriptors `dev` can hold
@@ -1106,6 +1107,8 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
     B->rampLen.assign(count, 0);
     B->series.assign(count, nullptr);
     B->taken.assign(count, 0);
+    B->lastSteps.resize(count);
+    for (int i = 0; i < count; i++) B->lastSteps[i] = lats[i]->stepsDone;
     B->fring.assign(count, nullptr);
     B->fringLen.assign(count, 0);
     auto bail = [&](hipError_t e, const char* what) {
@@ -1166,6 +1169,11 @@ int strh_batch_set_probe(strh_batch* B, int index, int64_t first, int64_t period
     if (count > 0) HIP_TRY(hipMalloc(&B->series[index], (size_t)count * 3 * sizeof(double)), "hipMalloc(series)");
     d.series = B->series[index];
     B->taken[index] = 0;
+    B->lastSteps[index] = L->stepsDone;
+    // unwritten samples read back as NaN, never as stale device memory
+    if (count > 0)
+        HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)B->series[index], 0x7ff80000u, (size_t)count * 6,
+                                  B->ctx->stream), "fill series");
     if (count > 0) {
         // One force buffer per sample of a probe launch's kProbeSpan steps, plus one.
         const int ring = (int)((kProbeSpan + period - 1) / period) + 1;
@@ -1190,6 +1198,11 @@ int strh_batch_run(strh_batch* B, const int64_t* steps) {
         K = std::max(K, steps[i]);
     }
     if (K == 0) return STRH_OK;
+    for (size_t i = 0; i < B->lats.size(); i++)
+        if (B->host[i].fCount > 0 && B->lats[i]->stepsDone != B->lastSteps[i])
+            return fail(STRH_ERR_INVALID, "lattice %zu was stepped outside the batch since its last batch run "
+                        "(%lld steps done, %lld expected): its probe samples would be skipped; set its probe "
+                        "schedule again", i, (long long)B->lats[i]->stepsDone, (long long)B->lastSteps[i]);
     if ((rc = batch_prepare(B, steps))) return rc;
     double sec = 0.0;
     if ((rc = batch_execute(B, steps, true, true, &sec))) return rc;
@@ -1205,6 +1218,7 @@ int strh_batch_run(strh_batch* B, const int64_t* steps) {
         }
         L->stepsDone += steps[i];
         L->gpuSeconds += sec;
+        B->lastSteps[i] = L->stepsDone;
     }
     return STRH_OK;
 }

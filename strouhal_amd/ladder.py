@@ -198,6 +198,7 @@ def main():
                     help="instead of the schedule: time this many steps of all rungs and of the "
                          "finest rung alone")
     ap.add_argument("--graph-steps", type=int, default=64)
+    ap.add_argument("--report", default="", help="write the credibility report (markdown) of mean C_D here")
     args = ap.parse_args()
     res = [int(r) for r in args.resolutions.split(",")]
     base_umax = 1.5 * args.mach / math.sqrt(3.0)      # Ma of U_mean = umax * 2/3 * sqrt(3)
@@ -233,7 +234,44 @@ def main():
         same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(sb, ss)) and all(
             a.case.sim.state_digest == b.case.sim.state_digest for a, b in zip(vb, vs))
         report.update({"sequential_s": ws, "speedup": ws / wb, "bitwise_identical": bool(same)})
+    if args.report:
+        md = credibility_report(vb, sb, rungs, report, wb, base_umax)
+        with open(args.report, "w") as fh:
+            fh.write(md)
+        report["credibility_report"] = args.report
     print(json.dumps(report), flush=True)
+
+
+def credibility_report(variants, series, rungs, report, wall_s, base_umax) -> str:
+    """Ladder.credibility and its report (Ladder.swift:74-202) for mean C_D:
+    the Mach-1 rungs form the resolution ladder, the finest rung's half-Mach
+    copy is the compressibility anchor. The verification list holds this
+    run's own checks: finite series, the batched / sequential identity when
+    the sequential run was made, and the finest rung against the DFG 2D-2
+    reference intervals."""
+    from strouhal_amd import credibility as cr
+    ladder, half, fine = {}, None, max(v.D for v in variants)
+    for v, s in zip(variants, series):
+        cd, _, _ = coefficients(v, s)
+        if v.mach_scale == 1.0:
+            ladder[v.D] = cd
+        elif v.D == fine:
+            half = cd
+    finest = next(v for v in variants if v.D == fine and v.mach_scale == 1.0)
+    st = next(r for r in rungs if r["D"] == fine and r["mach_scale"] == 1.0)
+    ma = base_umax * 2.0 / 3.0 * math.sqrt(3.0)           # Ma of the mean inlet speed
+    gates = [f"every rung's force series finite: {'yes' if all(r['finite'] for r in rungs) else 'NO'}",
+             f"finest rung (D = {fine}): St {st['strouhal']:.4f} (DFG 2D-2 interval 0.295-0.305), "
+             f"max C_L {st['max_cl']:.4f} (0.99-1.01)"]
+    if "bitwise_identical" in report:
+        gates.append("batched and sequential ladders bit-identical (every DDF digest, every FP64 force "
+                     f"sample): {'yes' if report['bitwise_identical'] else 'NO'}")
+    setup = (f"DFG 2D-2 cylinder in a channel 22D x 4.1D, Re 100, parabolic velocity inlet ramped over "
+             f"24,000 steps, Noble-Torczynski cylinder, outlet sponge; D in {sorted(ladder)} at Ma {ma:.3f} "
+             f"plus a half-Mach anchor at D = {fine}; {report['precision']} storage")
+    run = cr.credibility("mean C_D", ladder, half, finest.case.sim.state_digest, re=100.0, finest_mach=ma,
+                         geometry="bluff body in channel", setup=setup, wall_seconds=wall_s)
+    return cr.markdown(run, gates)
 
 
 if __name__ == "__main__":

@@ -33,10 +33,15 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
-def _stats(ms) -> dict:
+def _stats(ms, first_parity: int = 0) -> dict:
+    """Duration statistics of consecutive steps; even / odd: medians of the
+    steps taken from an even / odd step count (the two AA passes)."""
     ms = np.asarray(ms, dtype=float)
+    ev, od = (ms[0::2], ms[1::2]) if first_parity == 0 else (ms[1::2], ms[0::2])
     return {"steps": int(ms.size), "mean": float(ms.mean()), "median": float(np.median(ms)),
             "min": float(ms.min()), "max": float(ms.max()), "argmax": int(ms.argmax()),
+            "even_median": float(np.median(ev)) if ev.size else None,
+            "odd_median": float(np.median(od)) if od.size else None,
             "ms": [round(float(x), 4) for x in ms]}
 
 
@@ -48,16 +53,31 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--dispatch", action="store_true", help="print every step's duration")
     ap.add_argument("--replay", type=int, default=0, help="steps per pass of the snapshot replay")
+    ap.add_argument("--case", default="tgv", choices=["tgv", "sphere", "street"],
+                    help="tgv: the bench's n^3 Taylor-Green vortex; sphere: the 512x256x256 LES "
+                         "sphere (D 40, NT cells, inlets, sponge); street: the ladder's finest "
+                         "rung (D 64 at half Mach 0.1)")
     args = ap.parse_args()
     from strouhal_amd import Simulation, trt_omegas
     n = args.n
-    L = n / (2 * math.pi)
-    tau = 3 * (0.05 * L / 1600.0) + 0.5
     lam = 3.0 / 16.0 if args.precision == "fp32" else None
-    wp, wm = trt_omegas(tau, lam)
-    sim = Simulation(None, args.precision, nx=n, ny=n, nz=n, omega=wp, omega_minus=wm)
-    sim.init_field(2, 0.05)
-    head = {"precision": args.precision, "n": n, "warmup": args.warmup}
+    if args.case == "sphere":
+        from strouhal_amd.cases import sphere_case
+        sim = sphere_case(None, args.precision, D=40, size=(512, 256, 256), u=0.05, re=300.0,
+                          c_smago=0.01, center=(96.0, 128.0, 128.0), lam=None).sim
+        n = sim.cells
+    elif args.case == "street":
+        from strouhal_amd.cases import vortex_street
+        sim = vortex_street(None, args.precision, D=64, uin_max=0.5 * 1.5 * 0.1 / math.sqrt(3.0),
+                            lam=lam).sim
+        n = sim.cells
+    else:
+        L = n / (2 * math.pi)
+        tau = 3 * (0.05 * L / 1600.0) + 0.5
+        wp, wm = trt_omegas(tau, lam)
+        sim = Simulation(None, args.precision, nx=n, ny=n, nz=n, omega=wp, omega_minus=wm)
+        sim.init_field(2, 0.05)
+    head = {"case": args.case, "precision": args.precision, "n": n, "warmup": args.warmup}
     if args.replay > 0:
         sim.run(args.warmup)
         sim.snapshot()
@@ -71,13 +91,15 @@ def main():
         return
     if args.dispatch:
         sim.run(args.warmup)
-        print(json.dumps({**head, "dispatch": _stats(sim.run_timed(args.steps))}), flush=True)
+        par = sim.steps_done & 1
+        print(json.dumps({**head, "dispatch": _stats(sim.run_timed(args.steps), par)}), flush=True)
         return
     sim.run(args.steps + (args.steps & 1))
     mass = sim.mass_sum()
     B = 4 if args.precision == "fp32" else 2
-    print(f"profile_workload: {args.precision} {n}^3, {sim.steps_done} steps, "
-          f"ddf bytes {19 * n ** 3 * B}, mass {mass:.6e}", flush=True)
+    cells = sim.cells
+    print(f"profile_workload: {args.case} {args.precision} {cells} cells, {sim.steps_done} steps, "
+          f"ddf bytes {19 * cells * B}, mass {mass:.6e}", flush=True)
 
 
 if __name__ == "__main__":

@@ -34,6 +34,17 @@ def test_ladder_batched_equals_sequential(gpu, precision):
     for a, b in zip(vb, vs):
         assert a.case.sim.steps_done == b.case.sim.steps_done
         assert a.case.sim.state_digest == b.case.sim.state_digest
+    # the credibility report of the short ladder (Ladder.swift:74-202)
+    from strouhal_amd.ladder import credibility_report
+    rungs = []
+    for v, s in zip(vb, sb):
+        cd, cl, t = coefficients(v, s)
+        rungs.append({"D": v.D, "mach_scale": v.mach_scale, "finite": bool(np.all(np.isfinite(s))),
+                      **cycle_stats(cl, cd, t, v.D, v.case.u_mean)})
+    md = credibility_report(vb, sb, rungs, {"precision": precision, "bitwise_identical": True}, 1.0,
+                            BASE_UMAX)
+    assert "# Credibility report — mean C_D" in md and "## Resolution ladder" in md
+    assert vb[-2].case.sim.state_digest in md          # the finest Mach-1 rung's digest
 
 
 def _mixed(gpu, precision):
@@ -221,3 +232,20 @@ def test_probe_series_do_not_depend_on_how_runs_are_split(gpu):
     assert d1 == d0
     assert s0.shape == s1.shape == (40, 3)
     assert np.array_equal(s1, s0)
+
+
+def test_probed_lattice_stepped_outside_the_batch_is_refused(gpu):
+    from strouhal_amd._lib import StrouhalError
+    sims, box = _planar(gpu, "fp32")
+    bt = Batch(sims)
+    bt.set_probe(2, 3, 4, 10, *box)
+    bt.run([8, 8, 8])
+    sims[0].run(2)                     # no probes on lattice 0: allowed
+    bt.run([4, 4, 4])
+    sims[2].run(2)                     # the probed street steps alone
+    with pytest.raises(StrouhalError):
+        bt.run([4, 4, 4])
+    assert sims[2].steps_done == 14
+    bt.set_probe(2, 15, 4, 2, *box)    # a new schedule from the current step
+    bt.run([6, 6, 6])
+    assert bt.probe_series(2).shape == (2, 3)
