@@ -17,6 +17,7 @@ Usage (under rocprofv3, or alone):
     python3 strouhal_amd/profile_workload.py --precision fp32 --n 256 --steps 10
     python3 strouhal_amd/profile_workload.py --precision fp16s --dispatch --warmup 5 --steps 400
     python3 strouhal_amd/profile_workload.py --precision fp16s --replay 60 --warmup 4
+    python3 strouhal_amd/profile_workload.py --precision fp32 --copy 50
 """
 from __future__ import annotations
 
@@ -53,6 +54,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--dispatch", action="store_true", help="print every step's duration")
     ap.add_argument("--replay", type=int, default=0, help="steps per pass of the snapshot replay")
+    ap.add_argument("--copy", type=int, default=0,
+                    help="time this many device-to-device copies of the DDF image (a bandwidth ceiling)")
     ap.add_argument("--case", default="tgv", choices=["tgv", "sphere", "street"],
                     help="tgv: the bench's n^3 Taylor-Green vortex; sphere: the 512x256x256 LES "
                          "sphere (D 40, NT cells, inlets, sponge); street: the ladder's finest "
@@ -78,6 +81,23 @@ def main():
         sim = Simulation(None, args.precision, nx=n, ny=n, nz=n, omega=wp, omega_minus=wm)
         sim.init_field(2, 0.05)
     head = {"case": args.case, "precision": args.precision, "n": n, "warmup": args.warmup}
+    if args.copy > 0:
+        # device-to-device copies of the DDF image (strh_snapshot): read and
+        # write the same bytes as one step, with no arithmetic and no shift
+        import ctypes
+        import time
+        from strouhal_amd._lib import lib
+        sz = ctypes.c_size_t()
+        lib().strh_ddf_bytes(sim._h, ctypes.byref(sz))
+        sim.snapshot()
+        t0 = time.perf_counter()
+        for _ in range(args.copy):
+            sim.snapshot()
+        dt = time.perf_counter() - t0
+        print(json.dumps({**head, "copy": {"bytes_per_copy": 2 * sz.value, "copies": args.copy,
+                                           "seconds": dt, "GBps": 2 * sz.value * args.copy / dt / 1e9}}),
+              flush=True)
+        return
     if args.replay > 0:
         sim.run(args.warmup)
         sim.snapshot()
