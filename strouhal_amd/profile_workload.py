@@ -56,10 +56,11 @@ def main():
     ap.add_argument("--replay", type=int, default=0, help="steps per pass of the snapshot replay")
     ap.add_argument("--copy", type=int, default=0,
                     help="time this many device-to-device copies of the DDF image (a bandwidth ceiling)")
-    ap.add_argument("--case", default="tgv", choices=["tgv", "sphere", "street"],
+    ap.add_argument("--case", default="tgv", choices=["tgv", "sphere", "street", "cavity", "channel"],
                     help="tgv: the bench's n^3 Taylor-Green vortex; sphere: the 512x256x256 LES "
                          "sphere (D 40, NT cells, inlets, sponge); street: the ladder's finest "
-                         "rung (D 64 at half Mach 0.1)")
+                         "rung (D 64 at half Mach 0.1); cavity: the 128^2 Re 400 lid-driven "
+                         "cavity with its wall frame (SRT); channel: the 512x256 porous channel")
     args = ap.parse_args()
     from strouhal_amd import Simulation, trt_omegas
     n = args.n
@@ -68,6 +69,14 @@ def main():
         from strouhal_amd.cases import sphere_case
         sim = sphere_case(None, args.precision, D=40, size=(512, 256, 256), u=0.05, re=300.0,
                           c_smago=0.01, center=(96.0, 128.0, 128.0), lam=None).sim
+        n = sim.cells
+    elif args.case == "cavity":
+        from strouhal_amd.cases import cavity
+        sim = cavity(None, args.precision, n=128, re=400.0)
+        n = sim.cells
+    elif args.case == "channel":
+        from strouhal_amd.channel_adjoint import channel_case
+        sim, _ = channel_case(None)
         n = sim.cells
     elif args.case == "street":
         from strouhal_amd.cases import vortex_street
@@ -114,12 +123,15 @@ def main():
         par = sim.steps_done & 1
         print(json.dumps({**head, "dispatch": _stats(sim.run_timed(args.steps), par)}), flush=True)
         return
+    g0 = sim.gpu_seconds
     sim.run(args.steps + (args.steps & 1))
+    gpu_us = (sim.gpu_seconds - g0) / (args.steps + (args.steps & 1)) * 1e6
     mass = sim.mass_sum()
     B = 4 if args.precision == "fp32" else 2
     cells = sim.cells
     print(f"profile_workload: {args.case} {args.precision} {cells} cells, {sim.steps_done} steps, "
-          f"ddf bytes {19 * cells * B}, mass {mass:.6e}", flush=True)
+          f"{gpu_us:.3f} us per step (HIP events around the run), ddf bytes {19 * cells * B}, "
+          f"mass {mass:.6e}", flush=True)
 
 
 if __name__ == "__main__":
