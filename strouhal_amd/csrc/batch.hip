@@ -77,9 +77,15 @@ step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
 // (l, c) sums component c of lattice l with exactly the chunking of the
 // single-lattice probe (reduce_stage1 over Pb chunks, then reduce_stage2), so
 // the FP64 values are the same bits.
+// The probe's per-chunk partials, read back from LDS in stage 2's order.
+struct SharedPartials {
+    const double* p;
+    __device__ double operator()(uint32_t, uint32_t k) const { return p[k]; }
+};
+
 __global__ void __launch_bounds__(256)
 probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k, int span) {
-    __shared__ double lds[256];
+    __shared__ double lds[4];
     __shared__ double part[kBatchMaxChunks];
     const int l = blockIdx.x / 3, c = blockIdx.x % 3;
     if (l >= L) return;
@@ -97,15 +103,11 @@ probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base
         for (uint32_t pb = 0; pb < B.Pb; pb++) {
             const uint32_t lo = pb * B.chunk;
             const uint32_t hi = (B.M - lo) > B.chunk ? lo + B.chunk : B.M;
-            double acc = 0.0;
-            for (uint32_t q = lo + threadIdx.x; q < hi; q += 256u) acc += e(0, q);
-            const double sum = block_tree_sum(acc, lds);
+            const double sum = block_tree_sum(chunk_sum(e, 0u, lo, hi), lds);
             if (threadIdx.x == 0) part[pb] = sum;
             __syncthreads();
         }
-        double acc = 0.0;
-        for (uint32_t q = threadIdx.x; q < B.Pb; q += 256u) acc += part[q];
-        const double total = block_tree_sum(acc, lds);
+        const double total = block_tree_sum(chunk_sum(SharedPartials{part}, 0u, 0u, B.Pb), lds);
         if (threadIdx.x == 0) B.series[3 * ((t - B.fT0) / B.fPeriod) + c] = total;
         __syncthreads();
     }

@@ -872,15 +872,47 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
 // segment-major partials the same way. The order depends only on the lattice
 // size and the fixed chunking — never on timing — so every result is bitwise
 // reproducible run to run.
-__device__ __forceinline__ double block_tree_sum(double v, double* lds) {
-    lds[threadIdx.x] = v;
-    __syncthreads();
+//
+// Sum over the 64 lanes of a wave by a fixed butterfly of shuffles: at each
+// step a lane adds the value of the lane whose index differs in one bit.
+// IEEE addition is commutative, so both lanes of a pair form the same sum,
+// and after six steps every lane holds the same bits.
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) lds[threadIdx.x] = lds[threadIdx.x] + lds[threadIdx.x + s];
-        __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum of a 256-thread workgroup: the four wave sums through LDS, added
+// in wave order, (w0 + w1) + (w2 + w3). Every thread returns the same bits;
+// lds (at least 4 doubles) may be reused as soon as the call returns.
+__device__ __forceinline__ double block_tree_sum(double v, double* lds) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63u) == 0u) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double s = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    __syncthreads();
+    return s;
+}
+
+// A thread's share of sum e(seg, k), k in [lo, hi), in a 256-thread block:
+// the elements lo + t + 256 j, accumulated into four chains by j mod 4 (four
+// loads in flight per thread instead of one) and combined as (a0 + a1) +
+// (a2 + a3). The order depends only on lo, hi and the thread index.
+template <class Elem>
+__device__ __forceinline__ double chunk_sum(const Elem& e, uint32_t seg, uint32_t lo, uint32_t hi) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    uint32_t k = lo + threadIdx.x;
+    for (; k + 768u < hi; k += 1024u) {
+        a0 += e(seg, k);
+        a1 += e(seg, k + 256u);
+        a2 += e(seg, k + 512u);
+        a3 += e(seg, k + 768u);
     }
-    return lds[0];
+    if (k < hi) a0 += e(seg, k);
+    if (k + 256u < hi) a1 += e(seg, k + 256u);
+    if (k + 512u < hi) a2 += e(seg, k + 512u);
+    return (a0 + a1) + (a2 + a3);
 }
 
 struct BoxForceElem {

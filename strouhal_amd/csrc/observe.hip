@@ -89,21 +89,22 @@ init_kernel(typename St::T* __restrict__ f, uint32_t nx, uint32_t ny, uint32_t n
 template <class Elem>
 __global__ void __launch_bounds__(256)
 reduce_stage1(Elem e, uint32_t M, uint32_t chunk, double* __restrict__ partials) {
-    __shared__ double lds[256];
+    __shared__ double lds[4];
     const uint32_t lo = blockIdx.x * chunk;
     const uint32_t hi = (M - lo) > chunk ? lo + chunk : M;
-    double acc = 0.0;
-    for (uint32_t k = lo + threadIdx.x; k < hi; k += 256u) acc += e(blockIdx.y, k);
-    const double s = block_tree_sum(acc, lds);
+    const double s = block_tree_sum(chunk_sum(e, blockIdx.y, lo, hi), lds);
     if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
+struct PartialElem {
+    const double* p;
+    __device__ double operator()(uint32_t, uint32_t k) const { return p[k]; }
+};
+
 __global__ void __launch_bounds__(256)
 reduce_stage2(const double* __restrict__ partials, uint32_t P, double* __restrict__ out) {
-    __shared__ double lds[256];
-    double acc = 0.0;
-    for (uint32_t k = threadIdx.x; k < P; k += 256u) acc += partials[k];
-    const double s = block_tree_sum(acc, lds);
+    __shared__ double lds[4];
+    const double s = block_tree_sum(chunk_sum(PartialElem{partials}, 0u, 0u, P), lds);
     if (threadIdx.x == 0) out[0] = s;
 }
 
