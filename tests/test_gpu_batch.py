@@ -47,6 +47,28 @@ def test_ladder_batched_equals_sequential(gpu, precision):
     assert vb[-2].case.sim.state_digest in md          # the finest Mach-1 rung's digest
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp16s"])
+def test_full_size_ladder_batched_equals_sequential(gpu, precision):
+    # The ladder at its full size (D in {16, 24, 32, 48, 64}, each at Ma 0.1
+    # and half Mach: 10 lattices, 1,504,680 cells) on 1% of the reference
+    # schedule: the batch ends every lattice on the sequential run's bits,
+    # with the same FP64 force samples, and no rung goes non-finite (FP16
+    # rungs collide with SRT).
+    res = [16, 24, 32, 48, 64]
+    vb = build(gpu, precision, res, 0.01, BASE_UMAX)
+    vs = build(gpu, precision, res, 0.01, BASE_UMAX)
+    assert len(vb) == 10 and sum(v.case.sim.cells for v in vb) == 1_504_680
+    sb, _, _, _ = run_batched(vb, autotune=True, graph_steps=64)
+    ss, _ = run_sequential(vs)
+    for a, b, v in zip(sb, ss, vb):
+        assert a.shape == b.shape == (v.probes, 3)
+        assert np.all(np.isfinite(a)), (v.D, v.mach_scale)
+        assert np.array_equal(a, b), (v.D, v.mach_scale)
+    for a, b in zip(vb, vs):
+        assert a.case.sim.steps_done == b.case.sim.steps_done == a.total_steps
+        assert a.case.sim.state_digest == b.case.sim.state_digest, (a.D, a.mach_scale)
+
+
 def _mixed(gpu, precision):
     from strouhal_amd import Simulation, trt_omegas
     from strouhal_amd.cases import cavity
