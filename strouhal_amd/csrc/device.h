@@ -486,7 +486,7 @@ __device__ __forceinline__ void record_populations(const AdjHooks& hk, const flo
 // One cell of one AA pass (the body shared by the single-lattice kernel and
 // the multi-lattice batch kernel).
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
-          bool HOOK = false>
+          bool HOOK = false, bool OUT = true>
 __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                                           const uint8_t* __restrict__ flags_,
                                           const uint32_t* __restrict__ link_,
@@ -503,6 +503,10 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     const GLOBAL float* __restrict__ epsBuf = (const GLOBAL float*)epsBuf_;
     const GLOBAL float* __restrict__ alphaBuf = (const GLOBAL float*)alphaBuf_;
     using T = typename St::T;
+    // OUT = false compiles the per-cell force output out (steps that write no
+    // force): its six accumulators then do not stay live across the
+    // collision, which lowers the pass's VGPRs and raises its occupancy.
+    const bool writes = OUT && p.writeForce != 0;
     T* fs[19];
 #pragma unroll
     for (int i = 0; i < 19; i++) fs[i] = f + (size_t)i * p.lay.W;
@@ -541,7 +545,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         St::wr(fs[0], nbs, fh[0]);
 #pragma unroll
         for (int i = 1; i < 19; i++) St::wr(fs[kOpp[i]], nbs, fh[i]);
-        if (NT && p.writeForce != 0 && eps > 0.0f) force[n] = make_float4(ntx, nty, ntz, 0.0f);
+        if (NT && writes && eps > 0.0f) force[n] = make_float4(ntx, nty, ntz, 0.0f);
         return;
     }
 
@@ -602,7 +606,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
             constexpr int srcBit = kOpp[I] - 1;                                       \
             if ((sM >> srcBit) & 1u) {                                                \
                 if ((lM >> srcBit) & 1u) deferred |= 1u << I;                         \
-                if (p.writeForce != 0) {                                              \
+                if (writes) {                                                         \
                     ax = acc<-kCx[I]>(ax, fh[I]);                                     \
                     ay = acc<-kCy[I]>(ay, fh[I]);                                     \
                     az = acc<-kCz[I]>(az, fh[I]);                                     \
@@ -629,7 +633,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                 const float cu = (float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz; \
                 const float corr = 6.0f * wgt<I>() * cu;                              \
                 fh[I] += corr;                                                        \
-                if (p.writeForce != 0) {                                              \
+                if (writes) {                                                         \
                     ax = acc<-kCx[I]>(ax, corr);                                      \
                     ay = acc<-kCy[I]>(ay, corr);                                      \
                     az = acc<-kCz[I]>(az, corr);                                      \
@@ -701,7 +705,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                              * ((float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz); \
                     }                                                                 \
                 }                                                                     \
-                if (p.writeForce != 0) {                                              \
+                if (writes) {                                                         \
                     const float v = sign * fh[I] + 2.0f * wgt<I>();                   \
                     ax = acc<kCx[I]>(ax, v);                                          \
                     ay = acc<kCy[I]>(ay, v);                                          \
@@ -727,11 +731,12 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     STRH_PUSH(7) STRH_PUSH(8) STRH_PUSH(9) STRH_PUSH(10) STRH_PUSH(11) STRH_PUSH(12)
     STRH_PUSH(13) STRH_PUSH(14) STRH_PUSH(15) STRH_PUSH(16) STRH_PUSH(17) STRH_PUSH(18)
 #undef STRH_PUSH
-    if (p.writeForce != 0 && (sM != 0u || eps > 0.0f))               // 351-353
+    if (writes && (sM != 0u || eps > 0.0f))                        // 351-353
         force[n] = make_float4(ax, ay, az, 0.0f);
 }
 
-template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false>
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
+          bool OUT = true>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
@@ -739,8 +744,8 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             const float* __restrict__ alphaBuf) {
     const uint32_t n = blockIdx.x * kStepBS + threadIdx.x;
     if (n >= p.N) return;
-    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK>(f, flags, link, lidm, force, epsBuf, p, n,
-                                                        alphaBuf);
+    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK, false, OUT>(f, flags, link, lidm, force,
+                                                                    epsBuf, p, n, alphaBuf);
 }
 
 // Forward step with the adjoint hooks (Guo forcing with Brinkman porosity, or
@@ -754,8 +759,9 @@ step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ f
                    AdjHooks hk) {
     const uint32_t n = blockIdx.x * kStepBS + threadIdx.x;
     if (n >= p.N) return;
-    step_cell<St, PARITY, WALLS, NT, LES, true, BRINK, true>(f, flags, link, lidm, nullptr, epsBuf, p, n,
-                                                             alphaBuf, &hk);
+    // (writes no force: the force pointer is null and writeForce 0)
+    step_cell<St, PARITY, WALLS, NT, LES, true, BRINK, true, false>(f, flags, link, lidm, nullptr, epsBuf,
+                                                                    p, n, alphaBuf, &hk);
 }
 
 // ---- odd pass with LDS row staging (wall-free lattices) --------------------

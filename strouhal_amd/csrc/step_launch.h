@@ -36,9 +36,16 @@ static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
                            (float4*)L.force, L.eps, p, L.alpha);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(p.blocks),
-                       dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
-                       (float4*)L.force, L.eps, p, (const float*)nullptr);
+    // Steps that write no force run the kernel with the force output compiled
+    // out (only walls and partial cells have one).
+    if ((WALLS || NT) && p.writeForce == 0)
+        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE, false, false>), dim3(p.blocks),
+                           dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
+                           (float4*)L.force, L.eps, p, (const float*)nullptr);
+    else
+        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(p.blocks),
+                           dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
+                           (float4*)L.force, L.eps, p, (const float*)nullptr);
     return hipGetLastError();
 }
 
