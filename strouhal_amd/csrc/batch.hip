@@ -61,9 +61,14 @@ step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
         // Batched lattices are small enough to stay in the last-level cache
         // between passes: the default policy (measured against nontemporal
         // loads / stores on the street ladder).
+        // Probe steps are odd (strh_batch_set_probe refuses others), so the
+        // even pass never writes a force: its output is compiled out, which
+        // frees its accumulators' registers.
         using St = Store<Conv, kPolShifted>;
-        if (FORCED && B.hasForce) step_cell<St, 0, true, true, LES, true>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
-        else step_cell<St, 0, true, true, LES, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
+        if (FORCED && B.hasForce)
+            step_cell<St, 0, true, true, LES, true, false, false, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
+        else
+            step_cell<St, 0, true, true, LES, false, false, false, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
     } else if (PAR != 0) {
         using St = Store<Conv, kPolShifted>;
         if (FORCED && B.hasForce) step_cell<St, 1, true, true, LES, true>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
