@@ -94,3 +94,43 @@ def test_sphere_wake_demo_re100(gpu):
         prev = cd
     assert np.isfinite(cd)
     assert 1.0 <= cd <= 1.45, (cd, c.sim.steps_done)
+
+
+def _converged_cavity(gpu, precision, n, re, lam, max_steps, check_every=5000, tol=5e-7):
+    """The reference's cavity run to convergence (Cases.swift:167-214): every
+    check_every steps after the lid ramp, the relative change of the (u, v)
+    field; converged below tol, or on a round-off plateau (below 5e-5 in FP32,
+    2e-3 otherwise, and no better than 0.95 x the best of the four previous
+    checks)."""
+    from strouhal_amd.cases import cavity
+    sim = cavity(gpu, precision, n=n, re=re, lam=lam)
+    prev, history, converged = None, [], False
+    while sim.steps_done < max_steps:
+        sim.run(min(check_every, max_steps - sim.steps_done))
+        cur = sim.probe_moments()[:, :2].astype(np.float64)
+        if prev is not None and sim.steps_done > 5000:
+            nsum = float(np.sum(cur * cur))
+            res = float(np.sqrt(np.sum((cur - prev) ** 2) / nsum)) if nsum > 0 else 0.0
+            history.append(res)
+            floor = 5e-5 if precision == "fp32" else 2e-3
+            if res < tol:
+                converged = True
+            elif res < floor and len(history) >= 5 and res > 0.95 * min(history[-5:-1]):
+                converged = True
+        prev = cur
+        if converged:
+            break
+    return sim, converged
+
+
+@pytest.mark.parametrize("n,re,precision,lam,max_steps", [
+    (128, 100.0, "fp32", 0.25, 100_000),        # the smoke gate (main.swift:183-187)
+    (256, 1000.0, "fp32", 0.25, 1_200_000),     # the Re 1000 gate (main.swift:160-166, 189-193)
+    (256, 1000.0, "fp16s", None, 1_200_000),    # FP16 storage pairs with SRT there
+])
+def test_cavity_converges_to_ghia(gpu, n, re, precision, lam, max_steps):
+    sim, converged = _converged_cavity(gpu, precision, n, re, lam, max_steps)
+    m = sim.probe_moments()
+    rms = ghia_rms(m, n, n + 2, ULID, {100.0: "re100", 400.0: "re400", 1000.0: "re1000"}[re])
+    assert converged, (sim.steps_done, rms)
+    assert rms <= 0.02, (rms, sim.steps_done)

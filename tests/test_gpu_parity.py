@@ -421,12 +421,12 @@ def test_determinism_run_twice(gpu):
         digests = []
         for _ in range(2):
             sim = cavity(gpu, precision, n=128, re=1000.0, lam=0.25)
-            sim.run(2000)
+            sim.run(10_000)
             digests.append(sim.state_digest)
         assert digests[0] == digests[1]
         d2 = []
         for _ in range(2):
-            sim = Simulation(gpu, precision, nx=64, ny=64, nz=64, omega=1.9)
+            sim = Simulation(gpu, precision, nx=128, ny=128, nz=128, omega=1.9)
             sim.init_field(1, 0.05)
             sim.run(100)
             d2.append((sim.state_digest, sim.mass_sum(), sim.kinetic_energy()))
