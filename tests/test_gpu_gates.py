@@ -134,3 +134,55 @@ def test_cavity_converges_to_ghia(gpu, n, re, precision, lam, max_steps):
     rms = ghia_rms(m, n, n + 2, ULID, {100.0: "re100", 400.0: "re400", 1000.0: "re1000"}[re])
     assert converged, (sim.steps_done, rms)
     assert rms <= 0.02, (rms, sim.steps_done)
+
+
+def _dfg2(gpu, D, transient, sample_cycles, curved, uin_max=0.075):
+    """The reference's DFG 2D-2 run (Cases.swift:840-947): TRT 3/16 channel,
+    cylinder at (2D + 1/2, 2D + 1/2) as partial cells (curved) or solid
+    cells, C_D / C_L probed every 26 steps after the transient, cycle
+    statistics from the upward zero crossings of C_L."""
+    from strouhal_amd.cases import channel_flags, vortex_street
+    from strouhal_amd.ladder import cycle_stats
+    if curved:
+        c = vortex_street(gpu, "fp32", D=D, uin_max=uin_max)
+        sim, u_mean, box_x, box_y = c.sim, c.u_mean, c.box_x, c.box_y
+    else:
+        nx, ny = 22 * D + 2, int(4.1 * D) + 2
+        u_mean = float(np.float32(uin_max)) * 2.0 / 3.0
+        wp, wm = trt_omegas(3.0 * u_mean * D / 100.0 + 0.5, 3.0 / 16.0)
+        cx = cy = 0.5 + 2.0 * D
+        y, x = np.mgrid[0:ny, 0:nx]
+        fl = channel_flags(nx, ny)
+        fl[0][((x - cx) ** 2 + (y - cy) ** 2 <= D * D / 4.0) & (y > 0) & (y < ny - 1)] = Cell.solid
+        sim = Simulation(gpu, "fp32", nx=nx, ny=ny, nz=1, omega=wp, omega_minus=wm, uin=uin_max,
+                         ramp_steps=24_000, wants_forces=True, flags=fl)
+        sim.sponge = (float(nx - 1 - 3 * D), float(3 * D), 1.0)
+        box_x = (int(cx) - D // 2 - 3, int(cx) + D // 2 + 3)
+        box_y = (int(cy) - D // 2 - 3, int(cy) + D // 2 + 3)
+    sim.run(transient)
+    stride = 26
+    samples = int(D / (0.30 * u_mean) * sample_cycles / stride) + 64
+    cd, cl, t = [], [], []
+    for _ in range(samples):
+        sim.run(stride - 2)
+        f = sim.probe_force(box_x, box_y)
+        cd.append(2.0 * f[0] / (u_mean * u_mean * D))
+        cl.append(2.0 * f[1] / (u_mean * u_mean * D))
+        t.append(float(sim.steps_done))
+    return cycle_stats(cl, cd, t, D, u_mean), sim.state_digest
+
+
+def test_dfg_2d2_partial_cells_d64(gpu):
+    # runDFG2(D: 64, curved: true): transient 60,000 steps, 22 cycles sampled
+    st, _ = _dfg2(gpu, 64, 60_000, 22, curved=True)
+    assert st["cycles"] >= 15, st
+    assert 0.2950 <= st["strouhal"] <= 0.3050, st
+    assert 0.9900 <= st["max_cl"] <= 1.0100, st
+
+
+def test_dfg_2d2_digest_replay(gpu):
+    # runDFG2Replay: the unsteady run with its probe schedule, twice
+    a = _dfg2(gpu, 40, 30_000, 5, curved=False)
+    b = _dfg2(gpu, 40, 30_000, 5, curved=False)
+    assert a[1] == b[1]
+    assert a[0] == b[0]
