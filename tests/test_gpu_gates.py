@@ -186,3 +186,46 @@ def test_dfg_2d2_digest_replay(gpu):
     b = _dfg2(gpu, 40, 30_000, 5, curved=False)
     assert a[1] == b[1]
     assert a[0] == b[0]
+
+
+def _tgv1600_peak(gpu, n, u0=0.1, t_end=10.5):
+    """tgvRun of Cases.swift:1215-1282: TRT Lambda = 1/4 at Re 1600, E(t*)
+    every 0.1 t* (per-cell |u|^2 in FP32, summed in FP64 on the GPU),
+    eps = -dE/dt* by central differences, parabolic refinement at the peak."""
+    u0 = float(np.float32(u0))
+    lc = n / (2.0 * np.pi)
+    tau = 3.0 * u0 * lc / 1600.0 + 0.5
+    wp, wm = trt_omegas(tau, 0.25)
+    sim = Simulation(gpu, "fp32", nx=n, ny=n, nz=n, omega=wp, omega_minus=wm)
+    sim.init_field(2, u0)
+    tc = lc / u0
+    dt = 0.1
+    stride = max(2, int(tc * dt) & ~1)
+    e, t = [], []
+    for k in range(int(t_end / dt) + 1):
+        if k:
+            sim.run(stride)
+        e.append(sim.kinetic_energy() / (2.0 * sim.cells * u0 * u0))
+        t.append(sim.steps_done / tc)
+        assert np.isfinite(e[-1]), (n, t[-1])
+    eps = [(t[j], -(e[j + 1] - e[j - 1]) / (t[j + 1] - t[j - 1])) for j in range(1, len(e) - 1)]
+    k = max(range(len(eps)), key=lambda j: eps[j][1])
+    pt, pe = eps[k]
+    if 0 < k < len(eps) - 1:
+        y0, y1, y2 = eps[k - 1][1], eps[k][1], eps[k + 1][1]
+        den = y0 - 2 * y1 + y2
+        if abs(den) > 1e-12:
+            d = 0.5 * (y0 - y2) / den
+            pt, pe = eps[k][0] + d * dt, y1 - 0.25 * (y0 - y2) * d
+    return pt, pe
+
+
+def test_tgv1600_against_dns(gpu):
+    # runTGV1600 (Cases.swift:1301-1326) on its 288^3 / 320^3 pair; the DNS
+    # peak eps = 0.012856 at t* = 8.98 (Cases.swift:1214)
+    t288, e288 = _tgv1600_peak(gpu, 288)
+    t320, e320 = _tgv1600_peak(gpu, 320)
+    ref_t, ref_e = 8.98, 0.012856
+    assert abs(t320 - ref_t) <= 0.5, (t320, e320)
+    assert abs(e320 - e288) / ref_e <= 0.01, (e288, e320)
+    assert abs(e320 - ref_e) / ref_e <= 0.10, (t320, e320)
