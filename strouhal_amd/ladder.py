@@ -242,13 +242,10 @@ def main():
     print(json.dumps(report), flush=True)
 
 
-def credibility_report(variants, series, rungs, report, wall_s, base_umax) -> str:
-    """Ladder.credibility and its report (Ladder.swift:74-202) for mean C_D:
-    the Mach-1 rungs form the resolution ladder, the finest rung's half-Mach
-    copy is the compressibility anchor. The verification list holds this
-    run's own checks: finite series, the batched / sequential identity when
-    the sequential run was made, and the finest rung against the DFG 2D-2
-    reference intervals."""
+def credibility_run(variants, series, wall_s, base_umax, precision):
+    """Ladder.credibility (Ladder.swift:74-121) for mean C_D: the Mach-1 rungs
+    form the resolution ladder, the finest rung's half-Mach copy is the
+    compressibility anchor."""
     from strouhal_amd import credibility as cr
     ladder, half, fine = {}, None, max(v.D for v in variants)
     for v, s in zip(variants, series):
@@ -258,21 +255,30 @@ def credibility_report(variants, series, rungs, report, wall_s, base_umax) -> st
         elif v.D == fine:
             half = cd
     finest = next(v for v in variants if v.D == fine and v.mach_scale == 1.0)
-    st = next(r for r in rungs if r["D"] == fine and r["mach_scale"] == 1.0)
     ma = base_umax * 2.0 / 3.0 * math.sqrt(3.0)           # Ma of the mean inlet speed
+    setup = (f"DFG 2D-2 cylinder in a channel 22D x 4.1D, Re 100, parabolic velocity inlet ramped over "
+             f"24,000 steps, Noble-Torczynski cylinder, outlet sponge; D in {sorted(ladder)} at Ma {ma:.3f} "
+             f"plus a half-Mach anchor at D = {fine}; {precision} storage")
+    return cr.credibility("mean C_D", ladder, half, finest.case.sim.state_digest, re=100.0, finest_mach=ma,
+                          geometry="bluff body in channel", setup=setup, wall_seconds=wall_s)
+
+
+def credibility_report(variants, series, rungs, report, wall_s, base_umax) -> str:
+    """The credibility report (CredibilityReport.markdown, Ladder.swift:126-202)
+    of credibility_run. The verification list holds this run's own checks:
+    finite series, the batched / sequential identity when the sequential run
+    was made, and the finest rung against the DFG 2D-2 reference intervals."""
+    from strouhal_amd import credibility as cr
+    fine = max(v.D for v in variants)
+    st = next(r for r in rungs if r["D"] == fine and r["mach_scale"] == 1.0)
     gates = [f"every rung's force series finite: {'yes' if all(r['finite'] for r in rungs) else 'NO'}",
              f"finest rung (D = {fine}): St {st['strouhal']:.4f} (DFG 2D-2 interval 0.295-0.305), "
              f"max C_L {st['max_cl']:.4f} (0.99-1.01)"]
     if "bitwise_identical" in report:
         gates.append("batched and sequential ladders bit-identical (every DDF digest, every FP64 force "
                      f"sample): {'yes' if report['bitwise_identical'] else 'NO'}")
-    setup = (f"DFG 2D-2 cylinder in a channel 22D x 4.1D, Re 100, parabolic velocity inlet ramped over "
-             f"24,000 steps, Noble-Torczynski cylinder, outlet sponge; D in {sorted(ladder)} at Ma {ma:.3f} "
-             f"plus a half-Mach anchor at D = {fine}; {report['precision']} storage")
-    run = cr.credibility("mean C_D", ladder, half, finest.case.sim.state_digest, re=100.0, finest_mach=ma,
-                         geometry="bluff body in channel", setup=setup, wall_seconds=wall_s)
+    run = credibility_run(variants, series, wall_s, base_umax, report["precision"])
     return cr.markdown(run, gates)
-
 
 if __name__ == "__main__":
     main()

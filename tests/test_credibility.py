@@ -80,6 +80,11 @@ def test_validation_domain_verdicts():
     v = cr.classify(300.0, 0.1, 64, g)          # nearest: 2D-2, (300 - 100) / 50 = 4
     assert v.kind == "outside" and v.reasons == ["Re 300 outside validated 100–100"]
     assert cr.classify(100.0, 0.1, 64, "wing").kind == "outside"
+    # the reference's adversarial gate (Cases.swift:1146-1158): Re 250,000 at
+    # 48 cells per feature is outside, and its budget withholds the bar
+    far = cr.classify(250_000.0, 0.09, 48, g)
+    assert far.kind == "outside"
+    assert math.isnan(cr.Budget("C_D", 1.0, 0.01, 0.01, 0.01, far).combined)
 
 
 def test_budget_and_report():

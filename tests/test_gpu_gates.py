@@ -1,15 +1,19 @@
-"""GPU: the reference's 3-D and geometry gates on the hot path, restated with
-the same cases, step counts and thresholds (Sources/strouhal/Cases.swift).
+"""GPU: the reference's gates on the hot path (Sources/strouhal/Cases.swift,
+main.swift), restated with the same cases, step counts and thresholds.
 
 * z-periodic cavity: a 3-D lattice with 2-D physics reproduces Ghia Re 400
   and stays z-uniform (Cases.swift:642-686);
 * cubic cavity 64^3: mirror symmetry about the mid-z plane (690-725);
 * rotation: lid on +x moving +y is the standard cavity transposed (729-765);
-* sphere wake demo at Re 100: C_D inside the demo band (1043-1068).
+* sphere wake demo at Re 100: C_D inside the demo band (1043-1068);
+* cavities run to convergence against Ghia (167-279, main.swift:160-193);
+* DFG 2D-2 at D = 64 with partial cells, and its digest replay (840-999);
+* Taylor-Green Re 1600 against the DNS dissipation peak (1215-1326);
+* the credibility gate on the street ladder (1109-1174).
 
-These check the indexing of every axis, the wall and lid links and the
-partial-cell forces at full step counts; parity with the oracle is the
-business of test_gpu_parity.py.
+These check the indexing of every axis, the wall and lid links, the
+partial-cell forces and the observables at full step counts; parity with
+the oracle is the business of test_gpu_parity.py.
 """
 from __future__ import annotations
 
@@ -229,3 +233,21 @@ def test_tgv1600_against_dns(gpu):
     assert abs(t320 - ref_t) <= 0.5, (t320, e320)
     assert abs(e320 - e288) / ref_e <= 0.01, (e288, e320)
     assert abs(e320 - ref_e) / ref_e <= 0.10, (t320, e320)
+
+
+def test_credibility_gate(gpu):
+    # runCredibility (Cases.swift:1109-1174): the street ladder on its default
+    # rungs D in {32, 48, 64} with the half-Mach anchor at D = 64, full
+    # schedule, batched: a finite bar in-domain that covers the published
+    # mean C_D (~3.18, featflow level 6) within U + 0.05, every rung recorded.
+    import math
+    from strouhal_amd.ladder import build, credibility_run, run_batched
+    base_umax = 1.5 * 0.1 / math.sqrt(3.0)
+    vb = build(gpu, "fp32", [32, 48, 64], 1.0, base_umax)
+    series, wall, _, _ = run_batched(vb)
+    run = credibility_run(vb, series, wall, base_umax, "fp32")
+    b = run.budget
+    assert np.isfinite(b.combined) and b.combined > 0, b.headline
+    assert abs(b.value - 3.18) <= b.combined + 0.05, b.headline
+    assert [r.cells_per_feature for r in run.rungs] == [32, 48, 64]
+    assert np.isfinite(run.mach_half)
