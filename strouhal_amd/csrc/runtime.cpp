@@ -1058,7 +1058,11 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             if (rc) return rc;
             while (K - done >= B->graphSteps) {
                 hipError_t e = hipGraphLaunch(g, s);
-                if (e != hipSuccess) { hipGraphExecDestroy(g); return hip_fail(e, "graph launch"); }
+                if (e != hipSuccess) {
+                    (void)hipStreamSynchronize(s);   // earlier replays of g may still run
+                    hipGraphExecDestroy(g);
+                    return hip_fail(e, "graph launch");
+                }
                 done += B->graphSteps;
             }
             // the exec may be destroyed once its launches are queued
