@@ -239,12 +239,13 @@ typedef struct strh_objective {
 } strh_objective;
 
 typedef struct strh_vjp_info {
-    int64_t checkpoint_every;   /* steps per recomputed segment */
-    int64_t checkpoints;        /* DDF images kept */
+    int64_t checkpoint_every;   /* steps per recorded segment */
+    int64_t checkpoints;        /* DDF images kept (both levels) */
     double forward_seconds;     /* first forward pass (objective, checkpoints) */
-    double recompute_seconds;   /* segment replays that record the states */
+    double recompute_seconds;   /* segment replays (both levels) */
     double backward_seconds;    /* reverse steps */
     uint64_t workspace_bytes;   /* device memory of the call beyond the lattice */
+    int64_t outer_every;        /* steps per outer segment (= checkpoint_every: one level) */
 } strh_vjp_info;
 
 /* Advance T steps and return J (FP64, fixed-order reduction). */
@@ -266,8 +267,10 @@ STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, d
  * collision's Jacobian depends on the state only through them), and runs the
  * reverse steps over it. The last segment is recorded in the forward pass, so
  * K >= T replays nothing. K = 0: ceil(sqrt(T)); K = -1: the largest K whose
- * recording and checkpoints fit 80% of the free device memory (the K needing
- * least memory when none fits). The adjoint state is FP32;
+ * recording and checkpoints fit 80% of the free device memory; when none
+ * fits, the two-level plan of strh_vjp_nested with outer and inner segments
+ * of ceil(T^(1/3))^2 and ceil(T^(1/3)) steps if that fits (the single-level
+ * K needing least memory otherwise). The adjoint state is FP32;
  * every accumulation is FP64 in a fixed order, so the result does not depend
  * on K. Device memory: (ceil(T/K) - 1) DDF images + K * N * 16 bytes (19 * N
  * stored values under LES) +
@@ -275,6 +278,20 @@ STRH_API int strh_objective_eval(strh_lattice* lat, const strh_objective* obj, d
 STRH_API int strh_vjp(strh_lattice* lat, const strh_objective* obj, int64_t checkpoint_every,
                       double* J, double* grad_alpha, double* grad_eps, double* grad_uin,
                       strh_vjp_info* info);
+/* strh_vjp with two checkpoint levels, for horizons whose single-level
+ * recording and checkpoints do not fit device memory (no reference
+ * counterpart). The forward pass keeps the DDFs at the start of every outer
+ * segment of outer_every steps; the reverse pass replays each outer segment
+ * once, keeping the DDFs at the start of every inner segment of inner_every
+ * steps, then replays each inner segment again, recording it, and runs its
+ * reverse steps. Each step is replayed twice instead of once; device memory
+ * holds (ceil(T/outer) - 1) + (ceil(outer/inner) - 1) DDF images and inner
+ * recorded steps. inner_every >= outer_every is the single-level plan with
+ * K = outer_every. J, the gradients and the final state are the same bits as
+ * strh_vjp's for any outer_every, inner_every >= 1. */
+STRH_API int strh_vjp_nested(strh_lattice* lat, const strh_objective* obj, int64_t outer_every,
+                             int64_t inner_every, double* J, double* grad_alpha, double* grad_eps,
+                             double* grad_uin, strh_vjp_info* info);
 
 /* ---- host utilities (no counterpart in the reference, which hashes with
  * CryptoKit): SHA-256 of a host buffer, and a switch that forces the portable

@@ -33,6 +33,7 @@ EXPORTS = [
     "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
     "strh_batch_set_block_size", "strh_batch_autotune",
     "strh_batch_set_graph_steps", "strh_batch_gpu_seconds", "strh_objective_eval", "strh_vjp",
+    "strh_vjp_nested",
 ]
 
 
@@ -67,6 +68,7 @@ class VjpInfo(ctypes.Structure):
         ("checkpoint_every", ctypes.c_int64), ("checkpoints", ctypes.c_int64),
         ("forward_seconds", ctypes.c_double), ("recompute_seconds", ctypes.c_double),
         ("backward_seconds", ctypes.c_double), ("workspace_bytes", ctypes.c_uint64),
+        ("outer_every", ctypes.c_int64),
     ]
 
 
@@ -143,6 +145,8 @@ def lib():
         "strh_objective_eval": [vp, ctypes.POINTER(Objective), ctypes.POINTER(ctypes.c_double)],
         "strh_vjp": [vp, ctypes.POINTER(Objective), i64, ctypes.POINTER(ctypes.c_double), vp, vp, vp,
                      ctypes.POINTER(VjpInfo)],
+        "strh_vjp_nested": [vp, ctypes.POINTER(Objective), i64, i64, ctypes.POINTER(ctypes.c_double), vp, vp,
+                            vp, ctypes.POINTER(VjpInfo)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

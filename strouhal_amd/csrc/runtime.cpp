@@ -1454,47 +1454,46 @@ int strh_objective_eval(strh_lattice* L, const strh_objective* o, double* J) {
     return STRH_OK;
 }
 
-int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, double* gradAlpha,
-             double* gradEps, double* gradUin, strh_vjp_info* info) {
-    int rc = check_lat(L);
-    if (rc) return rc;
-    if (!J) return fail(STRH_ERR_INVALID, "null J");
-    AdjPlan P;
-    if ((rc = adj_plan(L, o, P))) return rc;
+}  // extern "C"
+
+namespace {
+
+// Memory of a VJP plan beyond the fixed buffers: K2 recorded steps, the outer
+// checkpoints (one DDF image per outer segment but the last) and the inner
+// ones (one per inner segment of an outer segment but the last).
+struct VjpPlan {
+    int64_t K1 = 0, K2 = 0;
+};
+
+double vjp_plan_bytes(int64_t T, int64_t K1, int64_t K2, double perStep, double ddf) {
+    const int64_t C1 = (T + K1 - 1) / K1, C2 = (K1 + K2 - 1) / K2;
+    return (double)K2 * perStep + (double)(C1 - 1 + C2 - 1) * ddf;
+}
+
+// The adjoint over T steps with two checkpoint levels. The forward pass keeps
+// the DDFs at the start of every outer segment of K1 steps. The reverse pass
+// takes the outer segments last to first; each is replayed once, keeping the
+// DDFs at the start of every inner segment of K2 steps and recording the last
+// inner segment; the others are replayed from their checkpoints, recording,
+// last to first, and the reverse steps run over each recording. K2 = K1 is
+// the single-level scheme (one replay per step); K2 < K1 replays each step
+// twice and needs ceil(T/K1) - 1 + ceil(K1/K2) - 1 images and K2 recorded
+// steps instead of ceil(T/K1) - 1 images and K1 recorded steps. The last
+// outer segment's inner checkpoints come from the forward pass itself. Every
+// replay runs the forward pass's kernel from the forward pass's bits, so the
+// recordings, and with them J and the gradients, do not depend on K1 or K2.
+int vjp_run(strh_lattice* L, const strh_objective* o, const AdjPlan& P, VjpPlan plan, double* J,
+            double* gradAlpha, double* gradEps, double* gradUin, strh_vjp_info* info) {
+    int rc = STRH_OK;
     const int64_t T = P.T;
+    int64_t K1 = std::min(plan.K1, T);
+    int64_t K2 = std::min(plan.K2, K1);
     const size_t N = L->N, S = L->S;
     const size_t ddf = ddf_bytes(L);
     const size_t M = P.ntCells.size();
     const bool les = L->cSmago > 0.0f;
-    if (K < -1) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= -1");
-    if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
-    if (K == -1) {
-        // fewest replays: the longest segment whose recording fits half the
-        // free device memory left after the fixed buffers
-        size_t freeB = 0, totalB = 0;
-        HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
-        const double fixed = (double)N * (3 * 8 + 4) + 2.0 * 19 * S * 4 + 2.0 * ddf;
-        const double budget = 0.8 * (double)freeB - fixed;
-        // recording per step (moments and partial-cell populations, or all
-        // populations under LES); K steps of it plus ceil(T/K) - 1 checkpoints.
-        // When no K fits, the one that needs least (the allocation then
-        // reports the shortfall).
-        const double perStep = les ? (double)N * 19 * L->B : (double)N * 16 + (double)M * 76;
-        auto need = [&](int64_t k) { return (double)k * perStep + (double)((T + k - 1) / k - 1) * (double)ddf; };
-        int64_t best = 1;
-        K = 0;
-        for (int64_t k = std::min<int64_t>(T, budget > perStep ? (int64_t)(budget / perStep) : 1); k >= 1; k--) {
-            if (need(k) <= budget) { K = k; break; }
-            if (need(k) < need(best)) best = k;
-        }
-        if (K == 0) {
-            for (int64_t k = 1; k <= T && (double)k * perStep <= need(best); k++)
-                if (need(k) < need(best)) best = k;
-            K = best;
-        }
-    }
-    if (K > T) K = T;
-    const int64_t C = (T + K - 1) / K;
+    const int64_t C1 = (T + K1 - 1) / K1;
+    const int64_t C2 = (K1 + K2 - 1) / K2;
     strh_ctx* c = L->ctx;
     hipStream_t st = c->stream;
 
@@ -1503,18 +1502,19 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     float4* rec = nullptr;
     uint8_t* rec19 = nullptr;
     float *lam0 = nullptr, *lam1 = nullptr, *zeroAlpha = nullptr;
-    uint8_t *ck = nullptr, *work = nullptr;
+    uint8_t *ck = nullptr, *ick = nullptr, *work = nullptr;
     if ((rc = W.get(&jacc, N * 8, "objective accumulator"))) return rc;
     if ((rc = W.get(&gA, N * 8, "porosity gradient"))) return rc;
     if ((rc = W.get(&gU, N * 8, "inlet gradient"))) return rc;
     if ((rc = W.get(&lam0, 19 * S * 4, "adjoint state"))) return rc;
     if ((rc = W.get(&lam1, 19 * S * 4, "adjoint state"))) return rc;
     if (les) {
-        if ((rc = W.get(&rec19, (size_t)K * 19 * N * L->B, "recorded populations"))) return rc;
+        if ((rc = W.get(&rec19, (size_t)K2 * 19 * N * L->B, "recorded populations"))) return rc;
     } else {
-        if ((rc = W.get(&rec, (size_t)K * N * 16, "recorded moments"))) return rc;
+        if ((rc = W.get(&rec, (size_t)K2 * N * 16, "recorded moments"))) return rc;
     }
-    if ((rc = W.get(&ck, (size_t)(C > 1 ? C - 1 : 1) * ddf, "checkpoints"))) return rc;
+    if ((rc = W.get(&ck, (size_t)(C1 > 1 ? C1 - 1 : 1) * ddf, "checkpoints"))) return rc;
+    if (C2 > 1 && (rc = W.get(&ick, (size_t)(C2 - 1) * ddf, "inner checkpoints"))) return rc;
     if ((rc = W.get(&work, ddf, "replay DDFs"))) return rc;
     const float* alpha = L->alpha;
     if (!alpha && !L->useEps && !les) {
@@ -1528,7 +1528,7 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     if (L->useEps) {
         const size_t m = std::max<size_t>(M, 1);
         if ((rc = W.get(&ntIdx, N * 4, "partial-cell index"))) return rc;
-        if (!les && (rc = W.get(&recNt, (size_t)K * 19 * m * 4, "recorded partial cells"))) return rc;
+        if (!les && (rc = W.get(&recNt, (size_t)K2 * 19 * m * 4, "recorded partial cells"))) return rc;
         if ((rc = W.get(&gE, m * 8, "solid-fraction gradient"))) return rc;
         HIP_TRY(hipMemcpy(ntIdx, P.ntIdx.data(), N * 4, hipMemcpyHostToDevice), "copy index");
         HIP_TRY(launch_fill_zero(gE, m * 8, st), "clear");
@@ -1539,16 +1539,37 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     HIP_TRY(launch_fill_zero(lam1, 19 * S * 4, st), "clear");
     const int64_t t0 = L->stepsDone;
 
-    // forward pass: checkpoints at the start of segments 0 .. C-2, the last
-    // segment recorded directly
+    // Steps [k0, k0 + len) of an outer segment on the DDFs f: the inner
+    // checkpoints at the start of every inner segment but the last, which is
+    // recorded (jacc: accumulate the objective, first forward pass only).
+    auto advance_outer = [&](void* f, int64_t k0, int64_t len, double* acc) -> int {
+        const int64_t n2 = (len + K2 - 1) / K2;
+        for (int64_t j = 0; j < n2; j++) {
+            const int64_t kk = k0 + j * K2, l2 = std::min(K2, len - j * K2);
+            const bool lastInner = j == n2 - 1;
+            if (!lastInner)
+                HIP_TRY(hipMemcpyAsync(ick + (size_t)j * ddf, f, ddf, hipMemcpyDeviceToDevice, st),
+                        "inner checkpoint");
+            int r = adj_forward(L, f, alpha, t0, kk, l2, o, P, lastInner ? rec : nullptr, acc, ntIdx,
+                                lastInner ? recNt : nullptr, lastInner ? rec19 : nullptr);
+            if (r) return r;
+        }
+        return STRH_OK;
+    };
+
+    // forward pass: outer checkpoints at the start of outer segments
+    // 0 .. C1-2; the last outer segment lays down its inner checkpoints and
+    // records its last inner segment
     HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
-    for (int64_t seg = 0; seg < C; seg++) {
-        const int64_t k0 = seg * K, len = std::min(K, T - k0);
-        const bool last = seg == C - 1;
-        if (!last)
+    for (int64_t seg = 0; seg < C1; seg++) {
+        const int64_t k0 = seg * K1, len = std::min(K1, T - k0);
+        if (seg != C1 - 1) {
             HIP_TRY(hipMemcpyAsync(ck + (size_t)seg * ddf, L->f, ddf, hipMemcpyDeviceToDevice, st), "checkpoint");
-        if ((rc = adj_forward(L, L->f, alpha, t0, k0, len, o, P, last ? rec : nullptr, jacc, ntIdx,
-                              last ? recNt : nullptr, last ? rec19 : nullptr))) return rc;
+            if ((rc = adj_forward(L, L->f, alpha, t0, k0, len, o, P, nullptr, jacc, ntIdx, nullptr, nullptr)))
+                return rc;
+        } else if ((rc = advance_outer(L->f, k0, len, jacc))) {
+            return rc;
+        }
     }
     HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
     HIP_TRY(hipEventSynchronize(c->ev1), "forward pass");
@@ -1556,55 +1577,69 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
     L->gpuSeconds += tf;
     L->stepsDone += T;
 
-    // reverse pass, one segment at a time
+    // reverse pass, one inner segment at a time
     float* lamN = lam0;
     float* lamT = lam1;
-    for (int64_t seg = C - 1; seg >= 0; seg--) {
-        const int64_t k0 = seg * K, len = std::min(K, T - k0);
-        if (seg != C - 1) {
+    for (int64_t seg = C1 - 1; seg >= 0; seg--) {
+        const int64_t k0 = seg * K1, len = std::min(K1, T - k0);
+        const int64_t n2 = (len + K2 - 1) / K2;
+        if (seg != C1 - 1) {
             HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
             HIP_TRY(hipMemcpyAsync(work, ck + (size_t)seg * ddf, ddf, hipMemcpyDeviceToDevice, st), "restore");
-            if ((rc = adj_forward(L, work, alpha, t0, k0, len, o, P, rec, nullptr, ntIdx, recNt, rec19))) return rc;
+            if ((rc = advance_outer(work, k0, len, nullptr))) return rc;
             HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
             HIP_TRY(hipEventSynchronize(c->ev1), "replay");
             tr += elapsed(c);
         }
-        HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
-        for (int64_t k = k0 + len - 1; k >= k0; k--) {
-            const int64_t t = t0 + k;
-            const StepParams sp = make_params(L, t, false);
-            AdjStep q;
-            memset(&q, 0, sizeof(q));
-            q.nx = (uint32_t)L->nx; q.ny = (uint32_t)L->ny; q.nz = (uint32_t)L->nz;
-            q.N = L->N; q.S = L->S;
-            q.divx = sp.divx; q.divy = sp.divy;
-            q.wp = sp.omega; q.wm = sp.omegaMinus;
-            q.fx = sp.fx; q.fy = sp.fy; q.fz = sp.fz;
-            q.rOdd = (double)ramp(L, (t & 1) ? t : t + 1);
-            q.uinTarget = L->uinTarget;
-            q.inflowUniform = L->inflowUniform ? 1 : 0;
-            const bool win = k >= o->spinup;
-            q.inletX = o->inlet_x; q.outletX = o->outlet_x;
-            q.cin = win ? (float)P.cin : 0.0f;
-            q.cout = win ? (float)P.cout : 0.0f;
-            q.wd = win ? (float)P.wd : 0.0f;
-            q.wf = win ? (float)P.wf : 0.0f;
-            q.M = (uint32_t)M;
-            q.spongeX0 = sp.spongeX0; q.spongeInvW = sp.spongeInvW; q.spongeTau = sp.spongeTau;
-            q.cSmago = sp.cSmago;
-            q.srt = sp.omegaMinus == sp.omega ? 1 : 0;
-            q.precision = L->precision;
-            HIP_TRY(launch_adjoint_step(q, L->walls, rec ? rec + (size_t)(k - k0) * N : nullptr, lamN, lamT,
-                                        L->flags, L->link, alpha, gA, gU, L->useEps ? L->eps : nullptr,
-                                        ntIdx, recNt ? recNt + (size_t)(k - k0) * 19 * M : nullptr, gE,
-                                        rec19 ? rec19 + (size_t)(k - k0) * 19 * N * L->B : nullptr, L->lidm,
-                                        sp, st),
-                    "adjoint step launch");
-            std::swap(lamN, lamT);
+        for (int64_t j = n2 - 1; j >= 0; j--) {
+            const int64_t kk = k0 + j * K2, l2 = std::min(K2, len - j * K2);
+            if (j != n2 - 1) {
+                HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
+                HIP_TRY(hipMemcpyAsync(work, ick + (size_t)j * ddf, ddf, hipMemcpyDeviceToDevice, st),
+                        "restore inner");
+                if ((rc = adj_forward(L, work, alpha, t0, kk, l2, o, P, rec, nullptr, ntIdx, recNt, rec19)))
+                    return rc;
+                HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
+                HIP_TRY(hipEventSynchronize(c->ev1), "replay");
+                tr += elapsed(c);
+            }
+            HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
+            for (int64_t k = kk + l2 - 1; k >= kk; k--) {
+                const int64_t t = t0 + k;
+                const StepParams sp = make_params(L, t, false);
+                AdjStep q;
+                memset(&q, 0, sizeof(q));
+                q.nx = (uint32_t)L->nx; q.ny = (uint32_t)L->ny; q.nz = (uint32_t)L->nz;
+                q.N = L->N; q.S = L->S;
+                q.divx = sp.divx; q.divy = sp.divy;
+                q.wp = sp.omega; q.wm = sp.omegaMinus;
+                q.fx = sp.fx; q.fy = sp.fy; q.fz = sp.fz;
+                q.rOdd = (double)ramp(L, (t & 1) ? t : t + 1);
+                q.uinTarget = L->uinTarget;
+                q.inflowUniform = L->inflowUniform ? 1 : 0;
+                const bool win = k >= o->spinup;
+                q.inletX = o->inlet_x; q.outletX = o->outlet_x;
+                q.cin = win ? (float)P.cin : 0.0f;
+                q.cout = win ? (float)P.cout : 0.0f;
+                q.wd = win ? (float)P.wd : 0.0f;
+                q.wf = win ? (float)P.wf : 0.0f;
+                q.M = (uint32_t)M;
+                q.spongeX0 = sp.spongeX0; q.spongeInvW = sp.spongeInvW; q.spongeTau = sp.spongeTau;
+                q.cSmago = sp.cSmago;
+                q.srt = sp.omegaMinus == sp.omega ? 1 : 0;
+                q.precision = L->precision;
+                HIP_TRY(launch_adjoint_step(q, L->walls, rec ? rec + (size_t)(k - kk) * N : nullptr, lamN, lamT,
+                                            L->flags, L->link, alpha, gA, gU, L->useEps ? L->eps : nullptr,
+                                            ntIdx, recNt ? recNt + (size_t)(k - kk) * 19 * M : nullptr, gE,
+                                            rec19 ? rec19 + (size_t)(k - kk) * 19 * N * L->B : nullptr, L->lidm,
+                                            sp, st),
+                        "adjoint step launch");
+                std::swap(lamN, lamT);
+            }
+            HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
+            HIP_TRY(hipEventSynchronize(c->ev1), "reverse steps");
+            tb += elapsed(c);
         }
-        HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
-        HIP_TRY(hipEventSynchronize(c->ev1), "reverse steps");
-        tb += elapsed(c);
     }
     if ((rc = reduce_to_host(L, jacc, J))) return rc;
     if (gradUin && (rc = reduce_to_host(L, gU, gradUin))) return rc;
@@ -1618,15 +1653,91 @@ int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, dou
         }
     }
     if (info) {
-        info->checkpoint_every = K;
-        info->checkpoints = C - 1;
+        info->checkpoint_every = K2;
+        info->checkpoints = (C1 - 1) + (C2 - 1);
         info->forward_seconds = tf;
         info->recompute_seconds = tr;
         info->backward_seconds = tb;
         info->workspace_bytes = W.bytes;
+        info->outer_every = K1;
     }
     if (!std::isfinite(*J)) return fail(STRH_ERR_NOT_FINITE, "objective is not finite");
     return STRH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int strh_vjp(strh_lattice* L, const strh_objective* o, int64_t K, double* J, double* gradAlpha,
+             double* gradEps, double* gradUin, strh_vjp_info* info) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!J) return fail(STRH_ERR_INVALID, "null J");
+    AdjPlan P;
+    if ((rc = adj_plan(L, o, P))) return rc;
+    const int64_t T = P.T;
+    const size_t N = L->N, S = L->S;
+    const double ddf = (double)ddf_bytes(L);
+    const size_t M = P.ntCells.size();
+    const bool les = L->cSmago > 0.0f;
+    if (K < -1) return fail(STRH_ERR_INVALID, "checkpoint_every must be >= -1");
+    VjpPlan plan;
+    if (K == 0) K = (int64_t)std::ceil(std::sqrt((double)T));
+    if (K > 0) {
+        plan.K1 = plan.K2 = std::min(K, T);
+    } else {
+        // fewest replays: the longest single-level segment whose recording
+        // and checkpoints fit 80% of the free device memory left after the
+        // fixed buffers; when none fits, two levels with outer and inner
+        // segments of about T^(2/3) and T^(1/3) steps (each step replayed
+        // twice, ~3 T^(1/3) images); when that does not fit either, the
+        // single-level plan needing least (the allocation reports the shortfall)
+        size_t freeB = 0, totalB = 0;
+        HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+        const double fixed = (double)N * (3 * 8 + 4) + 2.0 * 19 * S * 4 + 2.0 * ddf;
+        const double budget = 0.8 * (double)freeB - fixed;
+        // recording per step (moments and partial-cell populations, or all
+        // populations under LES)
+        const double perStep = les ? (double)N * 19 * L->B : (double)N * 16 + (double)M * 76;
+        auto need = [&](int64_t k) { return vjp_plan_bytes(T, k, k, perStep, ddf); };
+        int64_t best = 1;
+        K = 0;
+        for (int64_t k = std::min<int64_t>(T, budget > perStep ? (int64_t)(budget / perStep) : 1); k >= 1; k--) {
+            if (need(k) <= budget) { K = k; break; }
+            if (need(k) < need(best)) best = k;
+        }
+        if (K > 0) {
+            plan.K1 = plan.K2 = K;
+        } else {
+            const int64_t k2 = std::max<int64_t>(1, (int64_t)std::ceil(std::cbrt((double)T)));
+            const int64_t k1 = std::min<int64_t>(T, k2 * k2);
+            if (vjp_plan_bytes(T, k1, k2, perStep, ddf) <= budget) {
+                plan.K1 = k1;
+                plan.K2 = k2;
+            } else {
+                for (int64_t k = 1; k <= T && (double)k * perStep <= need(best); k++)
+                    if (need(k) < need(best)) best = k;
+                plan.K1 = plan.K2 = best;
+            }
+        }
+    }
+    return vjp_run(L, o, P, plan, J, gradAlpha, gradEps, gradUin, info);
+}
+
+int strh_vjp_nested(strh_lattice* L, const strh_objective* o, int64_t outer_every, int64_t inner_every,
+                    double* J, double* gradAlpha, double* gradEps, double* gradUin, strh_vjp_info* info) {
+    int rc = check_lat(L);
+    if (rc) return rc;
+    if (!J) return fail(STRH_ERR_INVALID, "null J");
+    if (outer_every < 1 || inner_every < 1)
+        return fail(STRH_ERR_INVALID, "outer_every and inner_every must be >= 1");
+    AdjPlan P;
+    if ((rc = adj_plan(L, o, P))) return rc;
+    VjpPlan plan;
+    plan.K1 = std::min(outer_every, P.T);
+    plan.K2 = std::min(inner_every, plan.K1);
+    return vjp_run(L, o, P, plan, J, gradAlpha, gradEps, gradUin, info);
 }
 
 }  // extern "C"

@@ -297,21 +297,29 @@ class Simulation:
 
     def vjp(self, inlet_x: int, outlet_x: int, spinup: int, avg: int, w_p: float = 1.0,
             w_d: float = 1.0, checkpoint_every: int = -1, want_alpha: bool = True,
-            w_f: float = 0.0, want_eps: bool = False):
+            w_f: float = 0.0, want_eps: bool = False, inner_every: int | None = None):
         """Discrete adjoint of `objective`: returns (J, dJ/dalpha per cell as
         float64 (nz, ny, nx) or None, dJ/d uin_target, info dict); with
         want_eps, info["grad_eps"] holds dJ/deps (nz, ny, nx) on the cells with
         a solid fraction. Leaves the lattice as `run(spinup + avg)` would.
         checkpoint_every: steps per replayed segment; 0 = ceil(sqrt(T)),
-        -1 = as long as memory allows."""
+        -1 = as long as memory allows (two levels when one does not fit).
+        inner_every: two checkpoint levels (strh_vjp_nested), outer segments of
+        checkpoint_every steps replayed once to checkpoint inner segments of
+        inner_every steps."""
         from ._lib import VjpInfo
         o = self._objective(inlet_x, outlet_x, spinup, avg, w_p, w_d, w_f)
         J, gu, info = ctypes.c_double(), ctypes.c_double(), VjpInfo()
         ga = np.zeros(self.cells, dtype=np.float64) if want_alpha else None
         ge = np.zeros(self.cells, dtype=np.float64) if want_eps else None
         ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
-        check(lib().strh_vjp(self._h, ctypes.byref(o), int(checkpoint_every), ctypes.byref(J),
-                             ptr(ga), ptr(ge), ctypes.byref(gu), ctypes.byref(info)))
+        if inner_every is None:
+            check(lib().strh_vjp(self._h, ctypes.byref(o), int(checkpoint_every), ctypes.byref(J),
+                                 ptr(ga), ptr(ge), ctypes.byref(gu), ctypes.byref(info)))
+        else:
+            check(lib().strh_vjp_nested(self._h, ctypes.byref(o), int(checkpoint_every), int(inner_every),
+                                        ctypes.byref(J), ptr(ga), ptr(ge), ctypes.byref(gu),
+                                        ctypes.byref(info)))
         meta = {k: getattr(info, k) for k, _ in VjpInfo._fields_}
         if want_eps:
             meta["grad_eps"] = ge.reshape(self.nz, self.ny, self.nx)

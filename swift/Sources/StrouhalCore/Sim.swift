@@ -214,6 +214,20 @@ public final class Simulation {                   // Sim.swift:153-459
         try check(strh_vjp(lat, &oc, Int64(checkpointEvery), &j, &ga, &ge, &gu, &info))
         return (j, ga, ge, gu)
     }
+
+    /// vjp with two checkpoint levels (strh_vjp_nested): outer segments of
+    /// outerEvery steps replayed once to checkpoint inner segments of
+    /// innerEvery steps, for horizons whose one-level plan does not fit.
+    public func vjpNested(_ o: Objective, outerEvery: Int, innerEvery: Int) throws
+        -> (j: Double, gradAlpha: [Double], gradEps: [Double], gradUin: Double) {
+        var oc = o.c
+        var j = 0.0, gu = 0.0
+        var ga = [Double](repeating: 0, count: cells)
+        var ge = [Double](repeating: 0, count: cells)
+        var info = strh_vjp_info()
+        try check(strh_vjp_nested(lat, &oc, Int64(outerEvery), Int64(innerEvery), &j, &ga, &ge, &gu, &info))
+        return (j, ga, ge, gu)
+    }
 }
 
 /// Many lattices advanced together (replaces Ladder's sequential rung loop,

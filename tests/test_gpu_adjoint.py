@@ -311,3 +311,38 @@ def test_vjp_les_16bit_storage_near_fp64_oracle(gpu, precision):
     err = np.max(np.abs(ge - ger)) / np.max(np.abs(ger))
     assert err <= tol, err
     assert np.all(ge[eps.reshape(-1) == 0] == 0.0)
+
+
+@pytest.mark.parametrize("kind", ["porous", "partial_cells", "les"])
+def test_two_level_checkpoints_match_one_level(gpu, kind):
+    """strh_vjp_nested: outer segments replayed once to lay down inner
+    checkpoints, inner segments replayed again while recording. Every replay
+    starts from the forward pass's bits, so J, the gradients and the final
+    state are the single-level plan's bits for any outer / inner lengths,
+    ragged ones included."""
+    if kind == "porous":
+        fl = _channel(18, 10)
+        make = lambda: _pair(gpu, "fp32", fl, _alpha(fl.shape, 1))[0]
+        obj, kw, plans = OBJ, {}, [(8, 3), (17, 4), (6, 1), (5, 5), (4, 9)]
+    elif kind == "partial_cells":
+        make = lambda: _nt_pair(gpu, "fp32")[0]
+        obj, kw, plans = NT_OBJ, {"want_eps": True}, [(6, 2), (15, 4), (7, 3)]
+    else:
+        make = lambda: _les_pair(gpu, "fp16s", 3.0 / 16.0, True)[0]
+        obj, kw, plans = LES_OBJ, {"want_eps": True}, [(5, 2), (13, 3), (4, 1)]
+    ref_sim = make()
+    J0, ga0, gu0, i0 = ref_sim.vjp(**obj, checkpoint_every=3, **kw)
+    for outer, inner in plans:
+        s = make()
+        J, ga, gu, info = s.vjp(**obj, checkpoint_every=outer, inner_every=inner, **kw)
+        assert J == J0 and gu == gu0, (outer, inner)
+        if ga0 is not None:
+            assert np.array_equal(ga, ga0), (outer, inner)
+        if "grad_eps" in i0:
+            assert np.array_equal(info["grad_eps"], i0["grad_eps"]), (outer, inner)
+        assert s.state_digest == ref_sim.state_digest
+        T = obj["spinup"] + obj["avg"]
+        o_eff = min(outer, T)
+        i_eff = min(inner, o_eff)
+        assert info["outer_every"] == o_eff and info["checkpoint_every"] == i_eff
+        assert info["checkpoints"] == (-(-T // o_eff) - 1) + (-(-o_eff // i_eff) - 1)
