@@ -19,6 +19,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libstrouhal_hip.so")
+# The streaming-ceiling measurement kernel (profile_workload.py --stream-ceiling)
+# lives in a library of its own, outside the solver's C ABI.
+STREAM_LIB = os.path.join(HERE, "libstrouhal_stream.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
@@ -53,6 +56,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         o = os.path.join(BUILD, u + ".o")
         cmds.append([hipcc, *DEVICE_FLAGS, inc, "-c", os.path.join(CSRC, u + ".hip"), "-o", o])
         objs.append(o)
+    stream_obj = os.path.join(BUILD, "stream_ceiling.o")
+    cmds.append([hipcc, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "stream_ceiling.hip"), "-o", stream_obj])
     for u in ("runtime", "sha256"):
         o = os.path.join(BUILD, u + ".o")
         cmds.append(["g++", *HOST_FLAGS, inc, "-c", os.path.join(CSRC, u + ".cpp"), "-o", o])
@@ -65,6 +70,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-lpthread"], verbose)
     os.replace(tmp, LIB)
+    tmp = STREAM_LIB + ".tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", stream_obj, "-o", tmp], verbose)
+    os.replace(tmp, STREAM_LIB)
     return LIB
 
 

@@ -18,6 +18,12 @@ Usage (under rocprofv3, or alone):
     python3 strouhal_amd/profile_workload.py --precision fp16s --dispatch --warmup 5 --steps 400
     python3 strouhal_amd/profile_workload.py --precision fp16s --replay 60 --warmup 4
     python3 strouhal_amd/profile_workload.py --precision fp32 --copy 50
+    python3 strouhal_amd/profile_workload.py --precision fp32 --stream-ceiling
+
+--stream-ceiling times a kernel that moves the step's bytes in the step's
+pattern (19 slots read, 19 written, blocked layout, nontemporal) and does no
+arithmetic, at 4, 8 and 16 bytes per lane access (libstrouhal_stream.so):
+the streaming bound of a step on this lattice.
 """
 from __future__ import annotations
 
@@ -56,14 +62,32 @@ def main():
     ap.add_argument("--replay", type=int, default=0, help="steps per pass of the snapshot replay")
     ap.add_argument("--copy", type=int, default=0,
                     help="time this many device-to-device copies of the DDF image (a bandwidth ceiling)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="time the no-arithmetic 19-slot pass over an n^3 image (see above)")
     ap.add_argument("--case", default="tgv", choices=["tgv", "sphere", "street", "cavity", "channel"],
                     help="tgv: the bench's n^3 Taylor-Green vortex; sphere: the 512x256x256 LES "
                          "sphere (D 40, NT cells, inlets, sponge); street: the ladder's finest "
                          "rung (D 64 at half Mach 0.1); cavity: the 128^2 Re 400 lid-driven "
                          "cavity with its wall frame (SRT); channel: the 512x256 porous channel")
     args = ap.parse_args()
-    from strouhal_amd import Simulation, trt_omegas
     n = args.n
+    if args.stream_ceiling:
+        import ctypes
+        sl = ctypes.CDLL(os.path.join(ROOT, "strouhal_amd", "libstrouhal_stream.so"))
+        sl.strh_stream_ceiling.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_float)]
+        B = 4 if args.precision == "fp32" else 2
+        cells = n ** 3
+        out = {"case": "stream_ceiling", "precision": args.precision, "n": n, "bytes_per_pass": 38 * cells * B}
+        for lane in (4, 8, 16):
+            ms = ctypes.c_float()
+            rc = sl.strh_stream_ceiling(cells, B, lane, 50, ctypes.byref(ms))
+            if rc != 0:
+                raise RuntimeError(f"strh_stream_ceiling failed ({rc})")
+            out[f"lane{lane}"] = {"ms": round(ms.value, 4), "TBps": round(38 * cells * B / ms.value / 1e9, 3)}
+        print(json.dumps(out), flush=True)
+        return
+    from strouhal_amd import Simulation, trt_omegas
     lam = 3.0 / 16.0 if args.precision == "fp32" else None
     if args.case == "sphere":
         from strouhal_amd.cases import sphere_case
