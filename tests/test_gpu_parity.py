@@ -144,6 +144,32 @@ def test_noble_torczynski_with_forces(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+def test_partial_cells_les_sponge_walls(gpu, precision):
+    """The LES sphere's kernels (walls, partial cells, Smagorinsky, sponge, no
+    body force): plain steps build the pass without the per-cell force output,
+    the probe's odd step with it; both stay bit-exact to the oracle."""
+    nx, ny, nz = 26, 12, 6
+    rng = np.random.default_rng(21)
+    eps = np.zeros((nz, ny, nx), dtype=np.float32)
+    eps[:, 3:9, 5:11] = rng.uniform(0.0, 1.0, size=(nz, 6, 6)).astype(np.float32)
+    eps[:, 5, 7] = 1.0
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.85, None, uin=0.05, c_smago=0.1,
+                     flags=_flags("channel", nx, ny, nz), wants_forces=True)
+    for s in (g, r):
+        s.set_solid_fractions(eps)
+        s.sponge = (18.0, 7.0, 1.0)
+    _seed_both(g, r, precision, seed=23)
+    _run_compare(g, r, [2, 4], precision)
+    fg = g.probe_force((3, 14), (1, 10))
+    fr = r.probe_force((3, 14), (1, 10))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+    assert_bit_exact(g.download_force(), r.force_field().astype(np.float32), "per-cell force")
+    scale = np.abs(r.force_field()[:, :3].astype(np.float64)).sum() + 1e-300
+    assert np.all(np.abs(fg - fr) <= 1e-13 * scale), (fg, fr)
+    _run_compare(g, r, [2], precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("kind,shape,force", [("channel", (30, 14, 2), (1e-5, 0.0, 0.0)),
                                               ("periodic", (64, 8, 3), (0.0, 0.0, 0.0)),
                                               ("mixed", (15, 9, 7), (2e-5, -1e-5, 5e-6))])
