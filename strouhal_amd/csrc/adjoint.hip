@@ -540,12 +540,14 @@ static hipError_t launch_hooked_t(const StepLaunch& L, const AdjHooks& hk, hipSt
     StepParams p = L.p;
     p.blocks = (p.N + kStepBS - 1) / kStepBS;
     // porosity excludes partial cells and LES
-#define STRH_HOOKED(NTV, LESV, BR) hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, NTV, LESV, BR>), \
+#define STRH_HOOKED(NTV, LESV, BR, FO) hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, NTV, LESV, BR, FO>), \
         dim3(p.blocks), dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm, L.eps, p, L.alpha, hk)
-    if (L.alpha) STRH_HOOKED(false, false, true);
-    else if (L.nt && L.les) STRH_HOOKED(true, true, false);
-    else if (L.nt) STRH_HOOKED(true, false, false);
-    else STRH_HOOKED(false, true, false);
+    // the forcing path only with a body force, as the plain step's kernel
+    // choice (porosity always uses it)
+    if (L.alpha) STRH_HOOKED(false, false, true, true);
+    else if (L.nt && L.les) { if (L.has_force) STRH_HOOKED(true, true, false, true); else STRH_HOOKED(true, true, false, false); }
+    else if (L.nt) { if (L.has_force) STRH_HOOKED(true, false, false, true); else STRH_HOOKED(true, false, false, false); }
+    else { if (L.has_force) STRH_HOOKED(false, true, false, true); else STRH_HOOKED(false, true, false, false); }
 #undef STRH_HOOKED
     return hipGetLastError();
 }

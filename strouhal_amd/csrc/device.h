@@ -396,7 +396,7 @@ constexpr uint32_t kStepBS = 256;
 // with cp = +cin on the inlet column and -cout on the outlet column.
 // The rate w+ a cell collides with (sponge ramp, then Smagorinsky), for the
 // hooks' partial-cell drag; the same float arithmetic as collide.
-template <bool LES>
+template <bool LES, bool FORCE>
 __device__ __forceinline__ float hook_rate(const float (&fh)[19], const StepParams& p, int x) {
     float wp = p.omega, wm = p.omegaMinus;
     if ((float)x > p.spongeX0) {
@@ -422,13 +422,16 @@ __device__ __forceinline__ float hook_rate(const float (&fh)[19], const StepPara
             pz += (float)kCz[i] * fh[i];
         }
         const float rho = 1.0f + rhom1, inv = 1.0f / rho;
-        const float ux = (px + 0.5f * p.fx) * inv, uy = (py + 0.5f * p.fy) * inv, uz = (pz + 0.5f * p.fz) * inv;
+        // the velocity exactly as collide forms it for this kernel
+        const float ux = (FORCE ? px + 0.5f * p.fx : px) * inv;
+        const float uy = (FORCE ? py + 0.5f * p.fy : py) * inv;
+        const float uz = (FORCE ? pz + 0.5f * p.fz : pz) * inv;
         les_rates<true>(fh, p, rhom1, rho, inv, ux, uy, uz, ux * ux + uy * uy + uz * uz, wp, wm);
     }
     return wp;
 }
 
-template <bool LES>
+template <bool LES, bool FORCE>
 __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[19], uint32_t n,
                                           int x, float alpha, float eps, const StepParams& p) {
     const float fx = p.fx;
@@ -438,7 +441,7 @@ __device__ __forceinline__ void run_hooks(const AdjHooks& hk, const float (&fh)[
         for (int i = 0; i < 19; i++) hk.recNt[(size_t)i * hk.M + k] = fh[i];
     }
     if (hk.jacc && hk.wf != 0.0 && eps > 0.0f) {   // partial-cell drag ntF_x = -2 sum c_x B (g_opp - g)
-        const double tw = 1.0 / (double)hook_rate<LES>(fh, p, x) - 0.5, e = (double)eps;
+        const double tw = 1.0 / (double)hook_rate<LES, FORCE>(fh, p, x) - 0.5, e = (double)eps;
         const double B = e * tw / ((1.0 - e) + tw);
         double nt = 0.0;
 #pragma unroll
@@ -535,7 +538,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         for (int i = 0; i < 19; i++) fh[i] = St::rd(fs[i], nbi);
         const float alpha = BRINK ? alphaBuf[n] : 0.0f;
         if (HOOK && hk->rec19) record_populations<St>(*hk, fh, p.N, n);
-        if (HOOK) run_hooks<LES>(*hk, fh, n, x, alpha, eps, p);
+        if (HOOK) run_hooks<LES, FORCE>(*hk, fh, n, x, alpha, eps, p);
         collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);
         // A separate copy of the offset keeps the stores on the scalar-base
         // form too (otherwise the shared base+offset sum is kept as 19
@@ -647,7 +650,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     }
 
     const float alpha = BRINK ? alphaBuf[n] : 0.0f;
-    if (HOOK) run_hooks<LES>(*hk, fh, n, x, alpha, eps, p);
+    if (HOOK) run_hooks<LES, FORCE>(*hk, fh, n, x, alpha, eps, p);
     collide<LES, FORCE, NT, BRINK>(fh, p, x, eps, ntx, nty, ntz, alpha);  // 315-316
     if (NT) {
         ax += ntx;
@@ -751,7 +754,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
 // Forward step with the adjoint hooks (Guo forcing with Brinkman porosity, or
 // partial cells and / or LES). The arithmetic is that of step_kernel: with
 // alpha = 0 and no body force the DDFs are the plain update's bits.
-template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool BRINK>
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool BRINK, bool FORCE>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                    const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
@@ -759,9 +762,11 @@ step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ f
                    AdjHooks hk) {
     const uint32_t n = blockIdx.x * kStepBS + threadIdx.x;
     if (n >= p.N) return;
-    // (writes no force: the force pointer is null and writeForce 0)
-    step_cell<St, PARITY, WALLS, NT, LES, true, BRINK, true, false>(f, flags, link, lidm, nullptr, epsBuf,
-                                                                    p, n, alphaBuf, &hk);
+    // (writes no force: the force pointer is null and writeForce 0; porosity
+    // acts through the forcing path, so BRINK comes with FORCE)
+    static_assert(FORCE || !BRINK, "porosity needs the forcing path");
+    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK, true, false>(f, flags, link, lidm, nullptr, epsBuf,
+                                                                     p, n, alphaBuf, &hk);
 }
 
 // ---- odd pass with LDS row staging (wall-free lattices) --------------------
