@@ -297,7 +297,9 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
 // The recording holds the populations in the storage format as loaded, before
 // the odd pass's moving-wall corrections; the kernel re-applies them from the
 // step's parameters sp with the forward's arithmetic.
-template <class Conv, bool WALLS, bool NT>
+// FORCE = false: the lattice has no body force, and the terms it would carry
+// (J = p + F/2, the Guo source's derivatives) are exact zeros left out.
+template <class Conv, bool WALLS, bool NT, bool FORCE>
 __global__ void __launch_bounds__(256)
 adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __restrict__ lamN,
                    float* __restrict__ lam, const uint8_t* __restrict__ flags,
@@ -379,7 +381,9 @@ adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __re
 #undef STRH_GMOM
     const float rho = 1.0f + rm1, ir = 1.0f / rho, ir2 = ir * ir;
     const float fx = q.fx, fy = q.fy, fz = q.fz;
-    const float Jx = px + 0.5f * fx, Jy = py + 0.5f * fy, Jz = pz + 0.5f * fz;
+    const float Jx = FORCE ? px + 0.5f * fx : px;
+    const float Jy = FORCE ? py + 0.5f * fy : py;
+    const float Jz = FORCE ? pz + 0.5f * fz : pz;
 
     // local rates: sponge, then Smagorinsky with Pi_neq in closed form,
     // Pi_ab = sum c_a c_b g - delta_ab (rho - 1)/3 - J_a J_b / rho
@@ -410,18 +414,24 @@ adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __re
     const float MJx = Mxx * Jx + Mxy * Jy + Mxz * Jz;
     const float MJy = Mxy * Jx + Myy * Jy + Myz * Jz;
     const float MJz = Mxz * Jx + Myz * Jy + Mzz * Jz;
-    const float MFx = Mxx * fx + Mxy * fy + Mxz * fz;
-    const float MFy = Mxy * fx + Myy * fy + Myz * fz;
-    const float MFz = Mxz * fx + Myz * fy + Mzz * fz;
     const float JMJ = Jx * MJx + Jy * MJy + Jz * MJz;
-    const float JMF = Jx * MFx + Jy * MFy + Jz * MFz;
     const float J2 = Jx * Jx + Jy * Jy + Jz * Jz;
-    const float JF = Jx * fx + Jy * fy + Jz * fz;
-    const float gRho = wp * (M0 - 4.5f * JMJ * ir2 + 1.5f * J2 * M0 * ir2)
-                     - ap * (9.0f * JMF - 3.0f * JF * M0) * ir2;
-    float gPx = wp * (9.0f * MJx - 3.0f * Jx * M0) * ir + 3.0f * wm * Mx + ap * (9.0f * MFx - 3.0f * fx * M0) * ir;
-    float gPy = wp * (9.0f * MJy - 3.0f * Jy * M0) * ir + 3.0f * wm * My + ap * (9.0f * MFy - 3.0f * fy * M0) * ir;
-    float gPz = wp * (9.0f * MJz - 3.0f * Jz * M0) * ir + 3.0f * wm * Mz + ap * (9.0f * MFz - 3.0f * fz * M0) * ir;
+    float gRho = wp * (M0 - 4.5f * JMJ * ir2 + 1.5f * J2 * M0 * ir2);
+    float gPx = wp * (9.0f * MJx - 3.0f * Jx * M0) * ir + 3.0f * wm * Mx;
+    float gPy = wp * (9.0f * MJy - 3.0f * Jy * M0) * ir + 3.0f * wm * My;
+    float gPz = wp * (9.0f * MJz - 3.0f * Jz * M0) * ir + 3.0f * wm * Mz;
+    float JMF = 0.0f, JF = 0.0f;
+    if (FORCE) {
+        const float MFx = Mxx * fx + Mxy * fy + Mxz * fz;
+        const float MFy = Mxy * fx + Myy * fy + Myz * fz;
+        const float MFz = Mxz * fx + Myz * fy + Mzz * fz;
+        JMF = Jx * MFx + Jy * MFy + Jz * MFz;
+        JF = Jx * fx + Jy * fy + Jz * fz;
+        gRho = gRho - ap * (9.0f * JMF - 3.0f * JF * M0) * ir2;
+        gPx = gPx + ap * (9.0f * MFx - 3.0f * fx * M0) * ir;
+        gPy = gPy + ap * (9.0f * MFy - 3.0f * fy * M0) * ir;
+        gPz = gPz + ap * (9.0f * MFz - 3.0f * fz * M0) * ir;
+    }
 
     // Omega_j = w+ ep_j + w- em_j + W_j (a+ xp_j + a- xm_j) with
     //   ep = feq+ - f+, em = feq- - f-, xp = 9 (c.u)(c.F) - 3 u.F, xm = 3 c.F;
@@ -438,12 +448,23 @@ adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __re
 #undef STRH_FPM
     Sfp = mu[0] * g[0] + 0.5f * Sfp;
     Sfm *= 0.5f;
+    // the partial-cell sums over the recorded populations, formed here so that
+    // the populations are not held through the rate chain below:
+    // sum_i mu_i (g_opp(i) - g_i) and the x momentum exchange's -2 c_x (g_i+1 - g_i)
+    float sumB = 0.0f, dNt = 0.0f;
+    if (NT) {
+#pragma unroll
+        for (int i = 1; i < 19; i++) sumB += mu[i] * (g[kOpp[i]] - g[i]);
+#pragma unroll
+        for (int i = 1; i < 19; i += 2) if (kCx[i]) dNt -= 2.0f * (float)kCx[i] * (g[i + 1] - g[i]);
+        asm("" : "+v"(sumB), "+v"(dNt));   // formed here, not sunk to their use
+    }
     const float Sep = M0 * (rm1 - 1.5f * J2 * ir) + 4.5f * JMJ * ir - Sfp;
     const float Sem = 3.0f * (Mx * Jx + My * Jy + Mz * Jz) - Sfm;
-    const float Sxp = 9.0f * JMF * ir - 3.0f * JF * ir * M0;
-    const float Sxm = 3.0f * (Mx * fx + My * fy + Mz * fz);
-    float gwp = Sep - 0.5f * Sxp;
-    float gwm = Sem - 0.5f * Sxm;
+    const float Sxp = FORCE ? 9.0f * JMF * ir - 3.0f * JF * ir * M0 : 0.0f;
+    const float Sxm = FORCE ? 3.0f * (Mx * fx + My * fy + Mz * fz) : 0.0f;
+    float gwp = FORCE ? Sep - 0.5f * Sxp : Sep;
+    float gwm = FORCE ? Sem - 0.5f * Sxm : Sem;
 
     const float cp = (x == q.inletX ? q.cin : 0.0f) - (x == q.outletX ? q.cout : 0.0f);
     float B = 0.0f, omB = 1.0f;
@@ -455,21 +476,20 @@ adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __re
             // with tw = 1/w+ - 1/2 at the local rate
             nt = true;
             const float e = epsBuf[n];
-            const float tw = 1.0f / wp - 0.5f;
+            // two divisions (1/w+, 1/den) feed every quotient of the block
+            const float rwp = 1.0f / wp;
+            const float tw = rwp - 0.5f;
             const float den = (1.0f - e) + tw;
-            B = e * tw / den;
+            const float rden = 1.0f / den;
+            B = e * tw * rden;
             omB = 1.0f - B;
-            const float muOm = wp * Sep + wm * Sem + ap * Sxp + am * Sxm;
-            float gB = -muOm, dNt = 0.0f;
-#pragma unroll
-            for (int i = 1; i < 19; i++) gB += mu[i] * (g[kOpp[i]] - g[i]);
-#pragma unroll
-            for (int i = 1; i < 19; i += 2) if (kCx[i]) dNt -= 2.0f * (float)kCx[i] * (g[i + 1] - g[i]);
+            const float muOm = FORCE ? wp * Sep + wm * Sem + ap * Sxp + am * Sxm : wp * Sep + wm * Sem;
+            const float gB = sumB - muOm;
             const float gBtot = gB + q.wf * dNt;
-            gEps[k] += (double)(gBtot * (tw * (1.0f + tw) / (den * den)));
+            gEps[k] += (double)(gBtot * (tw * (1.0f + tw) * (rden * rden)));
             gwp *= omB;
             gwm *= omB;
-            gwp += gBtot * (e * (1.0f - e) / (den * den)) * (-1.0f / (wp * wp));
+            gwp -= gBtot * (e * (1.0f - e) * (rden * rden)) * (rwp * rwp);
         }
     }
 
@@ -580,12 +600,14 @@ hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float4* rec,
     q.blocks = (q.N + 255u) / 256u;
     const dim3 g(q.blocks), b(256);
     if (rec19) {
-#define STRH_ADJL(C, W, NTV) hipLaunchKernelGGL((adjoint_les_kernel<C, W, NTV>), g, b, 0, s,              \
-                                                (const typename C::T*)rec19, lamNext, lam, flags, link,   \
-                                                lidm, gUin, eps, ntIdx, gEps, q, sp)
+#define STRH_ADJL(C, W, NTV, FO) hipLaunchKernelGGL((adjoint_les_kernel<C, W, NTV, FO>), g, b, 0, s,      \
+                                                    (const typename C::T*)rec19, lamNext, lam, flags, link, \
+                                                    lidm, gUin, eps, ntIdx, gEps, q, sp)
+#define STRH_ADJF(C, W, NTV) if (forced) STRH_ADJL(C, W, NTV, true); else STRH_ADJL(C, W, NTV, false);
 #define STRH_ADJC(C)                                                                                  \
-        if (ntIdx) { if (walls) STRH_ADJL(C, true, true); else STRH_ADJL(C, false, true); }           \
-        else { if (walls) STRH_ADJL(C, true, false); else STRH_ADJL(C, false, false); }
+        if (ntIdx) { if (walls) STRH_ADJF(C, true, true) else STRH_ADJF(C, false, true) }             \
+        else { if (walls) STRH_ADJF(C, true, false) else STRH_ADJF(C, false, false) }
+        const bool forced = !(q.fx == 0.0f && q.fy == 0.0f && q.fz == 0.0f);
         switch (q.precision) {
             case kFP32: STRH_ADJC(ConvF32); break;
             case kFP16S: STRH_ADJC(ConvF16); break;
@@ -593,6 +615,7 @@ hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float4* rec,
             default: return hipErrorInvalidValue;
         }
 #undef STRH_ADJC
+#undef STRH_ADJF
 #undef STRH_ADJL
         return hipGetLastError();
     }
