@@ -244,6 +244,14 @@ def main():
                            "unit": "GB/s", "frac": round(fp16_achieved / HBM_PEAK_GBS, 4)},
         "adjoint": adj,
         "vs_baseline_ref": "reference README.md:24-28, Apple M5 256^3 FP32 ~785 MLUPS",
+        # the reference's own bytes-per-cell model of its bench (19 x 2 DDFs, a
+        # 1-byte flag and half of its 8-byte odd-step masks: 157 / 81 B,
+        # Cases.swift:142-144), for comparison with its published GB/s
+        "reference_byte_model": {
+            leg: {"bytes_per_cell": 19 * 2 * b + 1 + 4,
+                  "GBps": round(r["mlups"] * (19 * 2 * b + 1 + 4) / 1000.0, 1)}
+            for leg, r, b in (("fp32", fp32, 4), ("fp16s", fp16, 2))
+        },
     }
     print(json.dumps(line), flush=True)
 
