@@ -217,6 +217,19 @@ STRH_API int strh_batch_autotune(strh_batch* batch, int* block_size);
 /* Steps per captured HIP graph (0: plain launches). Default 64. */
 STRH_API int strh_batch_set_graph_steps(strh_batch* batch, int steps);
 STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);
+/* Resident launches (the persistent launch of Ladder.swift:54-121's rungs):
+ * lattices one cell deep (nz = 1) are cut into tiles held in LDS, one
+ * workgroup per CU, and advanced kProbeSpan (64) steps per launch, trading
+ * only their edge populations between steps; groups of lattices that do not
+ * fit on chip together are time-sliced. Results are the same bits as the
+ * streaming kernel's. mode 0: never; 1 (default): in every run segment whose
+ * active lattices all qualify and fit on chip in one launch; 2: in every
+ * segment, time-sliced where needed (STRH_ERR_UNSUPPORTED when some lattice
+ * does not qualify).
+ * Each qualifying lattice holds 152 bytes per cell of exchange words while it
+ * is batched. A stalled exchange (which a correct build never produces)
+ * returns STRH_ERR_DEVICE after about a second. */
+STRH_API int strh_batch_set_resident(strh_batch* batch, int mode);
 
 /* ---- discrete adjoint (no reference counterpart: the reference has no
  * gradients) ------------------------------------------------------------------

@@ -30,7 +30,7 @@ DEVICE_FLAGS = [
     "-fvisibility=hidden", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 # Device translation units (the storage formats of the step kernels each have one).
-DEVICE_UNITS = ["step_fp32", "step_fp16", "step_bf16", "adjoint", "batch", "observe"]
+DEVICE_UNITS = ["step_fp32", "step_fp16", "step_bf16", "adjoint", "batch", "resident", "observe"]
 HOST_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden",
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wextra", "-Wno-unused-result",

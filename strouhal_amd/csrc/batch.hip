@@ -17,20 +17,6 @@ __device__ __forceinline__ int batch_find(const BatchLat* lats, int L, uint32_t 
     return l;
 }
 
-__device__ __forceinline__ bool is_force_step(const BatchLat& B, int64_t t) {
-    if (B.fCount <= 0 || t < B.fT0) return false;
-    const int64_t d = t - B.fT0;
-    return d % B.fPeriod == 0 && d / B.fPeriod < B.fCount;
-}
-
-// The force buffer sample j = (t - fT0) / fPeriod is written to: buffer
-// j % fRing of the ring, so the samples of up to kProbeSpan steps are reduced
-// by one probe launch.
-__device__ __forceinline__ float4* force_slot(const BatchLat& B, int64_t t) {
-    if (B.fCount <= 0 || t < B.fT0) return B.force;
-    return B.force + (size_t)(((t - B.fT0) / B.fPeriod) % B.fRing) * B.p.N;
-}
-
 // PAR = 0 / 1: the host guarantees that every active lattice is at that
 // parity in this launch (equal start parities, even graph lengths), so only
 // that pass is compiled in; PAR = -1 branches per lattice. LES = false when no

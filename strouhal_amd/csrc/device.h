@@ -874,6 +874,21 @@ step_odd_rows(typename St::T* __restrict__ f, float4* __restrict__ force,
         force[n] = make_float4(ntx, nty, ntz, 0.0f);
 }
 
+// ---- batch probe schedule ----------------------------------------------------
+__device__ __forceinline__ bool is_force_step(const BatchLat& B, int64_t t) {
+    if (B.fCount <= 0 || t < B.fT0) return false;
+    const int64_t d = t - B.fT0;
+    return d % B.fPeriod == 0 && d / B.fPeriod < B.fCount;
+}
+
+// The force buffer sample j = (t - fT0) / fPeriod is written to: buffer
+// j % fRing of the ring, so the samples of up to kProbeSpan steps are reduced
+// by one probe launch.
+__device__ __forceinline__ float4* force_slot(const BatchLat& B, int64_t t) {
+    if (B.fCount <= 0 || t < B.fT0) return B.force;
+    return B.force + (size_t)(((t - B.fT0) / B.fPeriod) % B.fRing) * B.p.N;
+}
+
 // ---- fixed-order FP64 reductions ------------------------------------------
 // The summed range is split into `segs` segments of M elements (the 19 DDF
 // slots for the mass sum, one segment otherwise). Stage 1: block (b, g) sums

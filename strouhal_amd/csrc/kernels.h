@@ -57,7 +57,29 @@ struct BatchLat {
     FastDiv fbw, fbh;        // probe box width / height
     uint32_t M, Pb, chunk;   // box cells, reduction chunks (as the single probe)
     double* series;          // 3 doubles per sample
+    // resident launches (resident.hip): the edge-exchange words [19][N] and
+    // the step whose exchange tag is 1 (the step the batch run started at)
+    uint64_t* xchg;
+    int64_t tagBase;
 };
+
+// One tile of a resident launch: cells [x0, x0 + w) x [y0, y0 + h) of lattice
+// `lat` (an index into the launch's BatchLat array; nz = 1, w, h >= 3), held
+// in LDS with a one-cell halo ring; divi divides by w - 2 (interior rows).
+struct ResTile {
+    int32_t lat, x0, y0, w, h;
+    FastDiv divi;
+};
+// Largest workgroup of the resident launch (one workgroup per CU); a launch
+// uses a multiple of 64 up to it, and each thread takes at most
+// kResidentMaxRounds cells of its tile.
+constexpr int kResidentBS = 768;
+constexpr int kResidentMaxRounds = 4;
+// A wave that waits this long for a neighbour's exchange word aborts the
+// batch (ticks of the 100 MHz wall clock: one second).
+constexpr uint64_t kResidentSpinTicks = 100000000ull;
+// LDS bytes a resident tile may use (the workgroup's static LDS stays below 256 bytes).
+constexpr size_t kResidentMaxLds = 163840 - 256;
 
 // Hooks of the recording forward step of the adjoint (walls, Guo forcing,
 // Brinkman porosity, partial cells, LES, sponge).
@@ -124,6 +146,14 @@ hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool f
 constexpr int kProbeSpan = 64;
 hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s);
 hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
+// Steps k0 .. k0 + K - 1 (K <= kProbeSpan) of every tile of one resident group;
+// lds is the largest tile's (w + 2)(h + 2) 19 sizeof(T) bytes.
+hipError_t launch_resident(int precision, bool les, bool forced, const BatchLat* lats, const ResTile* tiles,
+                           int nTiles, int bs, size_t lds, const int64_t* base, int k0, int K, int* abortFlag,
+                           hipStream_t s);
+// Allows kResidentMaxLds of dynamic LDS for every resident kernel of this
+// precision and returns how many of its workgroups one CU holds at that size.
+hipError_t resident_prepare(int precision, size_t lds, int* perCU);
 hipError_t launch_set_base(int64_t* base, int64_t v, hipStream_t s);
 void box_reduce_plan(uint32_t M, uint32_t max_partials, uint32_t* Pb, uint32_t* chunk);
 // Workgroup size of the LDS row-staged odd pass for this lattice, or 0.

@@ -111,6 +111,9 @@ size_t ddf_bytes(const strh_lattice* L) { return (size_t)ddf_elements(L->lay) * 
 // the adjoint, which run the direct odd pass, refuse such lattices.
 bool large_image(const strh_lattice* L) { return L->lay.blocks > 1 && ddf_bytes(L) >= (1ull << 32); }
 
+// A lattice the resident batch launch can hold: one cell deep, at least 3 x 3.
+bool resident_candidate(const strh_lattice* L) { return L->nz == 1 && L->nx >= 3 && L->ny >= 3; }
+
 // Copy cells [c0, c0 + count) of DDF slot `slot` between the device image
 // and the contiguous buffer `lin` (count elements), in at most three copies:
 // the rest of a partly covered first block, one pitched copy of the whole
@@ -904,6 +907,13 @@ struct strh_batch {
     int bs = 256;
     int graphSteps = 64;
     double gpuSeconds = 0.0;
+    // resident launches (resident.hip)
+    int resident = 1;                    // 0 never, 1 where every active lattice qualifies, 2 required
+    int residentSlots = 0;               // workgroups the device holds at once at the largest tile
+    std::vector<uint64_t*> xchg;         // per lattice, [19][N] exchange words, or null
+    int* abortFlag = nullptr;            // raised by a resident launch whose exchange stalled
+    ResTile* tiles = nullptr;            // every resident group's tiles of one run (device)
+    size_t tilesCap = 0;
 };
 
 namespace {
@@ -939,9 +949,23 @@ int batch_prepare(strh_batch* B, const int64_t* steps) {
         }
         d.ramp = B->ramp[i];
         d.rampSteps = rl;
+        if (B->resident != 0 && resident_candidate(L) && !B->xchg[i])
+            HIP_TRY(hipMalloc(&B->xchg[i], (size_t)19 * L->N * sizeof(uint64_t)), "hipMalloc(exchange)");
+        d.xchg = B->xchg[i];
+        d.tagBase = L->stepsDone;
     }
     return STRH_OK;
 }
+
+// One resident launch: the tiles [first, first + count) of the run's tile
+// array, covering a group of the segment's lattices that fits in LDS at once.
+struct ResGroup {
+    size_t first;
+    int count;
+    int bs;                  // workgroup size: the largest tile in as few full rounds as possible
+    size_t lds;              // dynamic LDS of the largest tile
+    bool les, forced;        // forced: every lattice of the group has a body force, else none
+};
 
 struct Segment {
     int64_t k0, k1;          // lock-step batch steps [k0, k1)
@@ -952,7 +976,63 @@ struct Segment {
     bool les;                // some active lattice has a Smagorinsky constant
     bool forced;             // some active lattice has a body force
     bool probes;             // some active lattice has force samples
+    std::vector<ResGroup> groups;   // resident launches, or empty for the streaming kernel
 };
+
+// ---- resident tiling ---------------------------------------------------------
+// A lattice one cell deep is cut into gx x gy tiles of near-equal size (each
+// at least 3 x 3 cells); a group of lattices shares the device's workgroup
+// slots in proportion to their cells, and groups are time-sliced: each runs
+// its steps in a launch of its own.
+struct TilePlan {
+    int gx = 0, gy = 0;
+    int wmax = 0, hmax = 0;
+};
+
+size_t tile_lds(int w, int h, int bytes) { return (size_t)(w + 2) * (size_t)(h + 2) * 19u * (size_t)bytes; }
+
+// The split of nx x ny into at most g tiles with the fewest cells in its
+// largest tile (then the shortest edge), or gx = 0 when none fits in LDS.
+TilePlan plan_lattice(int nx, int ny, int g, int bytes) {
+    TilePlan best;
+    int64_t bestCells = -1, bestEdge = 0;
+    for (int gx = 1; gx <= g && gx <= nx / 3; gx++) {
+        const int gy = std::min(g / gx, ny / 3);
+        if (gy < 1) continue;
+        const int w = (nx + gx - 1) / gx, h = (ny + gy - 1) / gy;
+        if (tile_lds(w, h, bytes) > kResidentMaxLds || w * h > kResidentMaxRounds * kResidentBS) continue;
+        const int64_t cells = (int64_t)w * h, edge = (int64_t)w + h;
+        if (bestCells < 0 || cells < bestCells || (cells == bestCells && edge < bestEdge)) {
+            best.gx = gx; best.gy = gy; best.wmax = w; best.hmax = h;
+            bestCells = cells; bestEdge = edge;
+        }
+    }
+    return best;
+}
+
+
+// Tiles of the lattices `members` (indices into lats) sharing `slots` workgroups;
+// false when some lattice gets no split that fits in LDS.
+bool plan_group(const std::vector<strh_lattice*>& lats, const std::vector<int>& members, int slots, int bytes,
+                std::vector<TilePlan>* plans) {
+    double cells = 0.0;
+    for (int i : members) cells += (double)lats[i]->N;
+    int used = 0;
+    std::vector<TilePlan> out;
+    for (int i : members) {
+        const strh_lattice* L = lats[i];
+        int g = std::max(1, (int)std::floor((double)slots * (double)L->N / cells));
+        // at most one tile per kResidentBS cells: smaller tiles leave threads idle
+        g = std::min(g, std::max(1, (int)((L->N + kResidentBS - 1) / kResidentBS)));
+        TilePlan tp = plan_lattice(L->nx, L->ny, g, bytes);
+        if (tp.gx == 0) return false;
+        used += tp.gx * tp.gy;
+        out.push_back(tp);
+    }
+    if (used > slots) return false;
+    if (plans) *plans = out;
+    return true;
+}
 
 // Launches steps k = kOff .. kOff+count-1 from base; parity0 is the common
 // parity at k = kOff (or -1), so step k runs the pass (parity0 + k - kOff) & 1 only.
@@ -960,6 +1040,17 @@ struct Segment {
 // reduced by one probe launch; each lattice's ring holds that many samples.
 int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int parity0, int kOff = 0) {
     hipStream_t s = B->ctx->stream;
+    if (!sg.groups.empty()) {
+        // resident: each group advances kProbeSpan steps per launch, then the probes
+        for (int j = 0; j < count; j += kProbeSpan) {
+            const int K = std::min(kProbeSpan, count - j);
+            for (const ResGroup& g : sg.groups)
+                HIP_TRY(launch_resident(B->precision, g.les, g.forced, sg.dev, B->tiles + g.first, g.count, g.bs, g.lds,
+                                        B->base, kOff + j, K, B->abortFlag, s), "resident launch");
+            if (sg.probes) HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, kOff + j, K, s), "batch probe launch");
+        }
+        return STRH_OK;
+    }
     int k0 = kOff;
     for (int j = 0; j < count; j++) {
         const int k = kOff + j;
@@ -997,6 +1088,73 @@ int batch_capture(strh_batch* B, const Segment& sg, hipGraphExec_t* out) {
 // descriptors and grid), as whole captured graphs of graphSteps steps plus
 // single launches for the remainder. Kernels read t = step0 + base + k, base
 // being set at each segment start and advanced by every graph replay.
+// Group the segment's lattices `members` (indices into B->lats; compact
+// descriptor index = position in members) into resident launches and append
+// their tiles to `tiles`. Returns false when some lattice does not qualify.
+bool plan_segment(strh_batch* B, const std::vector<int>& members, Segment& sg, std::vector<ResTile>& tiles) {
+    const int bytes = (int)B->lats[0]->B;
+    for (int i : members)
+        if (!resident_candidate(B->lats[i]) || !B->xchg[i]) return false;
+    std::vector<int> order(members.size());
+    for (size_t j = 0; j < order.size(); j++) order[j] = (int)j;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        return B->lats[members[a]]->N > B->lats[members[b]]->N;
+    });
+    auto forced_of = [&](int j) { return B->host[members[j]].hasForce != 0; };
+    std::vector<std::vector<int>> groups;   // positions in members
+    auto lat_ids = [&](const std::vector<int>& g) {
+        std::vector<int> v;
+        for (int j : g) v.push_back(members[j]);
+        return v;
+    };
+    for (int j : order) {
+        bool placed = false;
+        for (auto& g : groups) {
+            if (forced_of(g[0]) != forced_of(j)) continue;
+            std::vector<int> cand = g;
+            cand.push_back(j);
+            if (plan_group(B->lats, lat_ids(cand), B->residentSlots, bytes, nullptr)) { g = cand; placed = true; break; }
+        }
+        if (placed) continue;
+        if (!plan_group(B->lats, lat_ids({j}), B->residentSlots, bytes, nullptr)) return false;
+        groups.push_back({j});
+    }
+    for (const auto& g : groups) {
+        std::vector<TilePlan> plans;
+        plan_group(B->lats, lat_ids(g), B->residentSlots, bytes, &plans);
+        ResGroup rg;
+        rg.first = tiles.size();
+        rg.lds = 0;
+        rg.les = false;
+        rg.forced = forced_of(g[0]);
+        for (size_t q = 0; q < g.size(); q++) {
+            const strh_lattice* L = B->lats[members[g[q]]];
+            rg.les = rg.les || L->cSmago > 0.0f;
+            const TilePlan& tp = plans[q];
+            for (int b = 0; b < tp.gy; b++) {
+                for (int a = 0; a < tp.gx; a++) {
+                    ResTile t;
+                    t.lat = g[q];
+                    t.x0 = (int)((int64_t)a * L->nx / tp.gx);
+                    t.w = (int)((int64_t)(a + 1) * L->nx / tp.gx) - t.x0;
+                    t.y0 = (int)((int64_t)b * L->ny / tp.gy);
+                    t.h = (int)((int64_t)(b + 1) * L->ny / tp.gy) - t.y0;
+                    t.divi = make_fastdiv((uint32_t)(t.w - 2));
+                    rg.lds = std::max(rg.lds, tile_lds(t.w, t.h, bytes));
+                    tiles.push_back(t);
+                }
+            }
+        }
+        rg.count = (int)(tiles.size() - rg.first);
+        int cmax = 0;
+        for (size_t q = rg.first; q < tiles.size(); q++) cmax = std::max(cmax, tiles[q].w * tiles[q].h);
+        const int rounds = (cmax + kResidentBS - 1) / kResidentBS;
+        rg.bs = ((cmax + rounds - 1) / rounds + 63) / 64 * 64;
+        sg.groups.push_back(rg);
+    }
+    return true;
+}
+
 int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probes, double* seconds) {
     hipStream_t s = B->ctx->stream;
     const int n = (int)B->lats.size();
@@ -1005,9 +1163,18 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
     std::sort(ends.begin(), ends.end());
     ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
     if (ends.empty()) { if (seconds) *seconds = 0.0; return STRH_OK; }
+    if (B->resident != 0 && B->residentSlots == 0) {
+        int perCU = 0, cus = 0;
+        HIP_TRY(resident_prepare(B->precision, kResidentMaxLds, &perCU), "resident kernel attributes");
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->ctx->device), "CU count");
+        B->residentSlots = std::max(0, cus * perCU);
+        if (B->residentSlots == 0) return fail(STRH_ERR_UNSUPPORTED, "the resident kernel does not fit on a CU");
+    }
     // compact descriptor arrays of every segment, uploaded once
     std::vector<Segment> segs;
     std::vector<BatchLat> up;
+    std::vector<ResTile> tiles;
+    bool anyResident = false;
     int64_t k0 = 0;
     for (int64_t e : ends) {
         Segment sg;
@@ -1016,6 +1183,7 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         sg.les = false;
         sg.forced = false;
         sg.probes = false;
+        std::vector<int> members;
         for (int i = 0; i < n; i++) {
             if (steps[i] <= k0) continue;
             BatchLat d = B->host[i];
@@ -1029,7 +1197,23 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             sg.probes = sg.probes || d.fCount > 0;
             sg.blocks += d.nblocks;
             up.push_back(d);
+            members.push_back(i);
             sg.L++;
+        }
+        if (B->resident != 0) {
+            // mode 1 takes the resident launch where the segment's lattices fit
+            // on chip in one launch (time-sliced groups measured slower than
+            // the streaming kernel); mode 2 takes it wherever they qualify
+            const size_t mark = tiles.size();
+            const bool ok = plan_segment(B, members, sg, tiles);
+            if (!ok || (B->resident == 1 && sg.groups.size() > 1)) {
+                sg.groups.clear();
+                tiles.resize(mark);
+                if (B->resident == 2)
+                    return fail(STRH_ERR_UNSUPPORTED, "resident launches need lattices one cell deep (nz = 1, "
+                                "nx, ny >= 3) that fit in LDS");
+            }
+            anyResident = anyResident || !sg.groups.empty();
         }
         segs.push_back(sg);
         k0 = e;
@@ -1041,11 +1225,28 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         HIP_TRY(hipMalloc(&B->dev, up.size() * sizeof(BatchLat)), "hipMalloc(batch descriptors)");
         B->devCap = up.size();
     }
+    if (tiles.size() > B->tilesCap) {
+        if (B->tiles) hipFree(B->tiles);
+        B->tiles = nullptr;
+        B->tilesCap = 0;
+        HIP_TRY(hipMalloc(&B->tiles, tiles.size() * sizeof(ResTile)), "hipMalloc(resident tiles)");
+        B->tilesCap = tiles.size();
+    }
     HIP_TRY(hipStreamSynchronize(s), "batch descriptors");
     HIP_TRY(hipMemcpy(B->dev, up.data(), up.size() * sizeof(BatchLat), hipMemcpyHostToDevice),
             "upload batch descriptors");
+    if (!tiles.empty())
+        HIP_TRY(hipMemcpy(B->tiles, tiles.data(), tiles.size() * sizeof(ResTile), hipMemcpyHostToDevice),
+                "upload resident tiles");
     size_t off = 0;
     for (Segment& sg : segs) { sg.dev = B->dev + off; off += sg.L; }
+    if (anyResident) {
+        // exchange tags count from 1 in every run: clear the words and the abort flag
+        for (int i = 0; i < n; i++)
+            if (B->xchg[i] && steps[i] > 0)
+                HIP_TRY(launch_fill_zero(B->xchg[i], (size_t)19 * B->lats[i]->N * sizeof(uint64_t), s), "clear exchange");
+        HIP_TRY(launch_fill_zero(B->abortFlag, sizeof(int), s), "clear abort flag");
+    }
 
     HIP_TRY(hipEventRecord(B->ctx->ev0, s), "hipEventRecord");
     for (const Segment& sg : segs) {
@@ -1081,6 +1282,13 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, B->ctx->ev0, B->ctx->ev1), "hipEventElapsedTime");
     if (seconds) *seconds = ms * 1e-3;
+    if (anyResident) {
+        int aborted = 0;
+        HIP_TRY(hipMemcpy(&aborted, B->abortFlag, sizeof(int), hipMemcpyDeviceToHost), "read abort flag");
+        if (aborted)
+            return fail(STRH_ERR_DEVICE, "a resident launch stalled waiting for a neighbour tile's exchange; "
+                        "the batch's lattices are left in an undefined state");
+    }
     return STRH_OK;
 }
 
@@ -1115,12 +1323,14 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
     for (int i = 0; i < count; i++) B->lastSteps[i] = lats[i]->stepsDone;
     B->fring.assign(count, nullptr);
     B->fringLen.assign(count, 0);
+    B->xchg.assign(count, nullptr);
     auto bail = [&](hipError_t e, const char* what) {
         strh_batch_destroy(B);
         return hip_fail(e, what);
     };
     hipError_t e;
     if ((e = hipMalloc(&B->base, sizeof(int64_t))) != hipSuccess) return bail(e, "hipMalloc(base)");
+    if ((e = hipMalloc(&B->abortFlag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(abort flag)");
     // The batch kernel runs the walls path everywhere: wall-free lattices get
     // all-zero link words (identical arithmetic, every neighbour fluid).
     for (int i = 0; i < count; i++) {
@@ -1140,6 +1350,9 @@ int strh_batch_destroy(strh_batch* B) {
     for (float* r : B->ramp) if (r) hipFree(r);
     for (double* d : B->series) if (d) hipFree(d);
     for (void* p : B->fring) if (p) hipFree(p);
+    for (uint64_t* p : B->xchg) if (p) hipFree(p);
+    if (B->tiles) hipFree(B->tiles);
+    if (B->abortFlag) hipFree(B->abortFlag);
     if (B->dev) hipFree(B->dev);
     if (B->base) hipFree(B->base);
     delete B;
@@ -1247,6 +1460,12 @@ int strh_batch_set_block_size(strh_batch* B, int bs) {
 int strh_batch_set_graph_steps(strh_batch* B, int steps) {
     if (!B || steps < 0 || steps > 4096) return fail(STRH_ERR_INVALID, "graph steps must be in [0, 4096]");
     B->graphSteps = steps;
+    return STRH_OK;
+}
+
+int strh_batch_set_resident(strh_batch* B, int mode) {
+    if (!B || mode < 0 || mode > 2) return fail(STRH_ERR_INVALID, "resident mode must be 0, 1 or 2");
+    B->resident = mode;
     return STRH_OK;
 }
 

@@ -150,8 +150,9 @@ def run_sequential(variants):
     return series, time.perf_counter() - t0
 
 
-def run_batched(variants, autotune=True, graph_steps=64):
+def run_batched(variants, autotune=True, graph_steps=64, resident=1):
     b = Batch([v.case.sim for v in variants])
+    b.set_resident(resident)
     for i, v in enumerate(variants):
         first, period, count = probe_schedule(v)
         b.set_probe(i, first, period, count, v.case.box_x, v.case.box_y)
@@ -164,7 +165,7 @@ def run_batched(variants, autotune=True, graph_steps=64):
     return series, wall, bs, b
 
 
-def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64):
+def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64, resident=1):
     """Microseconds per lock-step step of every rung together and of the finest
     rung alone, from the rungs' initial state."""
     out = {}
@@ -175,6 +176,7 @@ def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64):
         cells = sum(s.cells for s in sims)
         b = Batch(sims)
         b.set_graph_steps(graph_steps)
+        b.set_resident(resident)
         b.run([16] * len(sims))
         g0 = b.gpu_seconds
         b.run([steps] * len(sims))
@@ -198,6 +200,9 @@ def main():
                     help="instead of the schedule: time this many steps of all rungs and of the "
                          "finest rung alone")
     ap.add_argument("--graph-steps", type=int, default=64)
+    ap.add_argument("--resident", type=int, default=1,
+                    help="0: streaming kernel only; 1: resident launches where the active rungs fit on chip "
+                         "in one launch; 2: resident launches everywhere (time-sliced groups)")
     ap.add_argument("--report", default="", help="write the credibility report (markdown) of mean C_D here")
     args = ap.parse_args()
     res = [int(r) for r in args.resolutions.split(",")]
@@ -206,15 +211,16 @@ def main():
     if args.step_rate > 0:
         print(json.dumps({"workload": f"DFG 2D-2 street ladder D={res}, {args.step_rate} steps per mode",
                           "precision": args.precision, "graph_steps": args.graph_steps,
+                          "resident": args.resident,
                           **step_rates(gpu, args.precision, res, base_umax, args.step_rate,
-                                       args.graph_steps)}), flush=True)
+                                       args.graph_steps, args.resident)}), flush=True)
         return
     report = {"workload": f"DFG 2D-2 street ladder D={res} x (Ma {args.mach}, half-Mach), "
-                          f"schedule x{args.scale}", "precision": args.precision}
+                          f"schedule x{args.scale}", "precision": args.precision, "resident": args.resident}
     vb = build(gpu, args.precision, res, args.scale, base_umax)
     cells = sum(v.case.sim.cells for v in vb)
     updates = sum(v.case.sim.cells * v.total_steps for v in vb)
-    sb, wb, bs, _ = run_batched(vb)
+    sb, wb, bs, _ = run_batched(vb, resident=args.resident)
     report.update({"lattices": len(vb), "cells": cells, "cell_updates": updates,
                    "batched_s": wb, "batched_mlups": updates / wb / 1e6, "block_size": bs})
     rungs = []

@@ -47,8 +47,8 @@ def test_ladder_batched_equals_sequential(gpu, precision):
     assert vb[-2].case.sim.state_digest in md          # the finest Mach-1 rung's digest
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16s"])
-def test_full_size_ladder_batched_equals_sequential(gpu, precision):
+@pytest.mark.parametrize("precision,resident", [("fp32", 1), ("fp16s", 1), ("fp32", 2), ("fp16s", 2)])
+def test_full_size_ladder_batched_equals_sequential(gpu, precision, resident):
     # The ladder at its full size (D in {16, 24, 32, 48, 64}, each at Ma 0.1
     # and half Mach: 10 lattices, 1,504,680 cells) on 1% of the reference
     # schedule: the batch ends every lattice on the sequential run's bits,
@@ -58,7 +58,9 @@ def test_full_size_ladder_batched_equals_sequential(gpu, precision):
     vb = build(gpu, precision, res, 0.01, BASE_UMAX)
     vs = build(gpu, precision, res, 0.01, BASE_UMAX)
     assert len(vb) == 10 and sum(v.case.sim.cells for v in vb) == 1_504_680
-    sb, _, _, _ = run_batched(vb, autotune=True, graph_steps=64)
+    # resident 1: LDS-resident launches for the segments whose rungs fit on
+    # chip together (the tail); 2: for every segment, in time-sliced groups
+    sb, _, _, _ = run_batched(vb, autotune=True, graph_steps=64, resident=resident)
     ss, _ = run_sequential(vs)
     for a, b, v in zip(sb, ss, vb):
         assert a.shape == b.shape == (v.probes, 3)
@@ -192,16 +194,19 @@ def _oracle_series(r, box, first, period, count, total):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16s", "bf16"])
-@pytest.mark.parametrize("graph", [16, 0])
-def test_batch_matches_oracle(gpu, precision, graph):
+@pytest.mark.parametrize("graph,resident", [(16, 0), (0, 0), (16, 2), (0, 2)])
+def test_batch_matches_oracle(gpu, precision, graph, resident):
     """Every lattice of a batch (wall-free, cavity, two street rungs with
     probes) against the FP32 oracle run alone: DDF images bit for bit, and
-    each FP64 force sample within 1e-13 of the oracle's probe_force."""
+    each FP64 force sample within 1e-13 of the oracle's probe_force. resident
+    0 streams every step through memory; 2 holds the lattices in LDS (tiles
+    exchanging their edges) for every segment of every run."""
     specs = _planar_specs()
     pairs = [_make(gpu, precision, sp, oracle=True) for sp in specs]
     sims = [g for g, _ in pairs]
     bt = Batch(sims)
     bt.set_graph_steps(graph)
+    bt.set_resident(resident)
     for i, (first, period, count) in _PROBES.items():
         bt.set_probe(i, first, period, count, *specs[i]["box"])
     for run in _RUNS:
@@ -271,3 +276,57 @@ def test_probed_lattice_stepped_outside_the_batch_is_refused(gpu):
     bt.set_probe(2, 15, 4, 2, *box)    # a new schedule from the current step
     bt.run([6, 6, 6])
     assert bt.probe_series(2).shape == (2, 3)
+
+
+def test_resident_launches_refuse_lattices_that_do_not_qualify(gpu):
+    from strouhal_amd._lib import StrouhalError
+    sims = _mixed(gpu, "fp32")                 # two of them are 3-D
+    bt = Batch(sims)
+    bt.set_resident(2)
+    with pytest.raises(StrouhalError):
+        bt.run([4, 4, 4])
+    assert all(s.steps_done == 0 for s in sims)
+    bt.set_resident(1)                         # falls back to the streaming kernel
+    bt.run([4, 4, 4])
+    assert all(s.steps_done == 4 for s in sims)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16s"])
+def test_resident_tiles_of_every_shape(gpu, precision):
+    """Lattices whose tiles wrap onto themselves (one tile across x or y),
+    ragged tile grids, walls through tile edges and corners, a moving lid,
+    partial cells: resident launches (mode 2) end on the streaming kernel's
+    bits and force samples."""
+    from strouhal_amd import Simulation
+    from strouhal_amd.cases import cavity_flags, disk_solid_fractions
+    rng = np.random.default_rng(7)
+
+    def lattices():
+        out = []
+        for nx, ny in ((3, 3), (5, 40), (47, 4), (130, 61), (61, 130)):
+            fl = (rng.random((1, ny, nx)) < 0.15).astype(np.uint8)      # scattered solid cells
+            s = Simulation(gpu, precision, nx=nx, ny=ny, nz=1, omega=1.7, omega_minus=1.2,
+                           flags=fl, wants_forces=True)
+            s.init_field(1, 0.04)
+            out.append(s)
+        c = Simulation(gpu, precision, nx=90, ny=90, nz=1, omega=1.6, lid=(0.06, 0.0, 0.0), ramp_steps=20,
+                       flags=cavity_flags(88), wants_forces=True)
+        c.set_solid_fractions(disk_solid_fractions(90, 90, 40.5, 50.5, 9.0))
+        out.append(c)
+        return out
+
+    seed_state = rng.bit_generator.state
+    runs = []
+    for mode in (0, 2):
+        rng.bit_generator.state = seed_state
+        sims = lattices()
+        bt = Batch(sims)
+        bt.set_resident(mode)
+        bt.set_graph_steps(16)
+        bt.set_probe(len(sims) - 1, 9, 6, 30, (20, 62), (30, 72))
+        bt.run([70, 33, 51, 70, 64, 95])
+        bt.run([9, 40, 0, 21, 17, 86])
+        runs.append(([s.state_digest for s in sims], bt.probe_series(len(sims) - 1)))
+    assert runs[0][0] == runs[1][0]
+    assert runs[0][1].shape == (29, 3)          # odd steps 9, 15, ..., 177 of 181
+    assert np.array_equal(runs[0][1], runs[1][1])
