@@ -78,8 +78,8 @@ constexpr int kResidentMaxRounds = 4;
 // A wave that waits this long for a neighbour's exchange word aborts the
 // batch (ticks of the 100 MHz wall clock: one second).
 constexpr uint64_t kResidentSpinTicks = 100000000ull;
-// LDS bytes a resident tile may use (the workgroup's static LDS stays below 256 bytes).
-constexpr size_t kResidentMaxLds = 163840 - 256;
+// LDS bytes a resident tile may use (the workgroup's static LDS stays below 512 bytes).
+constexpr size_t kResidentMaxLds = 163840 - 512;
 
 // Hooks of the recording forward step of the adjoint (walls, Guo forcing,
 // Brinkman porosity, partial cells, LES, sponge).

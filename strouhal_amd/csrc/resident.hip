@@ -400,6 +400,7 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
     extern __shared__ __align__(16) unsigned char smem[];
     T* s = (T*)smem;
     __shared__ int stop;
+    __shared__ float rampL[kProbeSpan];   // the launch's ramp factors, read once
     const ResTile tl = tiles[blockIdx.x];
     const BatchLat& B = lats[tl.lat];
     const int w = tl.w, h = tl.h, pitch = w + 2, nx = B.p.nx, ny = B.p.ny;
@@ -448,8 +449,12 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
             if (B.p.useEps) ceps[j] = ((const GLOBAL float*)B.eps)[n];
         }
     }
-    __syncthreads();
     const int64_t tBase = B.step0 + *base + k0;
+    if (tid < K) {
+        const int64_t t = tBase + tid;                                 // Sim.swift:258-266
+        rampL[tid] = t < B.rampSteps ? ((const GLOBAL float*)B.ramp)[t] : 1.0f;
+    }
+    __syncthreads();
     // the probe schedule followed step by step (one division per launch)
     int64_t tf = INT64_MAX, jf = 0;   // next force step and its sample index
     if (B.fCount > 0) {
@@ -463,7 +468,7 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
         StepParams p = B.p;
         const int par = (int)(t & 1);
         p.parity = par;
-        const float r = t < B.rampSteps ? B.ramp[t] : 1.0f;      // Sim.swift:258-266
+        const float r = rampL[k];
         p.uin = B.uinTarget * r;
         p.ulidX = B.lid[0] * r;
         p.ulidY = B.lid[1] * r;
