@@ -237,42 +237,6 @@ __device__ __forceinline__ void resident_cell(typename Conv::T* __restrict__ s, 
         force[n] = make_float4(ax, ay, az, 0.0f);
 }
 
-// ---- the edge exchange -----------------------------------------------------
-__device__ __forceinline__ void x_put(uint64_t* X, uint32_t slot, uint32_t N, uint32_t n, uint32_t tag,
-                                      uint32_t bits) {
-    __hip_atomic_store(X + (size_t)slot * N + n, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Waits (bounded) for word (slot, n) to carry `tag`; false once the batch is
-// aborted or a second has passed without it (which aborts the batch).
-__device__ __forceinline__ bool x_get(const uint64_t* X, uint32_t slot, uint32_t N, uint32_t n, uint32_t tag,
-                                      uint32_t& bits, int* abortFlag) {
-    const uint64_t* a = X + (size_t)slot * N + n;
-    uint64_t v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)(v >> 32) == tag) { bits = (uint32_t)v; return true; }
-    const uint64_t t0 = wall_clock64();
-    for (;;) {
-        __builtin_amdgcn_s_sleep(2);
-        v = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((uint32_t)(v >> 32) == tag) { bits = (uint32_t)v; return true; }
-        if (__hip_atomic_load(abortFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-        if (wall_clock64() - t0 > (uint64_t)kResidentSpinTicks) {
-            __hip_atomic_store(abortFlag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-    }
-}
-
-template <class T> __device__ __forceinline__ uint32_t raw_bits(T v) {
-    if constexpr (sizeof(T) == 4) return __builtin_bit_cast(uint32_t, v);
-    else return (uint32_t)v;
-}
-template <class T> __device__ __forceinline__ T from_bits(uint32_t b) {
-    if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, b);
-    else return (T)b;
-}
-
 // Direction sets as bit masks over i = 0..18 (bit i): the moving directions
 // with a component in the xy plane, and those with c_x = -1 / +1, c_y = -1 / +1.
 constexpr uint32_t dir_mask(int ax, int v) {
@@ -285,6 +249,29 @@ constexpr uint32_t dir_mask(int ax, int v) {
 }
 constexpr uint32_t kPlanar = dir_mask(0, -1) | dir_mask(0, 1) | dir_mask(1, -1) | dir_mask(1, 1);
 constexpr uint32_t kXm = dir_mask(0, -1), kXp = dir_mask(0, 1), kYm = dir_mask(1, -1), kYp = dir_mask(1, 1);
+
+// ---- the edge exchange -----------------------------------------------------
+// Word of (slot, cell (x, y)): slots whose direction moves along x are kept
+// x-major (index x ny + y), the others y-major (y nx + x), so the words a
+// tile trades across a vertical edge, and the straight ones across a
+// horizontal edge, lie next to each other in memory.
+__device__ __forceinline__ size_t x_word(uint32_t slot, uint32_t N, int x, int y, int nx, int ny) {
+    const bool alongX = ((kXm | kXp) >> slot) & 1u;
+    return (size_t)slot * N + (alongX ? (uint32_t)x * (uint32_t)ny + (uint32_t)y : (uint32_t)y * (uint32_t)nx + (uint32_t)x);
+}
+
+__device__ __forceinline__ void x_put(uint64_t* X, size_t word, uint32_t tag, uint32_t bits) {
+    __hip_atomic_store(X + word, ((uint64_t)tag << 32) | bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T> __device__ __forceinline__ uint32_t raw_bits(T v) {
+    if constexpr (sizeof(T) == 4) return __builtin_bit_cast(uint32_t, v);
+    else return (uint32_t)v;
+}
+template <class T> __device__ __forceinline__ T from_bits(uint32_t b) {
+    if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, b);
+    else return (T)b;
+}
 
 // The planar directions i with (lx + c_ix, ly + c_iy) inside the w x h tile.
 __device__ __forceinline__ uint32_t stays_in(int lx, int ly, int w, int h) {
@@ -317,7 +304,8 @@ __device__ __forceinline__ void x_publish(const T* __restrict__ s, const ResTile
     for (int r = threadIdx.x; r < cnt; r += (int)blockDim.x) {
         int lx, ly;
         if (par == 0) ring_cell(r, w, h, lx, ly); else halo_cell(r, w, h, lx, ly);
-        const uint32_t n = (uint32_t)wrap_coord(tl.y0 + ly, ny) * (uint32_t)nx + (uint32_t)wrap_coord(tl.x0 + lx, nx);
+        const int gx = wrap_coord(tl.x0 + lx, nx), gy = wrap_coord(tl.y0 + ly, ny);
+        const uint32_t n = (uint32_t)gy * (uint32_t)nx + (uint32_t)gx;
         const uint32_t lw = link[n];
         if (lw & kNotFluidBit) continue;
         const uint32_t li = (uint32_t)rl_index(lx, ly, pitch);
@@ -332,7 +320,7 @@ __device__ __forceinline__ void x_publish(const T* __restrict__ s, const ResTile
         while (m != 0u) {
             const int slot = __builtin_ctz(m);
             m &= m - 1u;
-            x_put(X, (uint32_t)slot, N, n, tag, raw_bits<T>(s[li * 19u + (uint32_t)slot]));
+            x_put(X, x_word((uint32_t)slot, N, gx, gy, nx, ny), tag, raw_bits<T>(s[li * 19u + (uint32_t)slot]));
         }
     }
 }
@@ -351,31 +339,51 @@ __device__ __forceinline__ bool x_collect(T* __restrict__ s, const ResTile& tl, 
     for (int r = threadIdx.x; r < cnt; r += (int)blockDim.x) {
         int lx, ly;
         if (par == 0) halo_cell(r, w, h, lx, ly); else ring_cell(r, w, h, lx, ly);
-        const uint32_t n = (uint32_t)wrap_coord(tl.y0 + ly, ny) * (uint32_t)nx + (uint32_t)wrap_coord(tl.x0 + lx, nx);
+        const int gx = wrap_coord(tl.x0 + lx, nx), gy = wrap_coord(tl.y0 + ly, ny);
+        const uint32_t n = (uint32_t)gy * (uint32_t)nx + (uint32_t)gx;
         const uint32_t lw = link[n];
         if (lw & kNotFluidBit) continue;
         const uint32_t li = (uint32_t)rl_index(lx, ly, pitch);
         const uint32_t open = ~((lw & 0x3ffffu) << 1);
         const uint32_t in = stays_in(lx, ly, w, h) & open;
         uint32_t m = par == 0 ? opposite_set(in) : opposite_set(kPlanar & ~stays_in(lx, ly, w, h) & open);
-        // every word of the cell in flight at once (a cell has at most 9),
-        // then the waits
+        // every word of the cell in flight at once (a cell has at most 9);
+        // while some are not there yet, all of those are polled together
         uint64_t v[9];
         int slot[9];
+        size_t word[9];
+        uint32_t pending = 0u;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             slot[k] = m != 0u ? __builtin_ctz(m) : -1;
             m &= m - 1u;
-            if (slot[k] >= 0)
-                v[k] = __hip_atomic_load(X + (size_t)slot[k] * N + n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v[k] = 0u;
+            word[k] = slot[k] >= 0 ? x_word((uint32_t)slot[k], N, gx, gy, nx, ny) : 0u;
+            if (slot[k] >= 0) {
+                v[k] = __hip_atomic_load(X + word[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pending |= 1u << k;
+            }
+        }
+        const uint64_t t0 = pending != 0u ? wall_clock64() : 0u;
+        for (;;) {
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if (((pending >> k) & 1u) && (uint32_t)(v[k] >> 32) == tag) pending &= ~(1u << k);
+            if (pending == 0u) break;
+            if (__hip_atomic_load(abortFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+            if (wall_clock64() - t0 > (uint64_t)kResidentSpinTicks) {
+                __hip_atomic_store(abortFlag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if ((pending >> k) & 1u)
+                    v[k] = __hip_atomic_load(X + word[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            if (slot[k] < 0) continue;
-            uint32_t b = (uint32_t)v[k];
-            if ((uint32_t)(v[k] >> 32) != tag && !x_get(X, (uint32_t)slot[k], N, n, tag, b, abortFlag)) return false;
-            s[li * 19u + (uint32_t)slot[k]] = from_bits<T>(b);
-        }
+        for (int k = 0; k < 9; k++)
+            if (slot[k] >= 0) s[li * 19u + (uint32_t)slot[k]] = from_bits<T>((uint32_t)v[k]);
     }
     return true;
 }
@@ -411,6 +419,73 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
     const GLOBAL uint32_t* lidm = (const GLOBAL uint32_t*)B.lidm;
     const DdfLayout lay = B.p.lay;
     if (tid == 0) stop = 0;
+    const int R = 2 * w + 2 * (h - 2), C = w * h;
+    const int ringRounds = (R + bsz - 1) / bsz, nRounds = (C + bsz - 1) / bsz;
+    const GLOBAL uint32_t* link = (const GLOBAL uint32_t*)B.link;
+    // Work order: the ring in the first rounds, then the interior; inside each
+    // part plain fluid cells first, then partial cells, cells with wall
+    // links, cells with moving-wall links (the corrections) and non-fluid
+    // cells, so that cells taking the same branches share waves. (A cell's update does
+    // not depend on the order.) The order is built in the LDS area the tile
+    // occupies next.
+    __shared__ int bucket[20];
+    uint16_t* order = (uint16_t*)smem;
+    const bool useEps = B.p.useEps != 0;
+    const GLOBAL float* epsG = (const GLOBAL float*)B.eps;
+    auto cell_of = [&](int e, int& lx, int& ly) {
+        if (e < R) {
+            ring_cell(e, w, h, lx, ly);
+        } else {
+            const uint32_t q = fdiv((uint32_t)(e - R), tl.divi);
+            lx = 1 + (int)((uint32_t)(e - R) - q * tl.divi.d);
+            ly = 1 + (int)q;
+        }
+    };
+    auto kind_of = [&](int e) -> int {
+        int lx, ly;
+        cell_of(e, lx, ly);
+        const uint32_t n = (uint32_t)(tl.y0 + ly) * (uint32_t)nx + (uint32_t)(tl.x0 + lx);
+        const uint32_t lw = link[n];
+        int k = 0;
+        if (lw & kNotFluidBit) k = 4;
+        else if ((lw & 0x3ffffu) != 0u) k = lidm[n] != 0u ? 3 : 2;
+        else if (useEps && epsG[n] > 0.0f) k = 1;
+        return (e < R ? 0 : 5) + k;
+    };
+    if (tid < 20) bucket[tid] = 0;
+    __syncthreads();
+    for (int e = tid; e < C; e += bsz) atomicAdd(&bucket[kind_of(e)], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int b = 0; b < 10; b++) { const int c = bucket[b]; bucket[b] = acc; bucket[10 + b] = 0; acc += c; }
+    }
+    __syncthreads();
+    for (int e = tid; e < C; e += bsz) {
+        const int b = kind_of(e);
+        order[bucket[b] + atomicAdd(&bucket[10 + b], 1)] = (uint16_t)e;
+    }
+    __syncthreads();
+    // this thread's cells: tile coordinates, link word, solid fraction
+    int cpos[4];
+    uint32_t clw[4];
+    float ceps[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = j * bsz + tid;
+        cpos[j] = 0;
+        clw[j] = kNotFluidBit;   // no cell: skipped like a non-fluid one
+        ceps[j] = 0.0f;
+        if (j < nRounds && q < C) {
+            int lx, ly;
+            cell_of((int)order[q], lx, ly);
+            const uint32_t n = (uint32_t)(tl.y0 + ly) * (uint32_t)nx + (uint32_t)(tl.x0 + lx);
+            cpos[j] = lx | (ly << 16);
+            clw[j] = link[n];
+            if (useEps) ceps[j] = epsG[n];
+        }
+    }
+    __syncthreads();   // the order is read: the tile may overwrite it
     // the tile and its halo ring from the image
     for (uint32_t c = (uint32_t)tid; c < cs; c += (uint32_t)bsz) {
         const int ly = (int)(c / (uint32_t)pitch) - 1, lx = (int)(c % (uint32_t)pitch) - 1;
@@ -421,33 +496,6 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
         for (int i = 0; i < 19; i++) v[i] = f[(size_t)e + (size_t)i * lay.W];
 #pragma unroll
         for (int i = 0; i < 19; i++) s[c * 19u + (uint32_t)i] = v[i];
-    }
-    const int R = 2 * w + 2 * (h - 2), C = w * h;
-    const int ringRounds = (R + bsz - 1) / bsz, nRounds = (C + bsz - 1) / bsz;
-    // this thread's cells: tile coordinates, link word, solid fraction
-    int cpos[4];
-    uint32_t clw[4];
-    float ceps[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int e = j * bsz + tid;
-        cpos[j] = 0;
-        clw[j] = kNotFluidBit;   // no cell: skipped like a non-fluid one
-        ceps[j] = 0.0f;
-        if (j < nRounds && e < C) {
-            int lx, ly;
-            if (e < R) {
-                ring_cell(e, w, h, lx, ly);
-            } else {
-                const uint32_t q = fdiv((uint32_t)(e - R), tl.divi);
-                lx = 1 + (int)((uint32_t)(e - R) - q * tl.divi.d);
-                ly = 1 + (int)q;
-            }
-            const uint32_t n = (uint32_t)(tl.y0 + ly) * (uint32_t)nx + (uint32_t)(tl.x0 + lx);
-            cpos[j] = lx | (ly << 16);
-            clw[j] = ((const GLOBAL uint32_t*)B.link)[n];
-            if (B.p.useEps) ceps[j] = ((const GLOBAL float*)B.eps)[n];
-        }
     }
     const int64_t tBase = B.step0 + *base + k0;
     if (tid < K) {
