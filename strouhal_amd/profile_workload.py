@@ -19,6 +19,13 @@ Usage (under rocprofv3, or alone):
     python3 strouhal_amd/profile_workload.py --precision fp16s --replay 60 --warmup 4
     python3 strouhal_amd/profile_workload.py --precision fp32 --copy 50
     python3 strouhal_amd/profile_workload.py --precision fp32 --stream-ceiling
+    python3 strouhal_amd/profile_workload.py --precision fp32 --resident
+
+--resident times the multi-lattice batch on lattices one cell deep, with the
+streaming kernel (resident mode 0) and with resident launches (mode 2), in
+microseconds per step: the ladder's finest rung (1410 x 264: walls, inflow,
+NT cylinder, sponge) and the same size with one feature in every cell
+(plain periodic, partial cells, sponge, wall links, moving-wall links).
 
 --stream-ceiling times a kernel that moves the step's bytes in the step's
 pattern (19 slots read, 19 written, blocked layout, nontemporal) and does no
@@ -52,6 +59,47 @@ def _stats(ms, first_parity: int = 0) -> dict:
             "ms": [round(float(x), 4) for x in ms]}
 
 
+def resident_costs(precision: str, steps: int = 1024) -> dict:
+    """us per step of one-lattice batches, streaming (mode 0) vs resident (mode 2)."""
+    from strouhal_amd import Simulation
+    from strouhal_amd.batch import Batch
+    from strouhal_amd.cases import vortex_street
+    nx, ny = 1410, 264
+    lam_m = 1.5 if precision == "fp32" else None
+
+    def lattice(kind):
+        if kind == "street":
+            return vortex_street(None, precision, D=64, uin_max=0.5 * 1.5 * 0.1 / math.sqrt(3.0),
+                                 lam=3.0 / 16.0 if precision == "fp32" else None).sim
+        fl = np.zeros((1, ny, nx), np.uint8)
+        if kind == "walls":
+            fl[:, :, ::3] = 1       # a solid column every third: every fluid cell has wall links
+        if kind == "moving_walls":
+            fl[:, :, ::3] = 3       # inflow columns: wall links with velocity corrections
+        s = Simulation(None, precision, nx=nx, ny=ny, nz=1, omega=1.9, omega_minus=lam_m, uin=0.05,
+                       wants_forces=True, flags=fl)
+        s.init_field(1, 0.04)
+        if kind == "partial":
+            s.set_solid_fractions(np.full((1, ny, nx), 0.3, np.float32))
+        if kind == "sponge":
+            s.sponge = (-1.0, float(nx), 1.0)
+        return s
+
+    out = {}
+    for kind in ("street", "plain", "partial", "sponge", "walls", "moving_walls"):
+        row = {}
+        for mode in (0, 2):
+            b = Batch([lattice(kind)])
+            b.set_resident(mode)
+            b.run([64])
+            g0 = b.gpu_seconds
+            b.run([steps])
+            row["streaming" if mode == 0 else "resident"] = round((b.gpu_seconds - g0) / steps * 1e6, 3)
+            b.close()
+        out[kind] = row
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16s", "bf16"])
@@ -64,6 +112,8 @@ def main():
                     help="time this many device-to-device copies of the DDF image (a bandwidth ceiling)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="time the no-arithmetic 19-slot pass over an n^3 image (see above)")
+    ap.add_argument("--resident", action="store_true",
+                    help="streaming vs resident batch kernel on lattices one cell deep (see above)")
     ap.add_argument("--case", default="tgv", choices=["tgv", "sphere", "street", "cavity", "channel"],
                     help="tgv: the bench's n^3 Taylor-Green vortex; sphere: the 512x256x256 LES "
                          "sphere (D 40, NT cells, inlets, sponge); street: the ladder's finest "
@@ -71,6 +121,10 @@ def main():
                          "cavity with its wall frame (SRT); channel: the 512x256 porous channel")
     args = ap.parse_args()
     n = args.n
+    if args.resident:
+        print(json.dumps({"case": "resident", "precision": args.precision,
+                          "us_per_step": resident_costs(args.precision)}), flush=True)
+        return
     if args.stream_ceiling:
         import ctypes
         sl = ctypes.CDLL(os.path.join(ROOT, "strouhal_amd", "libstrouhal_stream.so"))
