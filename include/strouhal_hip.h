@@ -230,6 +230,10 @@ STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);
  * is batched. A stalled exchange (which a correct build never produces)
  * returns STRH_ERR_DEVICE after about a second. */
 STRH_API int strh_batch_set_resident(strh_batch* batch, int mode);
+/* How long a resident launch's workgroup waits for a neighbour's exchange word
+ * before the run is abandoned with STRH_ERR_DEVICE (0 < seconds <= 60; default 1).
+ * After such a failure the batch's lattices are left in an undefined state. */
+STRH_API int strh_batch_set_resident_timeout(strh_batch* batch, double seconds);
 
 /* ---- discrete adjoint (no reference counterpart: the reference has no
  * gradients) ------------------------------------------------------------------

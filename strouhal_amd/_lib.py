@@ -32,7 +32,7 @@ EXPORTS = [
     "strh_sha256_force_portable", "strh_batch_create", "strh_batch_destroy",
     "strh_batch_set_probe", "strh_batch_run", "strh_batch_probe_series",
     "strh_batch_set_block_size", "strh_batch_autotune",
-    "strh_batch_set_graph_steps", "strh_batch_set_resident", "strh_batch_gpu_seconds", "strh_objective_eval", "strh_vjp",
+    "strh_batch_set_graph_steps", "strh_batch_set_resident", "strh_batch_set_resident_timeout", "strh_batch_gpu_seconds", "strh_objective_eval", "strh_vjp",
     "strh_vjp_nested",
 ]
 
@@ -142,6 +142,7 @@ def lib():
         "strh_batch_autotune": [vp, ctypes.POINTER(i32)],
         "strh_batch_set_graph_steps": [vp, i32],
         "strh_batch_set_resident": [vp, i32],
+        "strh_batch_set_resident_timeout": [vp, ctypes.c_double],
         "strh_batch_gpu_seconds": [vp, ctypes.POINTER(ctypes.c_double)],
         "strh_objective_eval": [vp, ctypes.POINTER(Objective), ctypes.POINTER(ctypes.c_double)],
         "strh_vjp": [vp, ctypes.POINTER(Objective), i64, ctypes.POINTER(ctypes.c_double), vp, vp, vp,

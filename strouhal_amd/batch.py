@@ -82,6 +82,11 @@ class Batch:
         that do not qualify)."""
         check(lib().strh_batch_set_resident(self._h, int(mode)))
 
+    def set_resident_timeout(self, seconds: float):
+        """How long a resident workgroup waits for a neighbour's exchange word
+        before the run fails (default 1 s)."""
+        check(lib().strh_batch_set_resident_timeout(self._h, float(seconds)))
+
     @property
     def gpu_seconds(self) -> float:
         v = ctypes.c_double()

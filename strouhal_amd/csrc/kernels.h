@@ -76,7 +76,7 @@ struct ResTile {
 constexpr int kResidentBS = 768;
 constexpr int kResidentMaxRounds = 4;
 // A wave that waits this long for a neighbour's exchange word aborts the
-// batch (ticks of the 100 MHz wall clock: one second).
+// batch by default (ticks of the 100 MHz wall clock: one second).
 constexpr uint64_t kResidentSpinTicks = 100000000ull;
 // LDS bytes a resident tile may use (the workgroup's static LDS stays below 512 bytes).
 constexpr size_t kResidentMaxLds = 163840 - 512;
@@ -150,7 +150,7 @@ hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
 // lds is the largest tile's (w + 2)(h + 2) 19 sizeof(T) bytes.
 hipError_t launch_resident(int precision, bool les, bool forced, const BatchLat* lats, const ResTile* tiles,
                            int nTiles, int bs, size_t lds, const int64_t* base, int k0, int K, int* abortFlag,
-                           hipStream_t s);
+                           uint64_t spinTicks, hipStream_t s);
 // Allows kResidentMaxLds of dynamic LDS for every resident kernel of this
 // precision and returns how many of its workgroups one CU holds at that size.
 hipError_t resident_prepare(int precision, size_t lds, int* perCU);

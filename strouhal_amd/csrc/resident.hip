@@ -329,7 +329,8 @@ __device__ __forceinline__ void x_publish(const T* __restrict__ s, const ResTile
 // from the same cell's link word). False if the batch aborted.
 template <class T, int BS>
 __device__ __forceinline__ bool x_collect(T* __restrict__ s, const ResTile& tl, const BatchLat& B,
-                                          const uint64_t* X, int par, uint32_t tag, int* abortFlag) {
+                                          const uint64_t* X, int par, uint32_t tag, int* abortFlag,
+                                          uint64_t spinTicks) {
     const int w = tl.w, h = tl.h, pitch = w + 2, nx = B.p.nx, ny = B.p.ny;
     const uint32_t N = B.p.N;
     const GLOBAL uint32_t* __restrict__ link = (const GLOBAL uint32_t*)B.link;
@@ -371,7 +372,7 @@ __device__ __forceinline__ bool x_collect(T* __restrict__ s, const ResTile& tl, 
                 if (((pending >> k) & 1u) && (uint32_t)(v[k] >> 32) == tag) pending &= ~(1u << k);
             if (pending == 0u) break;
             if (__hip_atomic_load(abortFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-            if (wall_clock64() - t0 > (uint64_t)kResidentSpinTicks) {
+            if (wall_clock64() - t0 > spinTicks) {
                 __hip_atomic_store(abortFlag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return false;
             }
@@ -402,7 +403,8 @@ template <class V> __device__ __forceinline__ V pick4(const V (&a)[4], int j) {
 template <class Conv, int BS, bool LES, bool FORCE>
 __global__ void __launch_bounds__(BS)
 step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ tiles,
-              const int64_t* __restrict__ base, int k0, int K, int* __restrict__ abortFlag) {
+              const int64_t* __restrict__ base, int k0, int K, int* __restrict__ abortFlag,
+              uint64_t spinTicks) {
     static_assert(kResidentMaxRounds == 4, "pick4 holds four rounds");
     using T = typename Conv::T;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -562,7 +564,7 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
         };
         if (par == 0) pass(std::integral_constant<int, 0>{});
         else pass(std::integral_constant<int, 1>{});
-        if (!x_collect<T, BS>(s, tl, B, B.xchg, par, tag, abortFlag)) stop = 1;
+        if (!x_collect<T, BS>(s, tl, B, B.xchg, par, tag, abortFlag, spinTicks)) stop = 1;
         __syncthreads();
         if (stop) return;
     }
@@ -586,9 +588,9 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
 template <class Conv, int BS>
 static void launch_resident_conv(bool les, bool forced, const BatchLat* lats, const ResTile* tiles, int nTiles,
                                  int bs, size_t lds, const int64_t* base, int k0, int K, int* abortFlag,
-                                 hipStream_t s) {
+                                 uint64_t spinTicks, hipStream_t s) {
 #define STRH_RES(LESV, FV) hipLaunchKernelGGL((step_resident<Conv, BS, LESV, FV>), dim3(nTiles), dim3(bs), lds, s, \
-                                              lats, tiles, base, k0, K, abortFlag)
+                                              lats, tiles, base, k0, K, abortFlag, spinTicks)
     if (les) { if (forced) STRH_RES(true, true); else STRH_RES(true, false); }
     else { if (forced) STRH_RES(false, true); else STRH_RES(false, false); }
 #undef STRH_RES
@@ -596,12 +598,12 @@ static void launch_resident_conv(bool les, bool forced, const BatchLat* lats, co
 
 hipError_t launch_resident(int precision, bool les, bool forced, const BatchLat* lats, const ResTile* tiles,
                            int nTiles, int bs, size_t lds, const int64_t* base, int k0, int K, int* abortFlag,
-                           hipStream_t s) {
+                           uint64_t spinTicks, hipStream_t s) {
     if (bs < 64 || bs > kResidentBS || bs % 64 != 0) return hipErrorInvalidValue;
     switch (precision) {
-        case kFP32: launch_resident_conv<ConvF32, kResidentBS>(les, forced, lats, tiles, nTiles, bs, lds, base, k0, K, abortFlag, s); break;
-        case kFP16S: launch_resident_conv<ConvF16, kResidentBS>(les, forced, lats, tiles, nTiles, bs, lds, base, k0, K, abortFlag, s); break;
-        case kBF16: launch_resident_conv<ConvBF16, kResidentBS>(les, forced, lats, tiles, nTiles, bs, lds, base, k0, K, abortFlag, s); break;
+        case kFP32: launch_resident_conv<ConvF32, kResidentBS>(les, forced, lats, tiles, nTiles, bs, lds, base, k0, K, abortFlag, spinTicks, s); break;
+        case kFP16S: launch_resident_conv<ConvF16, kResidentBS>(les, forced, lats, tiles, nTiles, bs, lds, base, k0, K, abortFlag, spinTicks, s); break;
+        case kBF16: launch_resident_conv<ConvBF16, kResidentBS>(les, forced, lats, tiles, nTiles, bs, lds, base, k0, K, abortFlag, spinTicks, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -912,6 +912,7 @@ struct strh_batch {
     int residentSlots = 0;               // workgroups the device holds at once at the largest tile
     std::vector<uint64_t*> xchg;         // per lattice, [19][N] exchange words, or null
     int* abortFlag = nullptr;            // raised by a resident launch whose exchange stalled
+    uint64_t residentSpin = kResidentSpinTicks;   // wall-clock ticks a wave waits before aborting
     ResTile* tiles = nullptr;            // every resident group's tiles of one run (device)
     size_t tilesCap = 0;
 };
@@ -1046,7 +1047,8 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
             const int K = std::min(kProbeSpan, count - j);
             for (const ResGroup& g : sg.groups)
                 HIP_TRY(launch_resident(B->precision, g.les, g.forced, sg.dev, B->tiles + g.first, g.count, g.bs, g.lds,
-                                        B->base, kOff + j, K, B->abortFlag, s), "resident launch");
+                                        B->base, kOff + j, K, B->abortFlag, B->residentSpin, s),
+                        "resident launch");
             if (sg.probes) HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, kOff + j, K, s), "batch probe launch");
         }
         return STRH_OK;
@@ -1466,6 +1468,17 @@ int strh_batch_set_graph_steps(strh_batch* B, int steps) {
 int strh_batch_set_resident(strh_batch* B, int mode) {
     if (!B || mode < 0 || mode > 2) return fail(STRH_ERR_INVALID, "resident mode must be 0, 1 or 2");
     B->resident = mode;
+    return STRH_OK;
+}
+
+int strh_batch_set_resident_timeout(strh_batch* B, double seconds) {
+    if (!B || !(seconds > 0.0) || seconds > 60.0) return fail(STRH_ERR_INVALID, "timeout must be in (0, 60] seconds");
+    int rc = set_device(B->ctx);
+    if (rc) return rc;
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, B->ctx->device), "wall clock rate");
+    if (khz <= 0) khz = 100000;
+    B->residentSpin = std::max<uint64_t>(1, (uint64_t)(seconds * 1e3 * (double)khz));
     return STRH_OK;
 }
 

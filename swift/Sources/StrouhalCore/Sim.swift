@@ -264,4 +264,7 @@ public final class Batch {
     /// 0: stream every step through memory; 1: hold lattices one cell deep in LDS where the
     /// active ones fit on chip in one launch; 2: everywhere, time-sliced.
     public func setResident(_ mode: Int) throws { try check(strh_batch_set_resident(h, Int32(mode))) }
+    public func setResidentTimeout(seconds: Double) throws {
+        try check(strh_batch_set_resident_timeout(h, seconds))
+    }
 }

@@ -330,3 +330,36 @@ def test_resident_tiles_of_every_shape(gpu, precision):
     assert runs[0][0] == runs[1][0]
     assert runs[0][1].shape == (29, 3)          # odd steps 9, 15, ..., 177 of 181
     assert np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_resident_exchange_timeout_fails_the_run_and_the_device_stays_usable(gpu):
+    """With a 10 ns wait limit a workgroup that finds a neighbour's exchange word
+    not yet written gives up at once and raises the abort flag; every waiting
+    workgroup returns and strh_batch_run reports the failure. The next batch,
+    with the default limit, runs normally."""
+    from strouhal_amd import Simulation
+    from strouhal_amd._lib import StrouhalError
+
+    def lattice():
+        s = Simulation(gpu, "fp32", nx=640, ny=256, nz=1, omega=1.8)
+        s.init_field(1, 0.04)
+        return s
+
+    a = lattice()
+    bt = Batch([a])
+    bt.set_resident(2)
+    with pytest.raises(StrouhalError):
+        bt.set_resident_timeout(0.0)
+    with pytest.raises(StrouhalError):
+        bt.set_resident_timeout(61.0)
+    bt.set_resident_timeout(1e-8)
+    with pytest.raises(StrouhalError):
+        bt.run([256])
+    bt.close()
+    b, c = lattice(), lattice()
+    bb, bc = Batch([b]), Batch([c])
+    bb.set_resident(2)
+    bc.set_resident(0)
+    bb.run([130])
+    bc.run([130])
+    assert b.state_digest == c.state_digest
