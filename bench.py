@@ -8,8 +8,11 @@ two variants: TRT Lambda = 3/16 with FP32 storage (the headline `value`) and
 SRT with FP16 shifted storage. A "step" is one LBM time step of all 256^3
 cells; the DDFs are resident in HBM before timing starts. The line also
 carries the adjoint/forward step-time ratio of the porous-channel objective
-(512x256x1, 6000 steps; strh_vjp against strh_run, HIP events). MLUPS counts every
-cell, as the reference's bench does (Sources/strouhal/Cases.swift:141).
+(512x256x1, 6000 steps; strh_vjp against strh_run, HIP events), and the
+reference's own bench workload and procedure (SRT omega 1.9, 2-D Taylor-Green
+init, 20 warm-up and 1,000 timed steps, FP32 and FP16) beside its published
+Apple-M5 figures. MLUPS counts every cell, as the reference's bench does
+(Sources/strouhal/Cases.swift:141).
 
 The roofline is HBM: algorithmic bytes per cell per step are 19 DDF reads +
 19 DDF writes = 152 B (FP32) / 76 B (FP16). The step kernel's average launch
@@ -39,6 +42,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 REFERENCE_M5_FP32_MLUPS = 785.0   # README.md:24-28 of the reference (Apple M5, 256^3 FP32)
+REFERENCE_M5_FP16_MLUPS = 1445.0  # README.md:29 of the reference (Apple M5, 256^3 FP16)
 
 
 def tgv_params(n: int, u0: float = 0.05, re: float = 1600.0):
@@ -83,6 +87,31 @@ def measure(precision: str, lam, n: int, steps: int, warmup: int):
         "finite": bool(math.isfinite(ke)),
         "bytes_per_cell": 152 if precision == "fp32" else 76,
     }
+
+
+def reference_bench(n: int, warmup: int = 20, steps: int = 1000):
+    """The reference's own bench workload and procedure (Cases.swift:132-151,
+    BASELINE.md section 1): 256^3 periodic, SRT omega = 1.9, initField mode 1
+    (2-D Taylor-Green) at A = 0.05, timed with HIP events over `steps` steps
+    after `warmup` (the measurement plan asks for >= 20 and >= 1,000), in FP32
+    and FP16 storage, beside the reference's published Apple-M5 figures."""
+    from strouhal_amd import Simulation
+    out = {"workload": f"{n}^3 periodic, SRT omega=1.9, initField mode 1 A=0.05",
+           "warmup": warmup, "steps": steps}
+    for precision, published in (("fp32", REFERENCE_M5_FP32_MLUPS), ("fp16s", REFERENCE_M5_FP16_MLUPS)):
+        sim = Simulation(None, precision, nx=n, ny=n, nz=n, omega=1.9)
+        sim.init_field(1, 0.05)
+        sim.run(warmup)
+        g0 = sim.gpu_seconds
+        sim.run(steps)
+        gpu = sim.gpu_seconds - g0
+        sim.close()
+        mlups = n ** 3 * steps / gpu / 1e6
+        b = 152 if precision == "fp32" else 76
+        out[precision] = {"mlups": round(mlups, 1), "ms_per_step": round(gpu / steps * 1e3, 4),
+                          "hbm_frac": round(mlups * 1e6 * b / (HBM_PEAK_GBS * 1e9), 4),
+                          "vs_reference_m5": round(mlups / published, 2)}
+    return out
 
 
 def _cpu_model() -> str:
@@ -187,6 +216,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-reference-bench", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -213,6 +243,7 @@ def main():
         "traffic": traffic,
     }
     fp16_achieved = fp16["bytes_per_cell"] * cells / (fp16["kernel_ms"] * 1e-3) / 1e9
+    ref = None if args.no_reference_bench else reference_bench(n)
     adj = None if args.no_adjoint else adjoint_ratio()
     cpu = None if args.no_cpu_baseline else cpu_baseline(n)
     line = {
@@ -243,6 +274,7 @@ def main():
         "fp16s_roofline": {"achieved": round(fp16_achieved, 1), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(fp16_achieved / HBM_PEAK_GBS, 4)},
         "adjoint": adj,
+        "reference_bench": ref,
         "vs_baseline_ref": "reference README.md:24-28, Apple M5 256^3 FP32 ~785 MLUPS",
         # the reference's own bytes-per-cell model of its bench (19 x 2 DDFs, a
         # 1-byte flag and half of its 8-byte odd-step masks: 157 / 81 B,
