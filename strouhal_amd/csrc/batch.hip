@@ -21,8 +21,9 @@ __device__ __forceinline__ int batch_find(const BatchLat* lats, int L, uint32_t 
 // parity in this launch (equal start parities, even graph lengths), so only
 // that pass is compiled in; PAR = -1 branches per lattice. LES = false when no
 // lattice of the batch has a Smagorinsky constant, FORCED = false when none has
-// a body force (otherwise each lattice takes its own forcing path).
-template <class Conv, int BS, int PAR, bool LES, bool FORCED>
+// a body force (otherwise each lattice takes its own forcing path). PLANAR: every
+// lattice is one cell deep, so the odd pass's 18 neighbours are 8 cells.
+template <class Conv, int BS, int PAR, bool LES, bool FORCED, bool PLANAR>
 __global__ void __launch_bounds__(BS)
 step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k) {
     const uint32_t b = blockIdx.x;
@@ -57,8 +58,10 @@ step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
             step_cell<St, 0, true, true, LES, false, false, false, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
     } else if (PAR != 0) {
         using St = Store<Conv, kPolShifted>;
-        if (FORCED && B.hasForce) step_cell<St, 1, true, true, LES, true>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
-        else step_cell<St, 1, true, true, LES, false>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
+        if (FORCED && B.hasForce)
+            step_cell<St, 1, true, true, LES, true, false, false, true, PLANAR>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
+        else
+            step_cell<St, 1, true, true, LES, false, false, false, true, PLANAR>(f, B.flags, B.link, B.lidm, fo, B.eps, p, n);
     }
 }
 
@@ -112,41 +115,44 @@ __global__ void set_base(int64_t* base, int64_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *base = v;
 }
 
-template <class Conv, int BS, int PAR>
+template <class Conv, int BS, int PAR, bool PLANAR>
 static void launch_batch_par(bool les, bool forced, const BatchLat* lats, int L, uint32_t totalBlocks,
                              const int64_t* base, int k, hipStream_t s) {
-#define STRH_BATCH(LESV, FV) hipLaunchKernelGGL((step_batch<Conv, BS, PAR, LESV, FV>), dim3(totalBlocks), \
+#define STRH_BATCH(LESV, FV) hipLaunchKernelGGL((step_batch<Conv, BS, PAR, LESV, FV, PLANAR>), dim3(totalBlocks), \
                                                 dim3(BS), 0, s, lats, L, base, k)
     if (les) { if (forced) STRH_BATCH(true, true); else STRH_BATCH(true, false); }
     else { if (forced) STRH_BATCH(false, true); else STRH_BATCH(false, false); }
 #undef STRH_BATCH
 }
 template <class Conv, int BS>
-static void launch_batch_conv(int parity, bool les, bool forced, const BatchLat* lats, int L,
+static void launch_batch_conv(int parity, bool les, bool forced, bool planar, const BatchLat* lats, int L,
                               uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
-    if (parity == 0) launch_batch_par<Conv, BS, 0>(les, forced, lats, L, totalBlocks, base, k, s);
-    else if (parity == 1) launch_batch_par<Conv, BS, 1>(les, forced, lats, L, totalBlocks, base, k, s);
-    else hipLaunchKernelGGL((step_batch<Conv, BS, -1, true, true>), dim3(totalBlocks), dim3(BS), 0, s,
+    if (parity == 0) launch_batch_par<Conv, BS, 0, false>(les, forced, lats, L, totalBlocks, base, k, s);
+    else if (parity == 1 && planar) launch_batch_par<Conv, BS, 1, true>(les, forced, lats, L, totalBlocks, base, k, s);
+    else if (parity == 1) launch_batch_par<Conv, BS, 1, false>(les, forced, lats, L, totalBlocks, base, k, s);
+    else hipLaunchKernelGGL((step_batch<Conv, BS, -1, true, true, false>), dim3(totalBlocks), dim3(BS), 0, s,
                             lats, L, base, k);
 }
 template <int BS>
-static hipError_t launch_batch_bs(int precision, int parity, bool les, bool forced, const BatchLat* lats,
-                                  int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
+static hipError_t launch_batch_bs(int precision, int parity, bool les, bool forced, bool planar,
+                                  const BatchLat* lats, int L, uint32_t totalBlocks, const int64_t* base, int k,
+                                  hipStream_t s) {
     switch (precision) {
-        case kFP32: launch_batch_conv<ConvF32, BS>(parity, les, forced, lats, L, totalBlocks, base, k, s); break;
-        case kFP16S: launch_batch_conv<ConvF16, BS>(parity, les, forced, lats, L, totalBlocks, base, k, s); break;
-        case kBF16: launch_batch_conv<ConvBF16, BS>(parity, les, forced, lats, L, totalBlocks, base, k, s); break;
+        case kFP32: launch_batch_conv<ConvF32, BS>(parity, les, forced, planar, lats, L, totalBlocks, base, k, s); break;
+        case kFP16S: launch_batch_conv<ConvF16, BS>(parity, les, forced, planar, lats, L, totalBlocks, base, k, s); break;
+        case kBF16: launch_batch_conv<ConvBF16, BS>(parity, les, forced, planar, lats, L, totalBlocks, base, k, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, const BatchLat* lats,
-                             int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s) {
+hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, bool planar,
+                             const BatchLat* lats, int L, uint32_t totalBlocks, const int64_t* base, int k,
+                             hipStream_t s) {
     switch (bs) {
-        case 128: return launch_batch_bs<128>(precision, parity, les, forced, lats, L, totalBlocks, base, k, s);
-        case 256: return launch_batch_bs<256>(precision, parity, les, forced, lats, L, totalBlocks, base, k, s);
-        case 512: return launch_batch_bs<512>(precision, parity, les, forced, lats, L, totalBlocks, base, k, s);
+        case 128: return launch_batch_bs<128>(precision, parity, les, forced, planar, lats, L, totalBlocks, base, k, s);
+        case 256: return launch_batch_bs<256>(precision, parity, les, forced, planar, lats, L, totalBlocks, base, k, s);
+        case 512: return launch_batch_bs<512>(precision, parity, les, forced, planar, lats, L, totalBlocks, base, k, s);
     }
     return hipErrorInvalidValue;
 }

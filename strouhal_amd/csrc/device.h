@@ -489,7 +489,7 @@ __device__ __forceinline__ void record_populations(const AdjHooks& hk, const flo
 // One cell of one AA pass (the body shared by the single-lattice kernel and
 // the multi-lattice batch kernel).
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
-          bool HOOK = false, bool OUT = true>
+          bool HOOK = false, bool OUT = true, bool PLANAR = false>
 __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
                                           const uint8_t* __restrict__ flags_,
                                           const uint32_t* __restrict__ link_,
@@ -561,9 +561,12 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         if (sM != 0u) lM = lidm[n];
     }
     if (NT && p.useEps) eps = epsBuf[n];
+    // PLANAR: the lattice is one cell deep (nz = 1), so the z shift of every
+    // direction wraps onto the cell's own plane: the 18 neighbours are 8
+    // distinct cells (and the cell itself), whose offsets are computed once.
     const uint32_t t = fdiv(n, p.divx);
     const int x = (int)(n - t * (uint32_t)p.nx);
-    const uint32_t z = fdiv(t, p.divy);
+    const uint32_t z = PLANAR ? 0u : fdiv(t, p.divy);
     const int y = (int)(t - z * (uint32_t)p.ny);
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int plane = nx * ny;
@@ -572,8 +575,8 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     o.xp = (x == nx - 1) ? 1 - nx : 1;
     o.ym = (y == 0) ? (ny - 1) * nx : -nx;
     o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
-    o.zm = ((int)z == 0) ? (nz - 1) * plane : -plane;
-    o.zp = ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
+    o.zm = PLANAR ? 0 : ((int)z == 0) ? (nz - 1) * plane : -plane;
+    o.zp = PLANAR ? 0 : ((int)z == nz - 1) ? -(nz - 1) * plane : plane;
     const int ni = (int)n;
     // byte offset of the neighbour n + d from the slot bases
     auto nbo = [&](int d) -> uint32_t { return cell_elem(p.lay, (uint32_t)(ni + d)) * (uint32_t)TB; };
@@ -667,15 +670,15 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
     // (the coordinates and the six axis offsets are recomputed from it too)
     const uint32_t tp = fdiv((uint32_t)nip, p.divx);
     const int xp = nip - (int)(tp * (uint32_t)nx);
-    const uint32_t zp = fdiv(tp, p.divy);
+    const uint32_t zp = PLANAR ? 0u : fdiv(tp, p.divy);
     const int yp = (int)(tp - zp * (uint32_t)ny);
     Offsets op;
     op.xm = (xp == 0) ? nx - 1 : -1;
     op.xp = (xp == nx - 1) ? 1 - nx : 1;
     op.ym = (yp == 0) ? (ny - 1) * nx : -nx;
     op.yp = (yp == ny - 1) ? -(ny - 1) * nx : nx;
-    op.zm = ((int)zp == 0) ? (nz - 1) * plane : -plane;
-    op.zp = ((int)zp == nz - 1) ? -(nz - 1) * plane : plane;
+    op.zm = PLANAR ? 0 : ((int)zp == 0) ? (nz - 1) * plane : -plane;
+    op.zp = PLANAR ? 0 : ((int)zp == nz - 1) ? -(nz - 1) * plane : plane;
     // ... and the slot bases from a pinned copy of f, so the pull's 19 base
     // pointers (38 SGPRs) are not held across the collision either
     T* fpin = f;
@@ -739,7 +742,7 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
 }
 
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
-          bool OUT = true>
+          bool OUT = true, bool PLANAR = false>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
@@ -747,8 +750,8 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             const float* __restrict__ alphaBuf) {
     const uint32_t n = blockIdx.x * kStepBS + threadIdx.x;
     if (n >= p.N) return;
-    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK, false, OUT>(f, flags, link, lidm, force,
-                                                                    epsBuf, p, n, alphaBuf);
+    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK, false, OUT, PLANAR>(f, flags, link, lidm, force,
+                                                                            epsBuf, p, n, alphaBuf);
 }
 
 // Forward step with the adjoint hooks (Guo forcing with Brinkman porosity, or

@@ -27,6 +27,7 @@ struct StepLaunch {
     bool has_force;  // body force non-zero
     uint32_t rows_bs;  // 256 / 512: odd pass uses LDS row staging; 0: direct
     bool large;        // blocked image of 4 GiB or more (row-staged pass with row bases)
+    bool planar;       // one cell deep (nz = 1): the direct odd pass's 18 neighbours are 8 cells
     const float* alpha;  // Brinkman porosity per cell, or null
     StepParams p;
 };
@@ -140,7 +141,8 @@ hipError_t launch_sum_doubles(const double* v, uint32_t M, double* partials, uin
 // parity: 0 / 1 when every active lattice is at that parity in this launch,
 // -1 to branch per lattice (then les and forced are taken as true); les: some
 // lattice has a Smagorinsky constant; forced: some lattice has a body force.
-hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, const BatchLat* lats,
+hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool forced, bool planar,
+                             const BatchLat* lats,
                              int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
 // Probes every force step among steps k .. k + span - 1 (span <= kProbeSpan).
 constexpr int kProbeSpan = 64;

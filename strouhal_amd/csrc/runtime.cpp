@@ -214,6 +214,7 @@ StepLaunch make_launch(const strh_lattice* L, int64_t t, bool writeForce) {
     s.has_force = !(L->bodyForce[0] == 0.0f && L->bodyForce[1] == 0.0f && L->bodyForce[2] == 0.0f);
     s.rows_bs = L->walls ? 0u : rows_block_size((uint32_t)L->nx, L->N);
     s.large = large_image(L);
+    s.planar = L->nz == 1;
     s.alpha = L->alpha;
     s.p = make_params(L, t, writeForce);
     return s;
@@ -976,6 +977,7 @@ struct Segment {
     int parity0;             // parity of every active lattice at k0, or -1 if they differ
     bool les;                // some active lattice has a Smagorinsky constant
     bool forced;             // some active lattice has a body force
+    bool planar;             // every active lattice is one cell deep
     bool probes;             // some active lattice has force samples
     std::vector<ResGroup> groups;   // resident launches, or empty for the streaming kernel
 };
@@ -1057,7 +1059,7 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
     for (int j = 0; j < count; j++) {
         const int k = kOff + j;
         const int par = parity0 < 0 ? -1 : (parity0 + j) & 1;
-        HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.dev, sg.L, sg.blocks, B->base, k, s),
+        HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.planar, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
         if (sg.probes && (k + 1 - k0 == kProbeSpan || j == count - 1)) {
             HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k0, k + 1 - k0, s), "batch probe launch");
@@ -1184,6 +1186,7 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         sg.parity0 = -2;
         sg.les = false;
         sg.forced = false;
+        sg.planar = true;
         sg.probes = false;
         std::vector<int> members;
         for (int i = 0; i < n; i++) {
@@ -1193,6 +1196,7 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             sg.parity0 = sg.parity0 == -2 ? par : (sg.parity0 == par ? par : -1);
             sg.les = sg.les || B->lats[i]->cSmago > 0.0f;
             sg.forced = sg.forced || d.hasForce != 0;
+            sg.planar = sg.planar && B->lats[i]->nz == 1;
             d.block0 = sg.blocks;
             d.nblocks = (B->lats[i]->N + (uint32_t)B->bs - 1) / (uint32_t)B->bs;
             if (!probes) d.fCount = 0;

@@ -8,7 +8,7 @@
 
 namespace strh {
 
-template <class Conv, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+template <class Conv, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool PLANAR>
 static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
     using T = typename Conv::T;
     if (PARITY == 1 && !WALLS && L.rows_bs != 0 && !L.alpha) {
@@ -31,7 +31,7 @@ static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
     p.blocks = (p.N + kStepBS - 1) / kStepBS;
     if (L.alpha) {   // Brinkman lattices: forcing path with per-cell force, no NT / LES
         if (NT || LES) return hipErrorNotSupported;
-        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, false, false, true, true>), dim3(p.blocks),
+        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, false, false, true, true, true, PLANAR>), dim3(p.blocks),
                            dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
                            (float4*)L.force, L.eps, p, L.alpha);
         return hipGetLastError();
@@ -39,20 +39,27 @@ static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
     // Steps that write no force run the kernel with the force output compiled
     // out (only walls and partial cells have one).
     if ((WALLS || NT) && p.writeForce == 0)
-        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE, false, false>), dim3(p.blocks),
+        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE, false, false, PLANAR>), dim3(p.blocks),
                            dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
                            (float4*)L.force, L.eps, p, (const float*)nullptr);
     else
-        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE>), dim3(p.blocks),
+        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE, false, true, PLANAR>), dim3(p.blocks),
                            dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
                            (float4*)L.force, L.eps, p, (const float*)nullptr);
     return hipGetLastError();
 }
 
+// The odd pass of a lattice one cell deep is instantiated with PLANAR (its z
+// shifts wrap onto the cell's own plane); the even pass reads no neighbour.
+template <class Conv, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE>
+static hipError_t pick_planar(const StepLaunch& L, hipStream_t s) {
+    return (PARITY == 1 && L.planar) ? launch_step_t<Conv, PARITY, WALLS, NT, LES, FORCE, PARITY == 1>(L, s)
+                                     : launch_step_t<Conv, PARITY, WALLS, NT, LES, FORCE, false>(L, s);
+}
 template <class Conv, int PARITY, bool WALLS, bool NT, bool LES>
 static hipError_t pick_force(const StepLaunch& L, hipStream_t s) {
-    return L.has_force ? launch_step_t<Conv, PARITY, WALLS, NT, LES, true>(L, s)
-                       : launch_step_t<Conv, PARITY, WALLS, NT, LES, false>(L, s);
+    return L.has_force ? pick_planar<Conv, PARITY, WALLS, NT, LES, true>(L, s)
+                       : pick_planar<Conv, PARITY, WALLS, NT, LES, false>(L, s);
 }
 template <class Conv, int PARITY, bool WALLS, bool NT>
 static hipError_t pick_les(const StepLaunch& L, hipStream_t s) {
