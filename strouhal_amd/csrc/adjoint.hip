@@ -31,7 +31,9 @@ template <int I> __device__ __forceinline__ float wgt_f() {
 // mu = P^T lambda^{t+1} for cell n: the pull from each neighbour, or at a
 // wall link the cell's own opposite slot (negated for the anti-bounce-back
 // outlet). Returns the cell's share of dJ/d(uin target) from inflow links.
-template <bool WALLS>
+// PLANAR: the lattice is one cell deep, so every z shift is the cell's own
+// plane and the 18 neighbours are 8 cells.
+template <bool WALLS, bool PLANAR>
 __device__ __forceinline__ double adj_gather(const AdjStep& q, uint32_t n, int x, int y, int z,
                                              const float* __restrict__ lamN,
                                              const uint8_t* __restrict__ flags, uint32_t sM,
@@ -44,8 +46,8 @@ __device__ __forceinline__ double adj_gather(const AdjStep& q, uint32_t n, int x
     o.xp = (x == nx - 1) ? 1 - nx : 1;
     o.ym = (y == 0) ? (ny - 1) * nx : -nx;
     o.yp = (y == ny - 1) ? -(ny - 1) * nx : nx;
-    o.zm = (z == 0) ? (nz - 1) * plane : -plane;
-    o.zp = (z == nz - 1) ? -(nz - 1) * plane : plane;
+    o.zm = PLANAR ? 0 : (z == 0) ? (nz - 1) * plane : -plane;
+    o.zp = PLANAR ? 0 : (z == nz - 1) ? -(nz - 1) * plane : plane;
     double gu = 0.0;
     mu[0] = ld_off(lamN, n * 4u);
     // unconditional loads (a wall link reads the cell's own opposite slot),
@@ -130,7 +132,7 @@ __device__ __forceinline__ void mu_moments(const float (&mu)[19], float& M0, flo
 #undef STRH_MSUM
 }
 
-template <bool WALLS, bool NT>
+template <bool WALLS, bool NT, bool PLANAR>
 __global__ void __launch_bounds__(256)
 adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ lamN,
                     float* __restrict__ lam, const uint8_t* __restrict__ flags,
@@ -150,10 +152,10 @@ adjoint_step_kernel(const float4* __restrict__ rec, const float* __restrict__ la
     const uint32_t n4 = n * 4u;
     const uint32_t t = fdiv(n, q.divx);
     const int x = (int)(n - t * q.nx);
-    const uint32_t z = fdiv(t, q.divy);
+    const uint32_t z = PLANAR ? 0u : fdiv(t, q.divy);
     const int y = (int)(t - z * q.ny);
     float mu[19];
-    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, sM, mu);
+    const double gu = adj_gather<WALLS, PLANAR>(q, n, x, y, (int)z, lamN, flags, sM, mu);
 
     const float4 m = ld_off(rec, n * 16u);
     const float a = alphaBuf ? alphaBuf[n] : 0.0f;
@@ -323,7 +325,7 @@ adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __re
     const uint32_t z = fdiv(t, q.divy);
     const int y = (int)(t - z * q.ny);
     float mu[19];
-    const double gu = adj_gather<WALLS>(q, n, x, y, (int)z, lamN, flags, sM, mu);
+    const double gu = adj_gather<WALLS, false>(q, n, x, y, (int)z, lamN, flags, sM, mu);
 
     float g[19];
 #pragma unroll
@@ -553,14 +555,14 @@ adjoint_les_kernel(const typename Conv::T* __restrict__ rec19, const float* __re
 }
 
 
-template <class Conv, int PARITY, bool WALLS>
+template <class Conv, int PARITY, bool WALLS, bool PLANAR>
 static hipError_t launch_hooked_t(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
     using T = typename Conv::T;
     using St = Store<Conv, PARITY == 0 ? kPolAligned : kPolShifted>;
     StepParams p = L.p;
     p.blocks = (p.N + kStepBS - 1) / kStepBS;
     // porosity excludes partial cells and LES
-#define STRH_HOOKED(NTV, LESV, BR, FO) hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, NTV, LESV, BR, FO>), \
+#define STRH_HOOKED(NTV, LESV, BR, FO) hipLaunchKernelGGL((step_kernel_hooked<St, PARITY, WALLS, NTV, LESV, BR, FO, PLANAR>), \
         dim3(p.blocks), dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm, L.eps, p, L.alpha, hk)
     // the forcing path only with a body force, as the plain step's kernel
     // choice (porosity always uses it)
@@ -575,8 +577,11 @@ static hipError_t launch_hooked_t(const StepLaunch& L, const AdjHooks& hk, hipSt
 template <class Conv>
 static hipError_t launch_hooked_c(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
     if (L.p.parity == 0)
-        return L.walls ? launch_hooked_t<Conv, 0, true>(L, hk, s) : launch_hooked_t<Conv, 0, false>(L, hk, s);
-    return L.walls ? launch_hooked_t<Conv, 1, true>(L, hk, s) : launch_hooked_t<Conv, 1, false>(L, hk, s);
+        return L.walls ? launch_hooked_t<Conv, 0, true, false>(L, hk, s) : launch_hooked_t<Conv, 0, false, false>(L, hk, s);
+    // the odd pass of a lattice one cell deep: 8 neighbour cells
+    if (L.planar)
+        return L.walls ? launch_hooked_t<Conv, 1, true, true>(L, hk, s) : launch_hooked_t<Conv, 1, false, true>(L, hk, s);
+    return L.walls ? launch_hooked_t<Conv, 1, true, false>(L, hk, s) : launch_hooked_t<Conv, 1, false, false>(L, hk, s);
 }
 
 hipError_t launch_step_hooked(const StepLaunch& L, const AdjHooks& hk, hipStream_t s) {
@@ -619,12 +624,15 @@ hipError_t launch_adjoint_step(const AdjStep& q0, bool walls, const float4* rec,
 #undef STRH_ADJL
         return hipGetLastError();
     }
-#define STRH_ADJ(W, NTV) hipLaunchKernelGGL((adjoint_step_kernel<W, NTV>), g, b, 0, s, rec, lamNext, lam, \
-                                            flags, link, alpha, gAlpha, gUin, eps, ntIdx, recNt, gEps, q)
+#define STRH_ADJ(W, NTV, PL) hipLaunchKernelGGL((adjoint_step_kernel<W, NTV, PL>), g, b, 0, s, rec, lamNext, lam, \
+                                                flags, link, alpha, gAlpha, gUin, eps, ntIdx, recNt, gEps, q)
+    const bool planar = q.nz == 1;
     if (ntIdx) {
-        if (walls) STRH_ADJ(true, true); else STRH_ADJ(false, true);
+        if (walls) { if (planar) STRH_ADJ(true, true, true); else STRH_ADJ(true, true, false); }
+        else { if (planar) STRH_ADJ(false, true, true); else STRH_ADJ(false, true, false); }
     } else {
-        if (walls) STRH_ADJ(true, false); else STRH_ADJ(false, false);
+        if (walls) { if (planar) STRH_ADJ(true, false, true); else STRH_ADJ(true, false, false); }
+        else { if (planar) STRH_ADJ(false, false, true); else STRH_ADJ(false, false, false); }
     }
 #undef STRH_ADJ
     return hipGetLastError();

@@ -757,7 +757,7 @@ step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
 // Forward step with the adjoint hooks (Guo forcing with Brinkman porosity, or
 // partial cells and / or LES). The arithmetic is that of step_kernel: with
 // alpha = 0 and no body force the DDFs are the plain update's bits.
-template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool BRINK, bool FORCE>
+template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool BRINK, bool FORCE, bool PLANAR = false>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
                    const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
@@ -768,8 +768,8 @@ step_kernel_hooked(typename St::T* __restrict__ f, const uint8_t* __restrict__ f
     // (writes no force: the force pointer is null and writeForce 0; porosity
     // acts through the forcing path, so BRINK comes with FORCE)
     static_assert(FORCE || !BRINK, "porosity needs the forcing path");
-    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK, true, false>(f, flags, link, lidm, nullptr, epsBuf,
-                                                                     p, n, alphaBuf, &hk);
+    step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK, true, false, PLANAR>(f, flags, link, lidm, nullptr,
+                                                                             epsBuf, p, n, alphaBuf, &hk);
 }
 
 // ---- odd pass with LDS row staging (wall-free lattices) --------------------
