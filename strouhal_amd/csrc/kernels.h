@@ -52,6 +52,7 @@ struct BatchLat {
     const float* ramp;       // ramp factor for t < rampSteps (host-evaluated)
     int64_t rampSteps;
     int32_t hasForce;
+    int32_t inflowRows;      // INFLOW cells with the parabolic profile: resident tiles tabulate it per row
     // momentum-exchange probe schedule: odd steps fT0 + j*fPeriod, j < fCount
     int64_t fT0, fPeriod, fCount;
     uint32_t bx0, by0;

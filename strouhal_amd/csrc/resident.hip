@@ -86,13 +86,39 @@ __device__ __forceinline__ bool inside(int lx, int ly, int w, int h) {
 // reads from the cell's own slots and stores the bounce-back into them. The
 // neighbour slots touched along a wall link belong to non-fluid cells, which
 // no fluid cell reads, the exchange skips and the write-back leaves alone.
-// lw and eps are the cell's link word and solid fraction (0 without one).
+// Wall velocity (Kernels.metal:81-87) of a wall cell dy rows from the cell
+// whose row's parabolic inflow value is prow[0]: the same arithmetic as
+// wall_vel, with the division done once per tile row and step.
+__device__ __forceinline__ void wall_vel_row(uint8_t wf, const float* prow, int dy, const StepParams& p,
+                                             float& ux, float& uy, float& uz) {
+    if (wf == kLid) { ux = p.ulidX; uy = p.ulidY; uz = p.ulidZ; return; }
+    uy = 0.0f; uz = 0.0f;
+    ux = p.inflowUniform != 0 ? p.uin : prow[dy];
+}
+
+// The parabolic inflow value max(4 uin yd (H - yd) / H^2, 0) of every row of a
+// tile and its halo (wall_vel's expression, bit for bit) for one step.
+__device__ __forceinline__ void inflow_rows(float* prof, int rows, int y0, int ny, float uin, int tid, int bsz) {
+    for (int r = tid; r < rows; r += bsz) {
+        const int wy = wrap_coord(y0 + r - 1, ny);
+        const float H = (float)ny - 2.0f;
+        const float yd = (float)wy - 0.5f;
+        const float v = 4.0f * uin * yd * (H - yd) / (H * H);
+        prof[r] = v > 0.0f ? v : 0.0f;
+    }
+}
+
+// lw and eps are the cell's link word and solid fraction (0 without one), lM
+// its moving-wall mask (0 without wall links); flL holds the flags of the tile
+// and its halo in LDS (index li + dx + dy pitch for the neighbour (dx, dy)),
+// and prow the parabolic inflow value of the cell's row in LDS (the rows
+// above and below at prow[1], prow[-1]), so a moving-wall correction waits on
+// no global load and divides nothing.
 template <class Conv, int PARITY, bool LES, bool FORCE>
 __device__ __forceinline__ void resident_cell(typename Conv::T* __restrict__ s, int li, int pitch,
-                                              const GLOBAL uint8_t* __restrict__ flags,
-                                              const GLOBAL uint32_t* __restrict__ lidm, const StepParams& p,
-                                              uint32_t n, int x, int y, uint32_t lw, float eps,
-                                              float4* __restrict__ force) {
+                                              const uint8_t* __restrict__ flL, const float* __restrict__ prow,
+                                              uint32_t lM, const StepParams& p, uint32_t n, int x,
+                                              uint32_t lw, float eps, float4* __restrict__ force) {
     using T = typename Conv::T;
     constexpr uint32_t TB = (uint32_t)sizeof(T), CB = 19u * TB;
     char* sb = (char*)s;
@@ -118,7 +144,6 @@ __device__ __forceinline__ void resident_cell(typename Conv::T* __restrict__ s, 
     // -------- odd: streaming pass (neighbours + walls), 261-354 --------
     if (lw & kNotFluidBit) return;
     const uint32_t sM = lw & 0x3ffffu;
-    const int nx = p.nx, ny = p.ny;
     const bool writes = p.writeForce != 0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     // byte offset of the row below / of / above the cell, one cell to the left
@@ -132,9 +157,8 @@ __device__ __forceinline__ void resident_cell(typename Conv::T* __restrict__ s, 
     STRH_RPULL(7) STRH_RPULL(8) STRH_RPULL(9) STRH_RPULL(10) STRH_RPULL(11) STRH_RPULL(12)
     STRH_RPULL(13) STRH_RPULL(14) STRH_RPULL(15) STRH_RPULL(16) STRH_RPULL(17) STRH_RPULL(18)
 #undef STRH_RPULL
-    uint32_t lM = 0u, deferred = 0u;
+    uint32_t deferred = 0u;
     if (sM != 0u) {   // wall links: reads from the own slots, moving-wall corrections, force terms
-        lM = lidm[n];
 #define STRH_RWALLIN(I)                                                               \
         {                                                                             \
             constexpr int srcBit = kOpp[I] - 1;                                       \
@@ -156,14 +180,12 @@ __device__ __forceinline__ void resident_cell(typename Conv::T* __restrict__ s, 
     if (deferred != 0u) {                                          // 292-314
 #define STRH_RDEFER(I)                                                                \
         if ((deferred >> I) & 1u) {                                                   \
-            const int sy = wrap_coord(y - kCy[I], ny);                                \
-            const uint32_t src = (uint32_t)sy * (uint32_t)nx + (uint32_t)wrap_coord(x - kCx[I], nx); \
-            const uint8_t wf = flags[src];                                            \
+            const uint8_t wf = flL[li - kCx[I] - kCy[I] * pitch];                     \
             if (wf == kOutflow) {                                                     \
                 fh[I] = -fh[I];                                                       \
             } else {                                                                  \
                 float uwx, uwy, uwz;                                                  \
-                wall_vel(wf, sy, p, uwx, uwy, uwz);                                   \
+                wall_vel_row(wf, prow, -kCy[I], p, uwx, uwy, uwz);                    \
                 const float cu = (float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz; \
                 const float corr = 6.0f * wgt<I>() * cu;                              \
                 fh[I] += corr;                                                        \
@@ -207,14 +229,12 @@ __device__ __forceinline__ void resident_cell(typename Conv::T* __restrict__ s, 
                 float corr = 0.0f;                                                    \
                 float sign = 1.0f;                                                    \
                 if ((lM >> dstBit) & 1u) {                                            \
-                    const int dy = wrap_coord(y + kCy[I], ny);                        \
-                    const uint32_t d = (uint32_t)dy * (uint32_t)nx + (uint32_t)wrap_coord(x + kCx[I], nx); \
-                    const uint8_t wf = flags[d];                                      \
+                    const uint8_t wf = flL[li + kCx[I] + kCy[I] * pitch];             \
                     if (wf == kOutflow) {                                             \
                         sign = -1.0f;                                                 \
                     } else {                                                          \
                         float uwx, uwy, uwz;                                          \
-                        wall_vel(wf, dy, p, uwx, uwy, uwz);                           \
+                        wall_vel_row(wf, prow, kCy[I], p, uwx, uwy, uwz);             \
                         corr = 6.0f * wgt<I>()                                        \
                              * ((float)kCx[I] * uwx + (float)kCy[I] * uwy + (float)kCz[I] * uwz); \
                     }                                                                 \
@@ -470,13 +490,14 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
     __syncthreads();
     // this thread's cells: tile coordinates, link word, solid fraction
     int cpos[4];
-    uint32_t clw[4];
+    uint32_t clw[4], clm[4];
     float ceps[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int q = j * bsz + tid;
         cpos[j] = 0;
         clw[j] = kNotFluidBit;   // no cell: skipped like a non-fluid one
+        clm[j] = 0u;
         ceps[j] = 0.0f;
         if (j < nRounds && q < C) {
             int lx, ly;
@@ -484,15 +505,22 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
             const uint32_t n = (uint32_t)(tl.y0 + ly) * (uint32_t)nx + (uint32_t)(tl.x0 + lx);
             cpos[j] = lx | (ly << 16);
             clw[j] = link[n];
+            if (!(clw[j] & kNotFluidBit) && (clw[j] & 0x3ffffu) != 0u) clm[j] = lidm[n];
             if (useEps) ceps[j] = epsG[n];
         }
     }
     __syncthreads();   // the order is read: the tile may overwrite it
-    // the tile and its halo ring from the image
+    // the tile and its halo ring from the image, their flags after them, then
+    // two buffers of per-row inflow values (steps of even / odd k)
+    uint8_t* flL = smem + (size_t)cs * 19u * sizeof(T);
+    float* profL = (float*)(flL + ((cs + 3u) & ~3u));
+    const int rows = h + 2;
+    const bool parabolic = B.inflowRows != 0;
     for (uint32_t c = (uint32_t)tid; c < cs; c += (uint32_t)bsz) {
         const int ly = (int)(c / (uint32_t)pitch) - 1, lx = (int)(c % (uint32_t)pitch) - 1;
         const uint32_t n = (uint32_t)wrap_coord(tl.y0 + ly, ny) * (uint32_t)nx + (uint32_t)wrap_coord(tl.x0 + lx, nx);
         const uint32_t e = cell_elem(lay, n);
+        flL[c] = flg[n];
         T v[19];   // all 19 loads in flight, then the LDS writes
 #pragma unroll
         for (int i = 0; i < 19; i++) v[i] = f[(size_t)e + (size_t)i * lay.W];
@@ -503,6 +531,10 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
     if (tid < K) {
         const int64_t t = tBase + tid;                                 // Sim.swift:258-266
         rampL[tid] = t < B.rampSteps ? ((const GLOBAL float*)B.ramp)[t] : 1.0f;
+    }
+    if (parabolic) {   // the first step's rows (its ramp factor read as rampL[0] is)
+        const float r0 = tBase < B.rampSteps ? ((const GLOBAL float*)B.ramp)[tBase] : 1.0f;
+        inflow_rows(profL, rows, tl.y0, ny, B.uinTarget * r0, tid, bsz);
     }
     __syncthreads();
     // the probe schedule followed step by step (one division per launch)
@@ -523,6 +555,7 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
         p.ulidX = B.lid[0] * r;
         p.ulidY = B.lid[1] * r;
         p.ulidZ = B.lid[2] * r;
+        const float* prof = profL + (k & 1) * rows;   // this step's inflow rows
         // the batch kernel's is_force_step / force_slot, incrementally
         float4* fo = B.force;
         p.writeForce = 0;
@@ -552,8 +585,9 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
                     const int pos = pick4(cpos, j);
                     const int lx = pos & 0xffff, ly = pos >> 16;
                     const int x = tl.x0 + lx, y = tl.y0 + ly;
-                    resident_cell<Conv, PAR, LES, FORCE>(s, rl_index(lx, ly, pitch), pitch, flg, lidm, p,
-                                                         (uint32_t)y * (uint32_t)nx + (uint32_t)x, x, y, lw,
+                    resident_cell<Conv, PAR, LES, FORCE>(s, rl_index(lx, ly, pitch), pitch, flL,
+                                                         prof + ly + 1, pick4(clm, j), p,
+                                                         (uint32_t)y * (uint32_t)nx + (uint32_t)x, x, lw,
                                                          pick4(ceps, j), fo);
                 }
             }
@@ -564,6 +598,10 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
         };
         if (par == 0) pass(std::integral_constant<int, 0>{});
         else pass(std::integral_constant<int, 1>{});
+        // the next step's rows into the other buffer: the last reads of it
+        // were in step k - 1, before the barrier that ended that step
+        if (parabolic && k + 1 < K)
+            inflow_rows(profL + ((k + 1) & 1) * rows, rows, tl.y0, ny, B.uinTarget * rampL[k + 1], tid, bsz);
         if (!x_collect<T, BS>(s, tl, B, B.xchg, par, tag, abortFlag, spinTicks)) stop = 1;
         __syncthreads();
         if (stop) return;

@@ -88,6 +88,7 @@ struct strh_lattice {
     void* snap = nullptr;       // snapshot of f
     int64_t snapSteps = -1;
     bool walls = false, useEps = false, wantsForces = false;
+    bool hasInflow = false;     // some cell is INFLOW
     // Sim.swift:164-172
     float omega = 1.0f, omegaMinus = 1.0f;
     float lid[3] = {0, 0, 0};
@@ -412,6 +413,7 @@ int strh_lattice_create(strh_ctx* c, const strh_lattice_desc* d, const uint8_t* 
     if ((e = hipMalloc(&L->flags, N)) != hipSuccess) return bail(e, "hipMalloc(flags)");
     if ((e = hipMemcpy(L->flags, flags, N, hipMemcpyHostToDevice)) != hipSuccess) return bail(e, "copy flags");
     L->walls = anyWall;
+    for (uint64_t n = 0; n < N && !L->hasInflow; n++) L->hasInflow = flags[n] == STRH_INFLOW;
     if (anyWall) {
         std::vector<uint32_t> link, lidm;
         build_masks(flags, d->nx, d->ny, d->nz, link, lidm);
@@ -936,6 +938,7 @@ int batch_prepare(strh_batch* B, const int64_t* steps) {
         d.uinTarget = L->uinTarget;
         for (int k = 0; k < 3; k++) d.lid[k] = L->lid[k];
         d.hasForce = !(L->bodyForce[0] == 0.0f && L->bodyForce[1] == 0.0f && L->bodyForce[2] == 0.0f);
+        d.inflowRows = L->hasInflow && !L->inflowUniform ? 1 : 0;
         // ramp factors of Sim.swift:258-266, evaluated by the same host code
         const int64_t rl = L->rampSteps > 0 ? L->rampSteps : 0;
         if (rl != B->rampLen[i]) {
@@ -992,7 +995,12 @@ struct TilePlan {
     int wmax = 0, hmax = 0;
 };
 
-size_t tile_lds(int w, int h, int bytes) { return (size_t)(w + 2) * (size_t)(h + 2) * 19u * (size_t)bytes; }
+// The tile and its halo ring (19 slots per cell), their flags (one byte per
+// cell, rounded up to four), then two buffers of one inflow value per row.
+size_t tile_lds(int w, int h, int bytes) {
+    const size_t cs = (size_t)(w + 2) * (size_t)(h + 2);
+    return cs * 19u * (size_t)bytes + ((cs + 3u) & ~(size_t)3u) + 2u * (size_t)(h + 2) * sizeof(float);
+}
 
 // The split of nx x ny into at most g tiles with the fewest cells in its
 // largest tile (then the shortest edge), or gx = 0 when none fits in LDS.
