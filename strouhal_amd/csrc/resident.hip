@@ -312,22 +312,24 @@ __device__ __forceinline__ uint32_t opposite_set(uint32_t m) {
 __device__ __forceinline__ int opp_dir(int i) { return ((i - 1) ^ 1) + 1; }
 
 // After a pass of parity `par`: publish this tile's share of the exchange.
+// lwRing / lwHalo: the link words of the ring and halo cells in their
+// ring_cell / halo_cell order, held in LDS for the launch.
 template <class T, int BS>
 __device__ __forceinline__ void x_publish(const T* __restrict__ s, const ResTile& tl,
-                                          const BatchLat& B, uint64_t* X, int par, uint32_t tag) {
+                                          const BatchLat& B, uint64_t* X, int par, uint32_t tag,
+                                          const uint32_t* __restrict__ lwRing,
+                                          const uint32_t* __restrict__ lwHalo) {
     const int w = tl.w, h = tl.h, pitch = w + 2, nx = B.p.nx, ny = B.p.ny;
     const uint32_t N = B.p.N;
-    const GLOBAL uint32_t* __restrict__ link = (const GLOBAL uint32_t*)B.link;
     // even: edge cell b publishes slot opp(i) for each i that leaves the tile;
     // odd: halo cell h publishes slot i for each i pushed into it from the tile
     const int cnt = par == 0 ? 2 * w + 2 * (h - 2) : 2 * (w + 2) + 2 * h;
     for (int r = threadIdx.x; r < cnt; r += (int)blockDim.x) {
         int lx, ly;
         if (par == 0) ring_cell(r, w, h, lx, ly); else halo_cell(r, w, h, lx, ly);
-        const int gx = wrap_coord(tl.x0 + lx, nx), gy = wrap_coord(tl.y0 + ly, ny);
-        const uint32_t n = (uint32_t)gy * (uint32_t)nx + (uint32_t)gx;
-        const uint32_t lw = link[n];
+        const uint32_t lw = par == 0 ? lwRing[r] : lwHalo[r];
         if (lw & kNotFluidBit) continue;
+        const int gx = wrap_coord(tl.x0 + lx, nx), gy = wrap_coord(tl.y0 + ly, ny);
         const uint32_t li = (uint32_t)rl_index(lx, ly, pitch);
         // only links between fluid cells carry a population across: along a
         // wall link the cell bounces its own populations back instead
@@ -350,20 +352,19 @@ __device__ __forceinline__ void x_publish(const T* __restrict__ s, const ResTile
 template <class T, int BS>
 __device__ __forceinline__ bool x_collect(T* __restrict__ s, const ResTile& tl, const BatchLat& B,
                                           const uint64_t* X, int par, uint32_t tag, int* abortFlag,
-                                          uint64_t spinTicks) {
+                                          uint64_t spinTicks, const uint32_t* __restrict__ lwRing,
+                                          const uint32_t* __restrict__ lwHalo) {
     const int w = tl.w, h = tl.h, pitch = w + 2, nx = B.p.nx, ny = B.p.ny;
     const uint32_t N = B.p.N;
-    const GLOBAL uint32_t* __restrict__ link = (const GLOBAL uint32_t*)B.link;
     // even: halo cell h loads slot opp(i) for each i with h + c_i in the tile;
     // odd: edge cell b loads slot i for each i with b - c_i outside the tile
     const int cnt = par == 0 ? 2 * (w + 2) + 2 * h : 2 * w + 2 * (h - 2);
     for (int r = threadIdx.x; r < cnt; r += (int)blockDim.x) {
         int lx, ly;
         if (par == 0) halo_cell(r, w, h, lx, ly); else ring_cell(r, w, h, lx, ly);
-        const int gx = wrap_coord(tl.x0 + lx, nx), gy = wrap_coord(tl.y0 + ly, ny);
-        const uint32_t n = (uint32_t)gy * (uint32_t)nx + (uint32_t)gx;
-        const uint32_t lw = link[n];
+        const uint32_t lw = par == 0 ? lwHalo[r] : lwRing[r];
         if (lw & kNotFluidBit) continue;
+        const int gx = wrap_coord(tl.x0 + lx, nx), gy = wrap_coord(tl.y0 + ly, ny);
         const uint32_t li = (uint32_t)rl_index(lx, ly, pitch);
         const uint32_t open = ~((lw & 0x3ffffu) << 1);
         const uint32_t in = stays_in(lx, ly, w, h) & open;
@@ -516,6 +517,15 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
     float* profL = (float*)(flL + ((cs + 3u) & ~3u));
     const int rows = h + 2;
     const bool parabolic = B.inflowRows != 0;
+    // the link words of the ring and halo cells, which the exchange reads every step
+    uint32_t* lwRing = (uint32_t*)(profL + 2 * rows);
+    uint32_t* lwHalo = lwRing + R;
+    const int Hc = 2 * (w + 2) + 2 * h;
+    for (int r = tid; r < R + Hc; r += bsz) {
+        int lx, ly;
+        if (r < R) ring_cell(r, w, h, lx, ly); else halo_cell(r - R, w, h, lx, ly);
+        lwRing[r] = link[(uint32_t)wrap_coord(tl.y0 + ly, ny) * (uint32_t)nx + (uint32_t)wrap_coord(tl.x0 + lx, nx)];
+    }
     for (uint32_t c = (uint32_t)tid; c < cs; c += (uint32_t)bsz) {
         const int ly = (int)(c / (uint32_t)pitch) - 1, lx = (int)(c % (uint32_t)pitch) - 1;
         const uint32_t n = (uint32_t)wrap_coord(tl.y0 + ly, ny) * (uint32_t)nx + (uint32_t)wrap_coord(tl.x0 + lx, nx);
@@ -578,7 +588,7 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
             for (int j = 0; j < nRounds; j++) {
                 if (j == ringRounds) {
                     __syncthreads();
-                    x_publish<T, BS>(s, tl, B, B.xchg, PAR, tag);
+                    x_publish<T, BS>(s, tl, B, B.xchg, PAR, tag, lwRing, lwHalo);
                 }
                 const uint32_t lw = pick4(clw, j);
                 if (!(lw & kNotFluidBit)) {
@@ -593,7 +603,7 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
             }
             if (ringRounds >= nRounds) {
                 __syncthreads();
-                x_publish<T, BS>(s, tl, B, B.xchg, PAR, tag);
+                x_publish<T, BS>(s, tl, B, B.xchg, PAR, tag, lwRing, lwHalo);
             }
         };
         if (par == 0) pass(std::integral_constant<int, 0>{});
@@ -602,7 +612,7 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
         // were in step k - 1, before the barrier that ended that step
         if (parabolic && k + 1 < K)
             inflow_rows(profL + ((k + 1) & 1) * rows, rows, tl.y0, ny, B.uinTarget * rampL[k + 1], tid, bsz);
-        if (!x_collect<T, BS>(s, tl, B, B.xchg, par, tag, abortFlag, spinTicks)) stop = 1;
+        if (!x_collect<T, BS>(s, tl, B, B.xchg, par, tag, abortFlag, spinTicks, lwRing, lwHalo)) stop = 1;
         __syncthreads();
         if (stop) return;
     }

@@ -996,10 +996,13 @@ struct TilePlan {
 };
 
 // The tile and its halo ring (19 slots per cell), their flags (one byte per
-// cell, rounded up to four), then two buffers of one inflow value per row.
+// cell, rounded up to four), two buffers of one inflow value per row, then the
+// link words of the edge ring and the halo ring.
 size_t tile_lds(int w, int h, int bytes) {
     const size_t cs = (size_t)(w + 2) * (size_t)(h + 2);
-    return cs * 19u * (size_t)bytes + ((cs + 3u) & ~(size_t)3u) + 2u * (size_t)(h + 2) * sizeof(float);
+    const size_t rings = (size_t)(2 * w + 2 * (h - 2)) + (size_t)(2 * (w + 2) + 2 * h);
+    return cs * 19u * (size_t)bytes + ((cs + 3u) & ~(size_t)3u) + 2u * (size_t)(h + 2) * sizeof(float) +
+           rings * sizeof(uint32_t);
 }
 
 // The split of nx x ny into at most g tiles with the fewest cells in its
