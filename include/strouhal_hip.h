@@ -222,10 +222,14 @@ STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);
  * workgroup per CU, and advanced kProbeSpan (64) steps per launch, trading
  * only their edge populations between steps; groups of lattices that do not
  * fit on chip together are time-sliced. Results are the same bits as the
- * streaming kernel's. mode 0: never; 1 (default): in every run segment whose
- * active lattices all qualify and fit on chip in one launch; 2: in every
- * segment, time-sliced where needed (STRH_ERR_UNSUPPORTED when some lattice
- * does not qualify).
+ * streaming kernel's. mode 0: never; 1: in every run segment whose active
+ * lattices all qualify and fit on chip in one launch; 2: in every segment,
+ * time-sliced where needed (STRH_ERR_UNSUPPORTED when some lattice does not
+ * qualify); 3 (default): as 1, and where the active lattices qualify but need
+ * time-sliced groups, the first steps of the segment time both the streaming
+ * and the resident graphs (one untimed and two timed replays of each) and the
+ * rest of the segment runs the faster (segments of at least 8 graph lengths;
+ * shorter ones stream).
  * Each qualifying lattice holds 152 bytes per cell of exchange words while it
  * is batched. A stalled exchange (which a correct build never produces)
  * returns STRH_ERR_DEVICE after about a second. */

@@ -2,8 +2,9 @@
 
 A batch advances several lattices together, each with its own step count,
 ramp and momentum-exchange probe schedule held on the device: lattices one
-cell deep are held in LDS for 64 steps per launch (resident launches), others
-stream through memory with one launch per step (replayed as HIP graphs). It replaces the
+cell deep are held in LDS for 64 steps per launch (resident launches, where
+they measure faster), others stream through memory with one launch per step
+(replayed as HIP graphs). It replaces the
 sequential rung loop of the reference's Ladder.credibility / Ladder.measure
 (Sources/StrouhalCore/Ladder.swift:54-121); every batched lattice evolves bit
 for bit as it does when run alone.
@@ -75,11 +76,13 @@ class Batch:
         check(lib().strh_batch_set_graph_steps(self._h, int(steps)))
 
     def set_resident(self, mode: int):
-        """0: stream every DDF through memory each step; 1 (default): hold
-        lattices one cell deep in LDS across steps (resident launches) where
-        the active lattices fit on chip in one launch; 2: resident launches
-        everywhere, time-sliced groups where needed (an error for lattices
-        that do not qualify)."""
+        """0: stream every DDF through memory each step; 1: hold lattices one
+        cell deep in LDS across steps (resident launches) where the active
+        lattices fit on chip in one launch; 2: resident launches everywhere,
+        time-sliced groups where needed (an error for lattices that do not
+        qualify); 3 (default): as 1, and where the active lattices would need
+        time-sliced groups, time both kinds of launch over the segment's first
+        steps and keep the faster."""
         check(lib().strh_batch_set_resident(self._h, int(mode)))
 
     def set_resident_timeout(self, seconds: float):

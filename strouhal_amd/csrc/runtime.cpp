@@ -911,7 +911,8 @@ struct strh_batch {
     int graphSteps = 64;
     double gpuSeconds = 0.0;
     // resident launches (resident.hip)
-    int resident = 1;                    // 0 never, 1 where every active lattice qualifies, 2 required
+    int resident = 3;                    // 0 never, 1 one launch per segment, 2 required, 3 measured
+    hipEvent_t trialEv[3] = {nullptr, nullptr, nullptr};   // mode 3's timing of the two launch kinds
     int residentSlots = 0;               // workgroups the device holds at once at the largest tile
     std::vector<uint64_t*> xchg;         // per lattice, [19][N] exchange words, or null
     int* abortFlag = nullptr;            // raised by a resident launch whose exchange stalled
@@ -983,6 +984,7 @@ struct Segment {
     bool planar;             // every active lattice is one cell deep
     bool probes;             // some active lattice has force samples
     std::vector<ResGroup> groups;   // resident launches, or empty for the streaming kernel
+    bool trial = false;      // time-sliced groups: time both launch kinds, keep the faster (mode 3)
 };
 
 // ---- resident tiling ---------------------------------------------------------
@@ -1218,11 +1220,14 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             sg.L++;
         }
         if (B->resident != 0) {
-            // mode 1 takes the resident launch where the segment's lattices fit
-            // on chip in one launch (time-sliced groups measured slower than
-            // the streaming kernel); mode 2 takes it wherever they qualify
+            // modes 1 and 3 take the resident launch where the segment's
+            // lattices fit on chip in one launch; where they need time-sliced
+            // groups mode 1 streams and mode 3 measures both (the faster
+            // depends on the storage and the lattices); mode 2 takes it
+            // wherever they qualify
             const size_t mark = tiles.size();
             const bool ok = plan_segment(B, members, sg, tiles);
+            sg.trial = ok && B->resident == 3 && sg.groups.size() > 1;
             if (!ok || (B->resident == 1 && sg.groups.size() > 1)) {
                 sg.groups.clear();
                 tiles.resize(mark);
@@ -1266,14 +1271,51 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
     }
 
     HIP_TRY(hipEventRecord(B->ctx->ev0, s), "hipEventRecord");
-    for (const Segment& sg : segs) {
-        HIP_TRY(launch_set_base(B->base, sg.k0, s), "set base");
-        const int64_t K = sg.k1 - sg.k0;
+    for (const Segment& seg : segs) {
+        HIP_TRY(launch_set_base(B->base, seg.k0, s), "set base");
+        const int64_t K = seg.k1 - seg.k0;
         int64_t done = 0;
-        if (useGraph && B->graphSteps > 0 && K >= B->graphSteps) {
-            hipGraphExec_t g = nullptr;
-            int rc = batch_capture(B, sg, &g);
+        Segment sg = seg;
+        if (seg.trial) sg.groups.clear();   // streams unless the trial below picks the groups
+        // Mode 3 with time-sliced groups: one untimed replay of each kind of
+        // graph, two timed replays of each, then the faster for the rest of
+        // the segment. Both kinds give the same bits, so the choice changes
+        // only the time.
+        const bool trial = seg.trial && useGraph && B->graphSteps > 0 && K >= 8 * (int64_t)B->graphSteps;
+        hipGraphExec_t kept = nullptr;   // the trial's faster graph, replayed for the rest
+        if (trial) {
+            for (hipEvent_t& ev : B->trialEv)
+                if (!ev) HIP_TRY(hipEventCreate(&ev), "hipEventCreate");
+            hipGraphExec_t gk[2] = {nullptr, nullptr};   // streaming, resident
+            int rc = batch_capture(B, sg, &gk[0]);
+            if (!rc) rc = batch_capture(B, seg, &gk[1]);
+            float ms[2] = {0.0f, 0.0f};
+            hipError_t e = hipSuccess;
+            for (int k = 0; k < 2 && !rc && e == hipSuccess; k++) { e = hipGraphLaunch(gk[k], s); done += B->graphSteps; }
+            for (int k = 0; k < 2 && !rc && e == hipSuccess; k++) {
+                e = hipEventRecord(B->trialEv[k], s);
+                for (int r = 0; r < 2 && e == hipSuccess; r++) { e = hipGraphLaunch(gk[k], s); done += B->graphSteps; }
+            }
+            if (!rc && e == hipSuccess) e = hipEventRecord(B->trialEv[2], s);
+            const hipError_t es = hipStreamSynchronize(s);   // before any exec is destroyed
+            if (!rc && e == hipSuccess) e = es;
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms[0], B->trialEv[0], B->trialEv[1]);
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms[1], B->trialEv[1], B->trialEv[2]);
+            const int pick = ms[1] < ms[0] ? 1 : 0;
+            for (int k = 0; k < 2; k++)
+                if (gk[k] && (rc || e != hipSuccess || k != pick)) hipGraphExecDestroy(gk[k]);
             if (rc) return rc;
+            if (e != hipSuccess) return hip_fail(e, "resident trial");
+            if (pick == 1) sg.groups = seg.groups;
+            kept = gk[pick];
+        }
+        if (useGraph && B->graphSteps > 0 && K - done >= B->graphSteps) {
+            hipGraphExec_t g = kept;
+            kept = nullptr;
+            if (!g) {
+                int rc = batch_capture(B, sg, &g);
+                if (rc) return rc;
+            }
             while (K - done >= B->graphSteps) {
                 hipError_t e = hipGraphLaunch(g, s);
                 if (e != hipSuccess) {
@@ -1288,6 +1330,7 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
             hipGraphExecDestroy(g);
             if (e != hipSuccess) return hip_fail(e, "graph replays");
         }
+        if (kept) hipGraphExecDestroy(kept);   // (the stream was synchronised in the trial)
         if (done < K) {
             const int par0 = sg.parity0 < 0 ? -1 : (int)((sg.parity0 + done) & 1);
             int rc = batch_launch_steps(B, sg, B->bs, (int)(K - done), par0);
@@ -1369,6 +1412,7 @@ int strh_batch_destroy(strh_batch* B) {
     for (void* p : B->fring) if (p) hipFree(p);
     for (uint64_t* p : B->xchg) if (p) hipFree(p);
     if (B->tiles) hipFree(B->tiles);
+    for (hipEvent_t ev : B->trialEv) if (ev) hipEventDestroy(ev);
     if (B->abortFlag) hipFree(B->abortFlag);
     if (B->dev) hipFree(B->dev);
     if (B->base) hipFree(B->base);
@@ -1481,7 +1525,7 @@ int strh_batch_set_graph_steps(strh_batch* B, int steps) {
 }
 
 int strh_batch_set_resident(strh_batch* B, int mode) {
-    if (!B || mode < 0 || mode > 2) return fail(STRH_ERR_INVALID, "resident mode must be 0, 1 or 2");
+    if (!B || mode < 0 || mode > 3) return fail(STRH_ERR_INVALID, "resident mode must be 0, 1, 2 or 3");
     B->resident = mode;
     return STRH_OK;
 }

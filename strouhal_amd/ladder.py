@@ -150,7 +150,7 @@ def run_sequential(variants):
     return series, time.perf_counter() - t0
 
 
-def run_batched(variants, autotune=True, graph_steps=64, resident=1):
+def run_batched(variants, autotune=True, graph_steps=64, resident=3):
     b = Batch([v.case.sim for v in variants])
     b.set_resident(resident)
     for i, v in enumerate(variants):
@@ -165,7 +165,7 @@ def run_batched(variants, autotune=True, graph_steps=64, resident=1):
     return series, wall, bs, b
 
 
-def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64, resident=1):
+def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64, resident=3):
     """Microseconds per lock-step step of every rung together and of the finest
     rung alone, from the rungs' initial state."""
     out = {}
@@ -200,9 +200,10 @@ def main():
                     help="instead of the schedule: time this many steps of all rungs and of the "
                          "finest rung alone")
     ap.add_argument("--graph-steps", type=int, default=64)
-    ap.add_argument("--resident", type=int, default=1,
+    ap.add_argument("--resident", type=int, default=3,
                     help="0: streaming kernel only; 1: resident launches where the active rungs fit on chip "
-                         "in one launch; 2: resident launches everywhere (time-sliced groups)")
+                         "in one launch; 2: resident launches everywhere (time-sliced groups); 3: as 1, and "
+                         "where time-sliced groups would be needed, the faster of the two measured")
     ap.add_argument("--report", default="", help="write the credibility report (markdown) of mean C_D here")
     args = ap.parse_args()
     res = [int(r) for r in args.resolutions.split(",")]

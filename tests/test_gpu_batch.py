@@ -47,7 +47,8 @@ def test_ladder_batched_equals_sequential(gpu, precision):
     assert vb[-2].case.sim.state_digest in md          # the finest Mach-1 rung's digest
 
 
-@pytest.mark.parametrize("precision,resident", [("fp32", 1), ("fp16s", 1), ("fp32", 2), ("fp16s", 2)])
+@pytest.mark.parametrize("precision,resident", [("fp32", 1), ("fp16s", 1), ("fp32", 2), ("fp16s", 2),
+                                                ("fp32", 3), ("fp16s", 3)])
 def test_full_size_ladder_batched_equals_sequential(gpu, precision, resident):
     # The ladder at its full size (D in {16, 24, 32, 48, 64}, each at Ma 0.1
     # and half Mach: 10 lattices, 1,504,680 cells) on 1% of the reference
@@ -59,7 +60,9 @@ def test_full_size_ladder_batched_equals_sequential(gpu, precision, resident):
     vs = build(gpu, precision, res, 0.01, BASE_UMAX)
     assert len(vb) == 10 and sum(v.case.sim.cells for v in vb) == 1_504_680
     # resident 1: LDS-resident launches for the segments whose rungs fit on
-    # chip together (the tail); 2: for every segment, in time-sliced groups
+    # chip together (the tail); 2: for every segment, in time-sliced groups;
+    # 3: as 1, and the segments that need groups switch to the faster kind
+    # after timing both (segments of 512 steps and more at this scale)
     sb, _, _, _ = run_batched(vb, autotune=True, graph_steps=64, resident=resident)
     ss, _ = run_sequential(vs)
     for a, b, v in zip(sb, ss, vb):
