@@ -186,6 +186,34 @@ def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64, resident=3
     return out
 
 
+def segment_rates(gpu, precision, res, base_umax, steps, graph_steps=64):
+    """Microseconds per lock-step step of each set of rungs that is active
+    together in the schedule (every rung, then without those that finish
+    first, ...), with the streaming kernel (resident mode 0) and with resident
+    launches, time-sliced where needed (mode 2), from the rungs' initial state.
+    The schedule's segment lengths weight them into its batched time."""
+    vs = build(gpu, precision, res, 0.01, base_umax)
+    order = sorted(range(len(vs)), key=lambda i: vs[i].total_steps)
+    out = []
+    for drop in range(len(vs)):
+        if drop > 0 and vs[order[drop - 1]].total_steps == vs[order[drop]].total_steps:
+            continue
+        act = sorted(order[drop:])
+        sims = [vs[i].case.sim for i in act]
+        row = {"rungs": [(vs[i].D, vs[i].mach_scale) for i in act]}
+        for mode in (0, 2):
+            b = Batch(sims)
+            b.set_graph_steps(graph_steps)
+            b.set_resident(mode)
+            b.run([64] * len(sims))
+            g0 = b.gpu_seconds
+            b.run([steps] * len(sims))
+            row["streaming_us" if mode == 0 else "resident_us"] = (b.gpu_seconds - g0) / steps * 1e6
+            b.close()
+        out.append(row)
+    return out
+
+
 def main():
     import argparse
     from strouhal_amd import default_gpu
@@ -199,6 +227,9 @@ def main():
     ap.add_argument("--step-rate", type=int, default=0,
                     help="instead of the schedule: time this many steps of all rungs and of the "
                          "finest rung alone")
+    ap.add_argument("--segment-rates", type=int, default=0,
+                    help="instead of the schedule: time this many steps of each set of rungs active "
+                         "together, streaming and resident")
     ap.add_argument("--graph-steps", type=int, default=64)
     ap.add_argument("--resident", type=int, default=3,
                     help="0: streaming kernel only; 1: resident launches where the active rungs fit on chip "
@@ -209,6 +240,13 @@ def main():
     res = [int(r) for r in args.resolutions.split(",")]
     base_umax = 1.5 * args.mach / math.sqrt(3.0)      # Ma of U_mean = umax * 2/3 * sqrt(3)
     gpu = default_gpu()
+    if args.segment_rates > 0:
+        print(json.dumps({"workload": f"DFG 2D-2 street ladder D={res}, the rung sets of its schedule, "
+                                      f"{args.segment_rates} steps per mode",
+                          "precision": args.precision, "graph_steps": args.graph_steps,
+                          "segments": segment_rates(gpu, args.precision, res, base_umax, args.segment_rates,
+                                                    args.graph_steps)}), flush=True)
+        return
     if args.step_rate > 0:
         print(json.dumps({"workload": f"DFG 2D-2 street ladder D={res}, {args.step_rate} steps per mode",
                           "precision": args.precision, "graph_steps": args.graph_steps,
