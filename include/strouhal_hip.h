@@ -267,6 +267,10 @@ typedef struct strh_vjp_info {
     double backward_seconds;    /* reverse steps */
     uint64_t workspace_bytes;   /* device memory of the call beyond the lattice */
     int64_t outer_every;        /* steps per outer segment (= checkpoint_every: one level) */
+    double reverse_pass_seconds;   /* replays and reverse steps together, wall time on the GPU:
+                                      with one level and room for a second recording, segment
+                                      s - 1 is replayed on a second stream while the reverse steps
+                                      of segment s run, so this is less than their sum */
 } strh_vjp_info;
 
 /* Advance T steps and return J (FP64, fixed-order reduction). */

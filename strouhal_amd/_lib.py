@@ -68,7 +68,7 @@ class VjpInfo(ctypes.Structure):
         ("checkpoint_every", ctypes.c_int64), ("checkpoints", ctypes.c_int64),
         ("forward_seconds", ctypes.c_double), ("recompute_seconds", ctypes.c_double),
         ("backward_seconds", ctypes.c_double), ("workspace_bytes", ctypes.c_uint64),
-        ("outer_every", ctypes.c_int64),
+        ("outer_every", ctypes.c_int64), ("reverse_pass_seconds", ctypes.c_double),
     ]
 
 

@@ -147,7 +147,7 @@ def main():
         np.save(args.save, ga)
     n = adj.cells
     step_ms = gpu_f / (T - 2) * 1e3
-    adj_s = info["forward_seconds"] + info["recompute_seconds"] + info["backward_seconds"]
+    adj_s = info["forward_seconds"] + info["reverse_pass_seconds"]   # replays and reverse steps may overlap
     report = {
         "workload": f"porous channel {args.nx}x{args.ny}x{args.nz} Re 50, Brinkman design field, "
                     f"spin-up {args.spinup} + window {args.avg}",

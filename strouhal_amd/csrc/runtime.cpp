@@ -60,7 +60,7 @@ constexpr uint64_t kMaxCells = 200000000ull;
 struct strh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string name;
     // device address of this device's static reduction scratch (observe.hip),
     // or null with the reason in scratchError
@@ -341,7 +341,8 @@ int strh_ctx_create(int device_ordinal, strh_ctx** out) {
         c->scratchError = hipGetErrorString(se);
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
         delete c;
         return fail(STRH_ERR_DEVICE, "stream/event creation failed");
     }
@@ -354,6 +355,8 @@ int strh_ctx_destroy(strh_ctx* c) {
     hipSetDevice(c->device);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->ev2) hipEventDestroy(c->ev2);
+    if (c->ev3) hipEventDestroy(c->ev3);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return STRH_OK;
@@ -1668,7 +1671,9 @@ int adj_plan(strh_lattice* L, const strh_objective* o, AdjPlan& P) {
 // (count * 19 * N values in the storage format).
 int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_t k0, int64_t count,
                 const strh_objective* o, const AdjPlan& P, float4* rec, double* jacc,
-                const int32_t* ntIdx = nullptr, float* recNt = nullptr, uint8_t* rec19 = nullptr) {
+                const int32_t* ntIdx = nullptr, float* recNt = nullptr, uint8_t* rec19 = nullptr,
+                hipStream_t stream = nullptr) {
+    if (!stream) stream = L->ctx->stream;
     const uint32_t M = (uint32_t)P.ntCells.size();
     for (int64_t s = 0; s < count; s++) {
         const int64_t k = k0 + s;
@@ -1689,7 +1694,7 @@ int adj_forward(strh_lattice* L, void* f, const float* alpha, int64_t t0, int64_
         hk.M = M;
         hk.wf = win ? P.wf : 0.0;
         hk.rec19 = rec19 ? rec19 + (size_t)s * 19 * L->N * L->B : nullptr;
-        HIP_TRY(launch_step_hooked(sl, hk, L->ctx->stream), "hooked step launch");
+        HIP_TRY(launch_step_hooked(sl, hk, stream), "hooked step launch");
     }
     return STRH_OK;
 }
@@ -1807,6 +1812,26 @@ int vjp_run(strh_lattice* L, const strh_objective* o, const AdjPlan& P, VjpPlan 
     if ((rc = W.get(&ck, (size_t)(C1 > 1 ? C1 - 1 : 1) * ddf, "checkpoints"))) return rc;
     if (C2 > 1 && (rc = W.get(&ick, (size_t)(C2 - 1) * ddf, "inner checkpoints"))) return rc;
     if ((rc = W.get(&work, ddf, "replay DDFs"))) return rc;
+    // One level with replays: when a second recording and replay image fit,
+    // segment s - 1 is replayed on a second stream while the reverse steps of
+    // segment s run (pipelined); otherwise the two alternate.
+    const size_t recBytes = les ? (size_t)K2 * 19 * N * L->B : (size_t)K2 * N * 16;
+    const size_t recNtBytes = L->useEps && !les ? (size_t)K2 * 19 * std::max<size_t>(M, 1) * 4 : 0;
+    bool pipe = false;
+    if (C2 == 1 && C1 > 1) {
+        size_t freeB = 0, totalB = 0;
+        HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+        pipe = freeB > recBytes + recNtBytes + ddf + ((size_t)1 << 30);
+    }
+    float4* recX[2] = {nullptr, nullptr};
+    uint8_t* rec19X[2] = {nullptr, nullptr};
+    float* recNtX[2] = {nullptr, nullptr};
+    uint8_t* workX[2] = {work, nullptr};
+    if (pipe) {
+        if (les) { if ((rc = W.get(&rec19X[1], recBytes, "recorded populations"))) return rc; }
+        else if ((rc = W.get(&recX[1], recBytes, "recorded moments"))) return rc;
+        if ((rc = W.get(&workX[1], ddf, "replay DDFs"))) return rc;
+    }
     const float* alpha = L->alpha;
     if (!alpha && !L->useEps && !les) {
         if ((rc = W.get(&zeroAlpha, N * 4, "zero porosity"))) return rc;
@@ -1820,6 +1845,7 @@ int vjp_run(strh_lattice* L, const strh_objective* o, const AdjPlan& P, VjpPlan 
         const size_t m = std::max<size_t>(M, 1);
         if ((rc = W.get(&ntIdx, N * 4, "partial-cell index"))) return rc;
         if (!les && (rc = W.get(&recNt, (size_t)K2 * 19 * m * 4, "recorded partial cells"))) return rc;
+        if (pipe && !les && (rc = W.get(&recNtX[1], (size_t)K2 * 19 * m * 4, "recorded partial cells"))) return rc;
         if ((rc = W.get(&gE, m * 8, "solid-fraction gradient"))) return rc;
         HIP_TRY(hipMemcpy(ntIdx, P.ntIdx.data(), N * 4, hipMemcpyHostToDevice), "copy index");
         HIP_TRY(launch_fill_zero(gE, m * 8, st), "clear");
@@ -1848,6 +1874,11 @@ int vjp_run(strh_lattice* L, const strh_objective* o, const AdjPlan& P, VjpPlan 
         return STRH_OK;
     };
 
+    recX[0] = rec; rec19X[0] = rec19; recNtX[0] = recNt;
+    if (pipe) {   // the forward pass records the last segment into buffer (C1 - 1) & 1
+        rec = recX[(C1 - 1) & 1]; rec19 = rec19X[(C1 - 1) & 1]; recNt = recNtX[(C1 - 1) & 1];
+    }
+
     // forward pass: outer checkpoints at the start of outer segments
     // 0 .. C1-2; the last outer segment lays down its inner checkpoints and
     // records its last inner segment
@@ -1868,69 +1899,139 @@ int vjp_run(strh_lattice* L, const strh_objective* o, const AdjPlan& P, VjpPlan 
     L->gpuSeconds += tf;
     L->stepsDone += T;
 
-    // reverse pass, one inner segment at a time
+    // reverse pass
     float* lamN = lam0;
     float* lamT = lam1;
-    for (int64_t seg = C1 - 1; seg >= 0; seg--) {
-        const int64_t k0 = seg * K1, len = std::min(K1, T - k0);
-        const int64_t n2 = (len + K2 - 1) / K2;
-        if (seg != C1 - 1) {
-            HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
-            HIP_TRY(hipMemcpyAsync(work, ck + (size_t)seg * ddf, ddf, hipMemcpyDeviceToDevice, st), "restore");
-            if ((rc = advance_outer(work, k0, len, nullptr))) return rc;
-            HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
-            HIP_TRY(hipEventSynchronize(c->ev1), "replay");
-            tr += elapsed(c);
+    // the reverse steps kk + l2 - 1 .. kk over the recording (recB, recNtB,
+    // rec19B) of steps kk .. kk + l2 - 1, on stream sr
+    auto reverse_steps = [&](int64_t kk, int64_t l2, float4* recB, float* recNtB, uint8_t* rec19B,
+                             hipStream_t sr) -> int {
+        for (int64_t k = kk + l2 - 1; k >= kk; k--) {
+            const int64_t t = t0 + k;
+            const StepParams sp = make_params(L, t, false);
+            AdjStep q;
+            memset(&q, 0, sizeof(q));
+            q.nx = (uint32_t)L->nx; q.ny = (uint32_t)L->ny; q.nz = (uint32_t)L->nz;
+            q.N = L->N; q.S = L->S;
+            q.divx = sp.divx; q.divy = sp.divy;
+            q.wp = sp.omega; q.wm = sp.omegaMinus;
+            q.fx = sp.fx; q.fy = sp.fy; q.fz = sp.fz;
+            q.rOdd = (double)ramp(L, (t & 1) ? t : t + 1);
+            q.uinTarget = L->uinTarget;
+            q.inflowUniform = L->inflowUniform ? 1 : 0;
+            const bool win = k >= o->spinup;
+            q.inletX = o->inlet_x; q.outletX = o->outlet_x;
+            q.cin = win ? (float)P.cin : 0.0f;
+            q.cout = win ? (float)P.cout : 0.0f;
+            q.wd = win ? (float)P.wd : 0.0f;
+            q.wf = win ? (float)P.wf : 0.0f;
+            q.M = (uint32_t)M;
+            q.spongeX0 = sp.spongeX0; q.spongeInvW = sp.spongeInvW; q.spongeTau = sp.spongeTau;
+            q.cSmago = sp.cSmago;
+            q.srt = sp.omegaMinus == sp.omega ? 1 : 0;
+            q.precision = L->precision;
+            HIP_TRY(launch_adjoint_step(q, L->walls, recB ? recB + (size_t)(k - kk) * N : nullptr, lamN, lamT,
+                                        L->flags, L->link, alpha, gA, gU, L->useEps ? L->eps : nullptr,
+                                        ntIdx, recNtB ? recNtB + (size_t)(k - kk) * 19 * M : nullptr, gE,
+                                        rec19B ? rec19B + (size_t)(k - kk) * 19 * N * L->B : nullptr, L->lidm,
+                                        sp, sr),
+                    "adjoint step launch");
+            std::swap(lamN, lamT);
         }
-        for (int64_t j = n2 - 1; j >= 0; j--) {
-            const int64_t kk = k0 + j * K2, l2 = std::min(K2, len - j * K2);
-            if (j != n2 - 1) {
+        return STRH_OK;
+    };
+    double tp = 0.0;   // wall time of the reverse pass
+    if (pipe) {
+        // Segment s's reverse steps (stream st, buffer s & 1) wait for its
+        // replay; the replay of segment s - 1 (stream sb, buffer (s - 1) & 1)
+        // waits for the reverse steps of segment s + 1, the last reader of that
+        // buffer. Nothing else is shared: the forward pass has completed.
+        hipStream_t sb = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&sb, hipStreamNonBlocking), "replay stream");
+        std::vector<hipEvent_t> ev(4 * (size_t)C1, nullptr);   // replay begin / end, reverse begin / end
+        auto cleanup = [&]() {
+            (void)hipStreamSynchronize(sb);
+            (void)hipStreamSynchronize(st);
+            for (hipEvent_t e : ev) if (e) hipEventDestroy(e);
+            hipStreamDestroy(sb);
+        };
+        for (hipEvent_t& e : ev)
+            if (hipEventCreate(&e) != hipSuccess) { cleanup(); return fail(STRH_ERR_DEVICE, "hipEventCreate"); }
+        auto repB = [&](int64_t s) { return ev[4 * s]; };
+        auto repE = [&](int64_t s) { return ev[4 * s + 1]; };
+        auto revB = [&](int64_t s) { return ev[4 * s + 2]; };
+        auto revE = [&](int64_t s) { return ev[4 * s + 3]; };
+        hipError_t e = hipEventRecord(c->ev0, st);
+        for (int64_t seg = C1 - 1; seg >= 0 && e == hipSuccess && !rc; seg--) {
+            if (seg >= 1) {
+                const int64_t sr = seg - 1, k0r = sr * K1, lr = std::min(K1, T - k0r);
+                const int b = (int)(sr & 1);
+                if (sr + 2 <= C1 - 1) e = hipStreamWaitEvent(sb, revE(sr + 2), 0);
+                if (e == hipSuccess) e = hipEventRecord(repB(sr), sb);
+                if (e == hipSuccess) e = hipMemcpyAsync(workX[b], ck + (size_t)sr * ddf, ddf, hipMemcpyDeviceToDevice, sb);
+                if (e == hipSuccess)
+                    rc = adj_forward(L, workX[b], alpha, t0, k0r, lr, o, P, recX[b], nullptr, ntIdx, recNtX[b],
+                                     rec19X[b], sb);
+                if (e == hipSuccess && !rc) e = hipEventRecord(repE(sr), sb);
+            }
+            if (e != hipSuccess || rc) break;
+            const int a = (int)(seg & 1);
+            const int64_t k0s = seg * K1, ls = std::min(K1, T - k0s);
+            if (seg != C1 - 1) e = hipStreamWaitEvent(st, repE(seg), 0);
+            if (e == hipSuccess) e = hipEventRecord(revB(seg), st);
+            if (e == hipSuccess) rc = reverse_steps(k0s, ls, recX[a], recNtX[a], rec19X[a], st);
+            if (e == hipSuccess && !rc) e = hipEventRecord(revE(seg), st);
+        }
+        if (e == hipSuccess && !rc) e = hipEventRecord(c->ev1, st);
+        if (e == hipSuccess && !rc) e = hipEventSynchronize(c->ev1);
+        if (e == hipSuccess && !rc) {
+            tp = elapsed(c);
+            float ms = 0.0f;
+            for (int64_t sg2 = 0; sg2 < C1 && e == hipSuccess; sg2++) {
+                if (sg2 < C1 - 1 && (e = hipEventElapsedTime(&ms, repB(sg2), repE(sg2))) == hipSuccess) tr += ms * 1e-3;
+                if (e == hipSuccess && (e = hipEventElapsedTime(&ms, revB(sg2), revE(sg2))) == hipSuccess) tb += ms * 1e-3;
+            }
+        }
+        cleanup();
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "pipelined reverse pass");
+    } else {
+        HIP_TRY(hipEventRecord(c->ev2, st), "hipEventRecord");
+        for (int64_t seg = C1 - 1; seg >= 0; seg--) {
+            const int64_t k0 = seg * K1, len = std::min(K1, T - k0);
+            const int64_t n2 = (len + K2 - 1) / K2;
+            if (seg != C1 - 1) {
                 HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
-                HIP_TRY(hipMemcpyAsync(work, ick + (size_t)j * ddf, ddf, hipMemcpyDeviceToDevice, st),
-                        "restore inner");
-                if ((rc = adj_forward(L, work, alpha, t0, kk, l2, o, P, rec, nullptr, ntIdx, recNt, rec19)))
-                    return rc;
+                HIP_TRY(hipMemcpyAsync(work, ck + (size_t)seg * ddf, ddf, hipMemcpyDeviceToDevice, st), "restore");
+                if ((rc = advance_outer(work, k0, len, nullptr))) return rc;
                 HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
                 HIP_TRY(hipEventSynchronize(c->ev1), "replay");
                 tr += elapsed(c);
             }
-            HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
-            for (int64_t k = kk + l2 - 1; k >= kk; k--) {
-                const int64_t t = t0 + k;
-                const StepParams sp = make_params(L, t, false);
-                AdjStep q;
-                memset(&q, 0, sizeof(q));
-                q.nx = (uint32_t)L->nx; q.ny = (uint32_t)L->ny; q.nz = (uint32_t)L->nz;
-                q.N = L->N; q.S = L->S;
-                q.divx = sp.divx; q.divy = sp.divy;
-                q.wp = sp.omega; q.wm = sp.omegaMinus;
-                q.fx = sp.fx; q.fy = sp.fy; q.fz = sp.fz;
-                q.rOdd = (double)ramp(L, (t & 1) ? t : t + 1);
-                q.uinTarget = L->uinTarget;
-                q.inflowUniform = L->inflowUniform ? 1 : 0;
-                const bool win = k >= o->spinup;
-                q.inletX = o->inlet_x; q.outletX = o->outlet_x;
-                q.cin = win ? (float)P.cin : 0.0f;
-                q.cout = win ? (float)P.cout : 0.0f;
-                q.wd = win ? (float)P.wd : 0.0f;
-                q.wf = win ? (float)P.wf : 0.0f;
-                q.M = (uint32_t)M;
-                q.spongeX0 = sp.spongeX0; q.spongeInvW = sp.spongeInvW; q.spongeTau = sp.spongeTau;
-                q.cSmago = sp.cSmago;
-                q.srt = sp.omegaMinus == sp.omega ? 1 : 0;
-                q.precision = L->precision;
-                HIP_TRY(launch_adjoint_step(q, L->walls, rec ? rec + (size_t)(k - kk) * N : nullptr, lamN, lamT,
-                                            L->flags, L->link, alpha, gA, gU, L->useEps ? L->eps : nullptr,
-                                            ntIdx, recNt ? recNt + (size_t)(k - kk) * 19 * M : nullptr, gE,
-                                            rec19 ? rec19 + (size_t)(k - kk) * 19 * N * L->B : nullptr, L->lidm,
-                                            sp, st),
-                        "adjoint step launch");
-                std::swap(lamN, lamT);
+            for (int64_t j = n2 - 1; j >= 0; j--) {
+                const int64_t kk = k0 + j * K2, l2 = std::min(K2, len - j * K2);
+                if (j != n2 - 1) {
+                    HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
+                    HIP_TRY(hipMemcpyAsync(work, ick + (size_t)j * ddf, ddf, hipMemcpyDeviceToDevice, st),
+                            "restore inner");
+                    if ((rc = adj_forward(L, work, alpha, t0, kk, l2, o, P, rec, nullptr, ntIdx, recNt, rec19)))
+                        return rc;
+                    HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
+                    HIP_TRY(hipEventSynchronize(c->ev1), "replay");
+                    tr += elapsed(c);
+                }
+                HIP_TRY(hipEventRecord(c->ev0, st), "hipEventRecord");
+                if ((rc = reverse_steps(kk, l2, rec, recNt, rec19, st))) return rc;
+                HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
+                HIP_TRY(hipEventSynchronize(c->ev1), "reverse steps");
+                tb += elapsed(c);
             }
-            HIP_TRY(hipEventRecord(c->ev1, st), "hipEventRecord");
-            HIP_TRY(hipEventSynchronize(c->ev1), "reverse steps");
-            tb += elapsed(c);
         }
+        HIP_TRY(hipEventRecord(c->ev3, st), "hipEventRecord");
+        HIP_TRY(hipEventSynchronize(c->ev3), "reverse pass");
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev2, c->ev3), "hipEventElapsedTime");
+        tp = (double)ms * 1e-3;
     }
     if ((rc = reduce_to_host(L, jacc, J))) return rc;
     if (gradUin && (rc = reduce_to_host(L, gU, gradUin))) return rc;
@@ -1951,6 +2052,7 @@ int vjp_run(strh_lattice* L, const strh_objective* o, const AdjPlan& P, VjpPlan 
         info->backward_seconds = tb;
         info->workspace_bytes = W.bytes;
         info->outer_every = K1;
+        info->reverse_pass_seconds = tp;
     }
     if (!std::isfinite(*J)) return fail(STRH_ERR_NOT_FINITE, "objective is not finite");
     return STRH_OK;

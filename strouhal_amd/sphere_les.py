@@ -100,7 +100,7 @@ def adjoint_run(gpu, precision: str, size, D: int, spinup: int, T: int, K: int =
     ge = info.pop("grad_eps").reshape(-1)
     surf = (eps > 0.0) & (eps < 1.0)
     core = eps >= 1.0
-    adj_s = info["forward_seconds"] + info["recompute_seconds"] + info["backward_seconds"]
+    adj_s = info["forward_seconds"] + info["reverse_pass_seconds"]   # replays and reverse steps may overlap
     n = sim.cells
     # C_D of the averaged partial-cell drag, and its gradient in those units
     cd_scale = 2.0 / (case.u * case.u * np.pi * D * D / 4.0)
