@@ -1812,13 +1812,16 @@ int vjp_run(strh_lattice* L, const strh_objective* o, const AdjPlan& P, VjpPlan 
     if ((rc = W.get(&ck, (size_t)(C1 > 1 ? C1 - 1 : 1) * ddf, "checkpoints"))) return rc;
     if (C2 > 1 && (rc = W.get(&ick, (size_t)(C2 - 1) * ddf, "inner checkpoints"))) return rc;
     if ((rc = W.get(&work, ddf, "replay DDFs"))) return rc;
-    // One level with replays: when a second recording and replay image fit,
-    // segment s - 1 is replayed on a second stream while the reverse steps of
-    // segment s run (pipelined); otherwise the two alternate.
+    // One level with replays on a lattice whose step kernels leave most of the
+    // GPU idle (at most 2^20 cells: latency-bound launches), when a second
+    // recording and replay image fit: segment s - 1 is replayed on a second
+    // stream while the reverse steps of segment s run (pipelined); otherwise
+    // the two alternate. (On bandwidth-bound lattices the overlap measured
+    // within +-5% of alternating: both kinds of kernel want the whole HBM.)
     const size_t recBytes = les ? (size_t)K2 * 19 * N * L->B : (size_t)K2 * N * 16;
     const size_t recNtBytes = L->useEps && !les ? (size_t)K2 * 19 * std::max<size_t>(M, 1) * 4 : 0;
     bool pipe = false;
-    if (C2 == 1 && C1 > 1) {
+    if (C2 == 1 && C1 > 1 && N <= ((size_t)1 << 20)) {
         size_t freeB = 0, totalB = 0;
         HIP_TRY(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
         pipe = freeB > recBytes + recNtBytes + ddf + ((size_t)1 << 30);

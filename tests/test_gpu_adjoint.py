@@ -132,6 +132,26 @@ def test_gradient_independent_of_checkpoint_interval(gpu):
         assert np.array_equal(ga, out[0][1])
 
 
+def test_replays_alternate_or_overlap_with_the_same_bits(gpu):
+    # One checkpoint level with replays: lattices of at most 2^20 cells replay
+    # segment s - 1 on a second stream while the reverse steps of segment s
+    # run; larger ones alternate the two on one stream. Either way J and the
+    # gradients are the bits of the plan that replays nothing (K = T), and the
+    # alternating pass takes at least its replays plus its reverse steps.
+    for nx, ny in ((18, 10), (1040, 1024)):
+        fl = _channel(nx, ny)
+        out = []
+        for k in (3, -1):
+            g, _ = _pair(gpu, "fp32", fl, _alpha(fl.shape, 2))
+            out.append(g.vjp(**OBJ, checkpoint_every=k))
+            g.close()
+        (J3, ga3, gu3, i3), (J0, ga0, gu0, i0) = out
+        assert i0["checkpoints"] == 0 and i3["checkpoint_every"] == 3 and i3["checkpoints"] == 5
+        assert J3 == J0 and gu3 == gu0 and np.array_equal(ga3, ga0), (nx, ny)
+        if nx * ny > (1 << 20):
+            assert i3["reverse_pass_seconds"] >= 0.999 * (i3["recompute_seconds"] + i3["backward_seconds"])
+
+
 def test_vjp_porous_channel_design_field(gpu):
     """The design-field channel at 128x64 (alpha in {0, 1}, 30% porous) over
     400 steps: gradient within 1e-4 of the FP64 oracle's largest component,
