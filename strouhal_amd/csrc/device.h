@@ -383,9 +383,6 @@ struct Offsets {   // cell-index offsets to the wrapped neighbour in each axis d
     }
 };
 
-// Workgroup size of the one-cell-per-thread kernel (64..512 measured within
-// 2% of each other at 256^3; 256 is the best or tied).
-constexpr uint32_t kStepBS = 256;
 
 // Adjoint hooks of the forward pass (AdjHooks in kernels.h). The collision's
 // Jacobian depends on the state only through rho and p, so the reverse step
@@ -741,14 +738,17 @@ __device__ __forceinline__ void step_cell(typename St::T* __restrict__ f,
         force[n] = make_float4(ax, ay, az, 0.0f);
 }
 
+// LIST: workgroup i takes block blockList[i] of kStepBS cells (a lattice's
+// blocks split by whether they hold partial cells, each part with its own
+// instantiation).
 template <class St, int PARITY, bool WALLS, bool NT, bool LES, bool FORCE, bool BRINK = false,
-          bool OUT = true, bool PLANAR = false>
+          bool OUT = true, bool PLANAR = false, bool LIST = false>
 __global__ void __launch_bounds__(kStepBS)
 step_kernel(typename St::T* __restrict__ f, const uint8_t* __restrict__ flags,
             const uint32_t* __restrict__ link, const uint32_t* __restrict__ lidm,
             float4* __restrict__ force, const float* __restrict__ epsBuf, StepParams p,
-            const float* __restrict__ alphaBuf) {
-    const uint32_t n = blockIdx.x * kStepBS + threadIdx.x;
+            const float* __restrict__ alphaBuf, const uint32_t* __restrict__ blockList) {
+    const uint32_t n = (LIST ? blockList[blockIdx.x] : blockIdx.x) * kStepBS + threadIdx.x;
     if (n >= p.N) return;
     step_cell<St, PARITY, WALLS, NT, LES, FORCE, BRINK, false, OUT, PLANAR>(f, flags, link, lidm, force,
                                                                             epsBuf, p, n, alphaBuf);

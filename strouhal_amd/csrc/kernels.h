@@ -13,6 +13,10 @@ namespace strh {
 // Everything one step launch needs. The boolean features select the kernel
 // specialisation; the host sets them from the lattice's contents so the
 // all-fluid periodic bulk runs the mask-free, eps-free, force-free kernel.
+// Workgroup size of the one-cell-per-thread kernel (64..512 measured within
+// 2% of each other at 256^3; 256 is the best or tied).
+constexpr uint32_t kStepBS = 256;
+
 struct StepLaunch {
     int precision;
     void* f;
@@ -28,6 +32,12 @@ struct StepLaunch {
     uint32_t rows_bs;  // 256 / 512: odd pass uses LDS row staging; 0: direct
     bool large;        // blocked image of 4 GiB or more (row-staged pass with row bases)
     bool planar;       // one cell deep (nz = 1): the direct odd pass's 18 neighbours are 8 cells
+    // partial-cell lattices of many blocks whose partial cells sit in few of
+    // them: the kStepBS-cell blocks without one (nPlain, then the nNt with
+    // one) in ntSplit, run by the kernel without the partial-cell path and
+    // the kernel with it; null: one kernel for every block
+    const uint32_t* ntSplit;
+    uint32_t nPlain, nNt;
     const float* alpha;  // Brinkman porosity per cell, or null
     StepParams p;
 };

@@ -81,6 +81,8 @@ struct strh_lattice {
     uint32_t* lidm = nullptr;   // N moving-wall masks (walls only)
     float* force = nullptr;     // 4N floats (wants_forces)
     float* eps = nullptr;       // N solid fractions (after set_solid_fractions)
+    uint32_t* ntSplit = nullptr;   // step blocks without, then with, partial cells (StepLaunch)
+    uint32_t nPlainBlocks = 0, nNtBlocks = 0;
     float* alpha = nullptr;     // N Brinkman porosities (after set_porosity)
     float* moments = nullptr;   // 4N floats, allocated on first probe
     double* partials = nullptr; // reduction scratch
@@ -216,6 +218,9 @@ StepLaunch make_launch(const strh_lattice* L, int64_t t, bool writeForce) {
     s.rows_bs = L->walls ? 0u : rows_block_size((uint32_t)L->nx, L->N);
     s.large = large_image(L);
     s.planar = L->nz == 1;
+    s.ntSplit = L->useEps ? L->ntSplit : nullptr;
+    s.nPlain = L->nPlainBlocks;
+    s.nNt = L->nNtBlocks;
     s.alpha = L->alpha;
     s.p = make_params(L, t, writeForce);
     return s;
@@ -261,7 +266,7 @@ void free_lattice(strh_lattice* L) {
     if (!L) return;
     if (L->ctx) hipSetDevice(L->ctx->device);
     void* ptrs[] = {L->f, L->flags, L->link, L->lidm, L->force, L->eps, L->alpha, L->moments,
-                    L->partials, L->result, L->snap};
+                    L->partials, L->result, L->snap, L->ntSplit};
     for (void* p : ptrs) if (p) hipFree(p);
     delete L;
 }
@@ -511,6 +516,30 @@ int strh_set_solid_fractions(strh_lattice* L, const float* eps) {
     if (!L->eps) HIP_TRY(hipMalloc(&L->eps, (size_t)L->N * 4), "hipMalloc(eps)");
     HIP_TRY(hipMemcpy(L->eps, eps, (size_t)L->N * 4, hipMemcpyHostToDevice), "copy eps");
     L->useEps = true;
+    // Lattices of at least 2^22 cells (3-D) whose partial cells lie in at most
+    // half of the kStepBS-cell blocks run those blocks with the partial-cell
+    // kernel and the rest without it (two launches per pass: on smaller
+    // lattices the second launch would cost more than it saves).
+    if (L->ntSplit) HIP_TRY(hipFree(L->ntSplit), "hipFree(split)");
+    L->ntSplit = nullptr;
+    L->nPlainBlocks = L->nNtBlocks = 0;
+    if (L->nz > 1 && L->N >= (1u << 22)) {
+        const uint32_t nb = (L->N + kStepBS - 1) / kStepBS;
+        std::vector<uint32_t> plain, nt;
+        for (uint32_t b = 0; b < nb; b++) {
+            bool any = false;
+            for (uint32_t n = b * kStepBS; n < std::min<uint32_t>(L->N, (b + 1) * kStepBS) && !any; n++)
+                any = eps[n] > 0.0f;
+            (any ? nt : plain).push_back(b);
+        }
+        if (nt.size() * 2 <= nb) {
+            plain.insert(plain.end(), nt.begin(), nt.end());
+            HIP_TRY(hipMalloc(&L->ntSplit, plain.size() * 4), "hipMalloc(split)");
+            HIP_TRY(hipMemcpy(L->ntSplit, plain.data(), plain.size() * 4, hipMemcpyHostToDevice), "copy split");
+            L->nPlainBlocks = nb - (uint32_t)nt.size();
+            L->nNtBlocks = (uint32_t)nt.size();
+        }
+    }
     return STRH_OK;
 }
 

@@ -33,19 +33,36 @@ static hipError_t launch_step_t(const StepLaunch& L, hipStream_t s) {
         if (NT || LES) return hipErrorNotSupported;
         hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, false, false, true, true, true, PLANAR>), dim3(p.blocks),
                            dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
-                           (float4*)L.force, L.eps, p, L.alpha);
+                           (float4*)L.force, L.eps, p, L.alpha, (const uint32_t*)nullptr);
         return hipGetLastError();
     }
     // Steps that write no force run the kernel with the force output compiled
     // out (only walls and partial cells have one).
-    if ((WALLS || NT) && p.writeForce == 0)
-        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE, false, false, PLANAR>), dim3(p.blocks),
-                           dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
-                           (float4*)L.force, L.eps, p, (const float*)nullptr);
-    else
-        hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NT, LES, FORCE, false, true, PLANAR>), dim3(p.blocks),
-                           dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,
-                           (float4*)L.force, L.eps, p, (const float*)nullptr);
+    const bool out = !((WALLS || NT) && p.writeForce == 0);
+#define STRH_STEP(NTV, OUTV, LISTV, GRID, LISTP)                                                    \
+    hipLaunchKernelGGL((step_kernel<St, PARITY, WALLS, NTV, LES, FORCE, false, OUTV, PLANAR, LISTV>), \
+                       dim3(GRID), dim3(kStepBS), 0, s, (T*)L.f, L.flags, L.link, L.lidm,          \
+                       (float4*)L.force, L.eps, p, (const float*)nullptr, (const uint32_t*)(LISTP))
+    // The split pays in the even pass (LES sphere 0.474 -> 0.429 ms FP16,
+    // 0.899 -> 0.880 FP32, 0.513 -> 0.460 BF16) and in the FP32 odd pass
+    // (1.066 -> 1.021 ms); the 16-bit odd passes measured slower split
+    // (0.698 -> 0.732 ms FP16, 0.772 -> 0.800 BF16), so they keep one kernel.
+    if constexpr (NT && !PLANAR && (PARITY == 0 || sizeof(T) == 4)) {
+        if (L.ntSplit) {
+            // the blocks without partial cells take the kernel without their
+            // path (fewer registers: more waves per SIMD); for cells with
+            // eps = 0 both compute the same bits (the golden path)
+            if (L.nPlain) { if (out) STRH_STEP(false, true, true, L.nPlain, L.ntSplit); else STRH_STEP(false, false, true, L.nPlain, L.ntSplit); }
+            if (L.nNt) { if (out) STRH_STEP(true, true, true, L.nNt, L.ntSplit + L.nPlain); else STRH_STEP(true, false, true, L.nNt, L.ntSplit + L.nPlain); }
+            return hipGetLastError();
+        }
+    }
+    if (out) {
+        STRH_STEP(NT, true, false, p.blocks, nullptr);
+    } else {
+        STRH_STEP(NT, false, false, p.blocks, nullptr);
+    }
+#undef STRH_STEP
     return hipGetLastError();
 }
 

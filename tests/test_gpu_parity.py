@@ -170,6 +170,33 @@ def test_partial_cells_les_sponge_walls(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+def test_partial_cells_in_few_blocks_of_a_large_lattice(gpu, precision):
+    """A lattice of 2^22 cells whose partial cells lie in few of its 256-cell
+    blocks runs those blocks with the partial-cell kernel and the others with
+    the kernel without that path: still bit-exact to the oracle, probe steps
+    included (the blocks on both sides of the split hold wall links)."""
+    nx, ny, nz = 256, 128, 128
+    rng = np.random.default_rng(31)
+    eps = np.zeros((nz, ny, nx), dtype=np.float32)
+    eps[58:70, 56:72, 40:60] = rng.uniform(0.0, 1.0, size=(12, 16, 20)).astype(np.float32)
+    eps[64, 64, 50] = 1.0
+    eps[3, 1, 10] = 0.5                     # a partial cell beside a wall
+    g, r = make_pair(gpu, precision, nx, ny, nz, 1.85, None, uin=0.05, c_smago=0.1,
+                     flags=_flags("channel", nx, ny, nz), wants_forces=True)
+    for s in (g, r):
+        s.set_solid_fractions(eps)
+        s.sponge = (200.0, 40.0, 1.0)
+    _seed_both(g, r, precision, seed=37)
+    _run_compare(g, r, [2], precision)
+    fg = g.probe_force((30, 70), (1, 80))
+    fr = r.probe_force((30, 70), (1, 80))
+    assert_bit_exact(g.download_ddf(), r.download_ddf(), "DDFs after force probe")
+    assert_bit_exact(g.download_force(), r.force_field().astype(np.float32), "per-cell force")
+    scale = np.abs(r.force_field()[:, :3].astype(np.float64)).sum() + 1e-300
+    assert np.all(np.abs(fg - fr) <= 1e-13 * scale), (fg, fr)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("kind,shape,force", [("channel", (30, 14, 2), (1e-5, 0.0, 0.0)),
                                               ("periodic", (64, 8, 3), (0.0, 0.0, 0.0)),
                                               ("mixed", (15, 9, 7), (2e-5, -1e-5, 5e-6))])
