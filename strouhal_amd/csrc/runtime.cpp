@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1040,7 +1041,8 @@ size_t tile_lds(int w, int h, int bytes) {
 }
 
 // The split of nx x ny into at most g tiles with the fewest cells in its
-// largest tile (then the shortest edge), or gx = 0 when none fits in LDS.
+// largest tile counted with its halo ring (then the fewest without), or gx = 0
+// when none fits in LDS.
 TilePlan plan_lattice(int nx, int ny, int g, int bytes) {
     TilePlan best;
     int64_t bestCells = -1, bestEdge = 0;
@@ -1049,7 +1051,9 @@ TilePlan plan_lattice(int nx, int ny, int g, int bytes) {
         if (gy < 1) continue;
         const int w = (nx + gx - 1) / gx, h = (ny + gy - 1) / gy;
         if (tile_lds(w, h, bytes) > kResidentMaxLds || w * h > kResidentMaxRounds * kResidentBS) continue;
-        const int64_t cells = (int64_t)w * h, edge = (int64_t)w + h;
+        // tile and halo cells: the cells a step updates plus the edge the
+        // exchange trades (long thin tiles trade more words per cell)
+        const int64_t cells = (int64_t)(w + 2) * (h + 2), edge = (int64_t)w * h;
         if (bestCells < 0 || cells < bestCells || (cells == bestCells && edge < bestEdge)) {
             best.gx = gx; best.gy = gy; best.wmax = w; best.hmax = h;
             bestCells = cells; bestEdge = edge;
@@ -1061,10 +1065,69 @@ TilePlan plan_lattice(int nx, int ny, int g, int bytes) {
 
 // Tiles of the lattices `members` (indices into lats) sharing `slots` workgroups;
 // false when some lattice gets no split that fits in LDS.
+// The fewest tiles that split lattice L with at most r * kResidentBS cells in
+// its largest tile (r = 1 .. kResidentMaxRounds rounds of the workgroup),
+// index r; 0 where no split into at most `slots` tiles does.
+using RoundTiles = std::array<int, kResidentMaxRounds + 1>;
+RoundTiles round_tiles(const strh_lattice* L, int slots, int bytes) {
+    RoundTiles t{};
+    for (int r = 1; r <= kResidentMaxRounds; r++) {
+        const int64_t cap = (int64_t)r * kResidentBS;
+        for (int g = std::max<int64_t>(1, ((int64_t)L->N + cap - 1) / cap); g <= slots; g++) {
+            const TilePlan tp = plan_lattice(L->nx, L->ny, g, bytes);
+            if (tp.gx != 0 && (int64_t)tp.wmax * tp.hmax <= cap) { t[r] = g; break; }
+        }
+    }
+    return t;
+}
+
+// rt: round_tiles of each lattice (indexed like lats), or null. With it the
+// group takes the fewest rounds r whose tile counts fit the slots: each
+// lattice starts from its fewest tiles of at most r rounds, and the slots left
+// go one by one to the lattice furthest below its share in proportion to its
+// cells (a lattice keeps the fewest tiles if the larger count splits it into
+// tiles of more than r rounds); without it (and when no r fits) slots are
+// shared in proportion alone.
 bool plan_group(const std::vector<strh_lattice*>& lats, const std::vector<int>& members, int slots, int bytes,
-                std::vector<TilePlan>* plans) {
+                std::vector<TilePlan>* plans, const std::vector<RoundTiles>* rt = nullptr) {
     double cells = 0.0;
     for (int i : members) cells += (double)lats[i]->N;
+    if (rt) {
+        const int r0 = std::max(1, (int)std::ceil(cells / ((double)slots * kResidentBS)));
+        for (int r = r0; r <= kResidentMaxRounds; r++) {
+            int used = 0;
+            bool ok = true;
+            for (int i : members) {
+                const int g = (*rt)[(size_t)i][(size_t)r];
+                if (g == 0) { ok = false; break; }
+                used += g;
+            }
+            if (!ok || used > slots) continue;
+            if (plans) {
+                std::vector<int> gs;
+                for (int i : members) gs.push_back((*rt)[(size_t)i][(size_t)r]);
+                for (; used < slots; used++) {
+                    int pick = -1;
+                    double deficit = 0.0;
+                    for (size_t q = 0; q < members.size(); q++) {
+                        const double d = (double)slots * (double)lats[members[q]]->N / cells - (double)gs[q];
+                        if (d > deficit) { deficit = d; pick = (int)q; }
+                    }
+                    if (pick < 0) break;
+                    gs[(size_t)pick]++;
+                }
+                plans->clear();
+                for (size_t q = 0; q < members.size(); q++) {
+                    const strh_lattice* L = lats[members[q]];
+                    TilePlan tp = plan_lattice(L->nx, L->ny, gs[q], bytes);
+                    if (tp.gx == 0 || (int64_t)tp.wmax * tp.hmax > (int64_t)r * kResidentBS)
+                        tp = plan_lattice(L->nx, L->ny, (*rt)[(size_t)members[q]][(size_t)r], bytes);
+                    plans->push_back(tp);
+                }
+            }
+            return true;
+        }
+    }
     int used = 0;
     std::vector<TilePlan> out;
     for (int i : members) {
@@ -1150,6 +1213,8 @@ bool plan_segment(strh_batch* B, const std::vector<int>& members, Segment& sg, s
         return B->lats[members[a]]->N > B->lats[members[b]]->N;
     });
     auto forced_of = [&](int j) { return B->host[members[j]].hasForce != 0; };
+    std::vector<RoundTiles> rt(B->lats.size());
+    for (int i : members) rt[(size_t)i] = round_tiles(B->lats[i], B->residentSlots, bytes);
     std::vector<std::vector<int>> groups;   // positions in members
     auto lat_ids = [&](const std::vector<int>& g) {
         std::vector<int> v;
@@ -1163,10 +1228,10 @@ bool plan_segment(strh_batch* B, const std::vector<int>& members, Segment& sg, s
             if (forced_of(g[0]) != forced_of(j)) continue;
             std::vector<int> cand = g;
             cand.push_back(j);
-            if (plan_group(B->lats, lat_ids(cand), B->residentSlots, bytes, nullptr)) { g = cand; placed = true; break; }
+            if (plan_group(B->lats, lat_ids(cand), B->residentSlots, bytes, nullptr, &rt)) { g = cand; placed = true; break; }
         }
         if (placed) continue;
-        if (!plan_group(B->lats, lat_ids({j}), B->residentSlots, bytes, nullptr)) return false;
+        if (!plan_group(B->lats, lat_ids({j}), B->residentSlots, bytes, nullptr, &rt)) return false;
         groups.push_back({j});
     }
     // A group's step costs about one round of cells per thread plus one more
@@ -1180,7 +1245,7 @@ bool plan_segment(strh_batch* B, const std::vector<int>& members, Segment& sg, s
         int64_t c = 0;
         for (const auto& g : gs) {
             std::vector<TilePlan> plans;
-            if (g.empty() || !plan_group(B->lats, lat_ids(g), B->residentSlots, bytes, &plans)) return -1;
+            if (g.empty() || !plan_group(B->lats, lat_ids(g), B->residentSlots, bytes, &plans, &rt)) return -1;
             int cmax = 0;
             for (const TilePlan& tp : plans) cmax = std::max(cmax, tp.wmax * tp.hmax);
             c += (cmax + kResidentBS - 1) / kResidentBS + 1;
@@ -1199,7 +1264,7 @@ bool plan_segment(strh_batch* B, const std::vector<int>& members, Segment& sg, s
                 if (pick >= 0 && fill[q] >= fill[(size_t)pick]) continue;
                 std::vector<int> cand = gs[q];
                 cand.push_back(j);
-                if (plan_group(B->lats, lat_ids(cand), B->residentSlots, bytes, nullptr)) pick = (int)q;
+                if (plan_group(B->lats, lat_ids(cand), B->residentSlots, bytes, nullptr, &rt)) pick = (int)q;
             }
             if (pick < 0) { ok = false; break; }
             gs[(size_t)pick].push_back(j);
@@ -1211,7 +1276,7 @@ bool plan_segment(strh_batch* B, const std::vector<int>& members, Segment& sg, s
     }
     for (const auto& g : groups) {
         std::vector<TilePlan> plans;
-        plan_group(B->lats, lat_ids(g), B->residentSlots, bytes, &plans);
+        plan_group(B->lats, lat_ids(g), B->residentSlots, bytes, &plans, &rt);
         ResGroup rg;
         rg.first = tiles.size();
         rg.lds = 0;
