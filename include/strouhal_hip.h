@@ -214,12 +214,13 @@ STRH_API int strh_batch_probe_series(strh_batch* batch, int index, double* out,
  * restored after). */
 STRH_API int strh_batch_set_block_size(strh_batch* batch, int block_size);
 STRH_API int strh_batch_autotune(strh_batch* batch, int* block_size);
-/* Steps per captured HIP graph (0: plain launches). Default 64. */
+/* Steps per captured HIP graph (0: plain launches). Default 256. */
 STRH_API int strh_batch_set_graph_steps(strh_batch* batch, int steps);
 STRH_API int strh_batch_gpu_seconds(strh_batch* batch, double* seconds);
 /* Resident launches (the persistent launch of Ladder.swift:54-121's rungs):
  * lattices one cell deep (nz = 1) are cut into tiles held in LDS, one
- * workgroup per CU, and advanced kProbeSpan (64) steps per launch, trading
+ * workgroup per CU, and advanced up to 256 steps per launch (64 where a
+ * probed lattice's ring of force buffers for 256 steps would exceed 1 GiB), trading
  * only their edge populations between steps; groups of lattices that do not
  * fit on chip together are time-sliced. Results are the same bits as the
  * streaming kernel's. mode 0: never; 1: in every run segment whose active

@@ -2,9 +2,9 @@
 
 A batch advances several lattices together, each with its own step count,
 ramp and momentum-exchange probe schedule held on the device: lattices one
-cell deep are held in LDS for 64 steps per launch (resident launches, where
-they measure faster), others stream through memory with one launch per step
-(replayed as HIP graphs). It replaces the
+cell deep are held in LDS for up to 256 steps per launch (resident launches,
+where they measure faster), others stream through memory with one launch per
+step (replayed as HIP graphs). It replaces the
 sequential rung loop of the reference's Ladder.credibility / Ladder.measure
 (Sources/StrouhalCore/Ladder.swift:54-121); every batched lattice evolves bit
 for bit as it does when run alone.

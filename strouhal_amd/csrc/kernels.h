@@ -90,8 +90,9 @@ constexpr int kResidentMaxRounds = 4;
 // A wave that waits this long for a neighbour's exchange word aborts the
 // batch by default (ticks of the 100 MHz wall clock: one second).
 constexpr uint64_t kResidentSpinTicks = 100000000ull;
-// LDS bytes a resident tile may use (the workgroup's static LDS stays below 512 bytes).
-constexpr size_t kResidentMaxLds = 163840 - 512;
+// LDS bytes a resident tile may use (the workgroup's static LDS, the ramp
+// factors of kProbeSpan steps among it, stays below 1,536 bytes).
+constexpr size_t kResidentMaxLds = 163840 - 1536;
 
 // Hooks of the recording forward step of the adjoint (walls, Guo forcing,
 // Brinkman porosity, partial cells, LES, sponge).
@@ -156,7 +157,11 @@ hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool f
                              const BatchLat* lats,
                              int L, uint32_t totalBlocks, const int64_t* base, int k, hipStream_t s);
 // Probes every force step among steps k .. k + span - 1 (span <= kProbeSpan).
-constexpr int kProbeSpan = 64;
+// A batch reduces its force samples, and advances a resident group per
+// launch, over spans of kProbeSpan steps, or kProbeSpanSmall where a
+// lattice's ring of force buffers for the longer span would be too large.
+constexpr int kProbeSpan = 256;
+constexpr int kProbeSpanSmall = 64;
 hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s);
 hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
 // Steps k0 .. k0 + K - 1 (K <= kProbeSpan) of every tile of one resident group;

@@ -538,9 +538,9 @@ step_resident(const BatchLat* __restrict__ lats, const ResTile* __restrict__ til
         for (int i = 0; i < 19; i++) s[c * 19u + (uint32_t)i] = v[i];
     }
     const int64_t tBase = B.step0 + *base + k0;
-    if (tid < K) {
-        const int64_t t = tBase + tid;                                 // Sim.swift:258-266
-        rampL[tid] = t < B.rampSteps ? ((const GLOBAL float*)B.ramp)[t] : 1.0f;
+    for (int q = tid; q < K; q += bsz) {
+        const int64_t t = tBase + q;                                   // Sim.swift:258-266
+        rampL[q] = t < B.rampSteps ? ((const GLOBAL float*)B.ramp)[t] : 1.0f;
     }
     if (parabolic) {   // the first step's rows (its ramp factor read as rampL[0] is)
         const float r0 = tBase < B.rampSteps ? ((const GLOBAL float*)B.ramp)[tBase] : 1.0f;

@@ -939,9 +939,10 @@ struct strh_batch {
     std::vector<int64_t> lastSteps;      // stepsDone of each lattice after its last batch run
     std::vector<void*> fring;            // per probed lattice, ring of force buffers (device)
     std::vector<int> fringLen;           // buffers in that ring
+    std::vector<int> fringSpan;          // steps whose samples the ring holds (kProbeSpan or kProbeSpanSmall)
     size_t devCap = 0;                   // descriptors `dev` can hold
     int bs = 256;
-    int graphSteps = 64;
+    int graphSteps = kProbeSpan;
     double gpuSeconds = 0.0;
     // resident launches (resident.hip)
     int resident = 3;                    // 0 never, 1 one launch per segment, 2 required, 3 measured
@@ -1145,16 +1146,25 @@ bool plan_group(const std::vector<strh_lattice*>& lats, const std::vector<int>& 
     return true;
 }
 
+// The batch's span: the shortest span of its lattices' force rings.
+int batch_span(const strh_batch* B) {
+    int span = kProbeSpan;
+    for (size_t i = 0; i < B->lats.size(); i++)
+        if (B->fring[i]) span = std::min(span, B->fringSpan[i]);
+    return span;
+}
+
 // Launches steps k = kOff .. kOff+count-1 from base; parity0 is the common
 // parity at k = kOff (or -1), so step k runs the pass (parity0 + k - kOff) & 1 only.
-// The force samples of every kProbeSpan steps (and of the last few) are
+// The force samples of every span of steps (and of the last few) are
 // reduced by one probe launch; each lattice's ring holds that many samples.
 int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int parity0, int kOff = 0) {
     hipStream_t s = B->ctx->stream;
+    const int span = batch_span(B);
     if (!sg.groups.empty()) {
-        // resident: each group advances kProbeSpan steps per launch, then the probes
-        for (int j = 0; j < count; j += kProbeSpan) {
-            const int K = std::min(kProbeSpan, count - j);
+        // resident: each group advances a span of steps per launch, then the probes
+        for (int j = 0; j < count; j += span) {
+            const int K = std::min(span, count - j);
             for (const ResGroup& g : sg.groups)
                 HIP_TRY(launch_resident(B->precision, g.les, g.forced, sg.dev, B->tiles + g.first, g.count, g.bs, g.lds,
                                         B->base, kOff + j, K, B->abortFlag, B->residentSpin, s),
@@ -1169,7 +1179,7 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
         const int par = parity0 < 0 ? -1 : (parity0 + j) & 1;
         HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.planar, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
-        if (sg.probes && (k + 1 - k0 == kProbeSpan || j == count - 1)) {
+        if (sg.probes && (k + 1 - k0 == span || j == count - 1)) {
             HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k0, k + 1 - k0, s), "batch probe launch");
             k0 = k + 1;
         }
@@ -1521,6 +1531,7 @@ int strh_batch_create(strh_ctx* c, strh_lattice* const* lats, int count, strh_ba
     for (int i = 0; i < count; i++) B->lastSteps[i] = lats[i]->stepsDone;
     B->fring.assign(count, nullptr);
     B->fringLen.assign(count, 0);
+    B->fringSpan.assign(count, kProbeSpan);
     B->xchg.assign(count, nullptr);
     auto bail = [&](hipError_t e, const char* what) {
         strh_batch_destroy(B);
@@ -1582,6 +1593,7 @@ int strh_batch_set_probe(strh_batch* B, int index, int64_t first, int64_t period
     if (B->fring[index]) hipFree(B->fring[index]);
     B->fring[index] = nullptr;
     B->fringLen[index] = 0;
+    B->fringSpan[index] = kProbeSpan;
     if (count > 0) HIP_TRY(hipMalloc(&B->series[index], (size_t)count * 3 * sizeof(double)), "hipMalloc(series)");
     d.series = B->series[index];
     B->taken[index] = 0;
@@ -1591,9 +1603,16 @@ int strh_batch_set_probe(strh_batch* B, int index, int64_t first, int64_t period
         HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)B->series[index], 0x7ff80000u, (size_t)count * 6,
                                   B->ctx->stream), "fill series");
     if (count > 0) {
-        // One force buffer per sample of a probe launch's kProbeSpan steps, plus one.
-        const int ring = (int)((kProbeSpan + period - 1) / period) + 1;
+        // One force buffer per sample of a probe launch's span, plus one: the
+        // long span where that ring stays within 1 GiB, else the short one.
+        int span = kProbeSpan;
+        int ring = (int)((span + period - 1) / period) + 1;
+        if ((size_t)ring * L->N * 16 > ((size_t)1 << 30)) {
+            span = kProbeSpanSmall;
+            ring = (int)((span + period - 1) / period) + 1;
+        }
         const size_t bytes = (size_t)ring * L->N * 16;
+        B->fringSpan[index] = span;
         HIP_TRY(hipMalloc(&B->fring[index], bytes), "hipMalloc(force ring)");
         B->fringLen[index] = ring;
         // Cells outside the momentum-exchange set are never written on force

@@ -150,7 +150,7 @@ def run_sequential(variants):
     return series, time.perf_counter() - t0
 
 
-def run_batched(variants, autotune=True, graph_steps=64, resident=3):
+def run_batched(variants, autotune=True, graph_steps=256, resident=3):
     b = Batch([v.case.sim for v in variants])
     b.set_resident(resident)
     for i, v in enumerate(variants):
@@ -165,7 +165,7 @@ def run_batched(variants, autotune=True, graph_steps=64, resident=3):
     return series, wall, bs, b
 
 
-def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64, resident=3):
+def step_rates(gpu, precision, res, base_umax, steps, graph_steps=256, resident=3):
     """Microseconds per lock-step step of every rung together and of the finest
     rung alone, from the rungs' initial state."""
     out = {}
@@ -186,7 +186,7 @@ def step_rates(gpu, precision, res, base_umax, steps, graph_steps=64, resident=3
     return out
 
 
-def segment_rates(gpu, precision, res, base_umax, steps, graph_steps=64):
+def segment_rates(gpu, precision, res, base_umax, steps, graph_steps=256):
     """Microseconds per lock-step step of each set of rungs that is active
     together in the schedule (every rung, then without those that finish
     first, ...), with the streaming kernel (resident mode 0) and with resident
@@ -230,7 +230,7 @@ def main():
     ap.add_argument("--segment-rates", type=int, default=0,
                     help="instead of the schedule: time this many steps of each set of rungs active "
                          "together, streaming and resident")
-    ap.add_argument("--graph-steps", type=int, default=64)
+    ap.add_argument("--graph-steps", type=int, default=256)
     ap.add_argument("--resident", type=int, default=3,
                     help="0: streaming kernel only; 1: resident launches where the active rungs fit on chip "
                          "in one launch; 2: resident launches everywhere (time-sliced groups); 3: as 1, and "

@@ -47,9 +47,9 @@ def test_ladder_batched_equals_sequential(gpu, precision):
     assert vb[-2].case.sim.state_digest in md          # the finest Mach-1 rung's digest
 
 
-@pytest.mark.parametrize("precision,resident", [("fp32", 1), ("fp16s", 1), ("fp32", 2), ("fp16s", 2),
-                                                ("fp32", 3), ("fp16s", 3)])
-def test_full_size_ladder_batched_equals_sequential(gpu, precision, resident):
+@pytest.mark.parametrize("precision,resident,graph", [("fp32", 1, 64), ("fp16s", 1, 256), ("fp32", 2, 256),
+                                                      ("fp16s", 2, 64), ("fp32", 3, 64), ("fp16s", 3, 64)])
+def test_full_size_ladder_batched_equals_sequential(gpu, precision, resident, graph):
     # The ladder at its full size (D in {16, 24, 32, 48, 64}, each at Ma 0.1
     # and half Mach: 10 lattices, 1,504,680 cells) on 1% of the reference
     # schedule: the batch ends every lattice on the sequential run's bits,
@@ -62,8 +62,10 @@ def test_full_size_ladder_batched_equals_sequential(gpu, precision, resident):
     # resident 1: LDS-resident launches for the segments whose rungs fit on
     # chip together (the tail); 2: for every segment, in time-sliced groups;
     # 3: as 1, and the segments that need groups switch to the faster kind
-    # after timing both (segments of 512 steps and more at this scale)
-    sb, _, _, _ = run_batched(vb, autotune=True, graph_steps=64, resident=resident)
+    # after timing both (segments of 8 graphs and more). 256-step graphs hold
+    # resident launches of 256 steps, with the force samples of each reduced
+    # after it.
+    sb, _, _, _ = run_batched(vb, autotune=True, graph_steps=graph, resident=resident)
     ss, _ = run_sequential(vs)
     for a, b, v in zip(sb, ss, vb):
         assert a.shape == b.shape == (v.probes, 3)
@@ -107,6 +109,42 @@ def test_mixed_batch_matches_solo_runs(gpu, bs, graph):
     for x, y in zip(batched, solo):
         assert x.steps_done == y.steps_done
         assert x.state_digest == y.state_digest
+
+
+def test_probe_rings_too_large_for_the_long_span(gpu):
+    """A probed lattice whose ring of force buffers for 256 steps would pass
+    1 GiB (600,000 cells, a sample every second step) takes 64-step spans: its
+    resident launches and probe reductions then cover 64 steps inside the
+    256-step graphs. Resident and streaming batches give the same samples, and
+    the state is the solo run's."""
+    from strouhal_amd import Simulation
+    from strouhal_amd.cases import channel_flags
+
+    def make():
+        nx, ny = 1000, 600
+        s = Simulation(gpu, "fp16s", nx=nx, ny=ny, nz=1, omega=1.7, uin=0.05, ramp_steps=20,
+                       flags=channel_flags(nx, ny), wants_forces=True)
+        s.init_field(0, 0.0)
+        return s
+
+    out = []
+    for mode in (2, 0):
+        sim = make()
+        bt = Batch([sim])
+        bt.set_resident(mode)
+        bt.set_graph_steps(256)
+        bt.set_probe(0, 1, 2, 300, (1, 40), (1, 598))     # odd steps 1, 3, ..., 599
+        bt.run([600])
+        out.append((bt.probe_series(0), sim.state_digest))
+        bt.close()
+        sim.close()
+    solo = make()
+    solo.run(600)
+    (s2, d2), (s0, d0) = out
+    assert s2.shape == s0.shape == (300, 3)
+    assert np.all(np.isfinite(s2)) and s2[-1, 0] != 0.0
+    assert np.array_equal(s2, s0)
+    assert d2 == d0 == solo.state_digest
 
 
 def test_autotune_keeps_state(gpu):
