@@ -90,8 +90,8 @@ def measure(precision: str, lam, n: int, steps: int, warmup: int):
 
 
 def reference_bench(n: int, warmup: int = 20, steps: int = 1000):
-    """The reference's own bench workload and procedure (Cases.swift:132-151,
-    BASELINE.md section 1): 256^3 periodic, SRT omega = 1.9, initField mode 1
+    """The reference's own bench workload and procedure (Cases.swift:132-151):
+    256^3 periodic, SRT omega = 1.9, initField mode 1
     (2-D Taylor-Green) at A = 0.05, timed with HIP events over `steps` steps
     after `warmup` (the measurement plan asks for >= 20 and >= 1,000), in FP32
     and FP16 storage, beside the reference's published Apple-M5 figures."""
@@ -220,7 +220,7 @@ def main():
     ap.add_argument("--no-reference-bench", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU z-slab sharding is a design, not a measured path.", file=sys.stderr)
         sys.exit(2)
     steps, warmup = args.steps, args.warmup

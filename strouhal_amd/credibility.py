@@ -304,7 +304,7 @@ def markdown(r: CredibilityRun, build_gates, now: datetime.datetime | None = Non
     notes = [n for n in b.notes if n]
     if notes:
         out += ["## Notes\n"] + [f"- {n}" for n in notes] + [""]
-    out += ["## Solver verification (this build)\n"] + [f"- {g}" for g in build_gates] + [""]
+    out += ["## Solver verification (this solver)\n"] + [f"- {g}" for g in build_gates] + [""]
     out += ["## Reproducibility\n", f"- Final state digest (SHA-256): `{r.digest}`",
             f"- Total wall time for the full credibility run: {r.wall_seconds:.1f} s",
             "- The same build, settings and machine give the same state bit for bit and the same QoIs.\n",

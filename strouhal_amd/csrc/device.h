@@ -265,9 +265,8 @@ template <int I, bool FORCE> __device__ __forceinline__ void pair_nt(
     ntz = acc<-kCz[I]>(ntz, tb);
 }
 
-// BRINK: Brinkman porosity alpha (new in this build, no reference
-// counterpart): drag force -alpha rho u with u taken implicitly,
-// u = (p + Fb/2) / (rho (1 + alpha/2)); the Guo terms then see the per-cell
+// BRINK: Brinkman porosity alpha (no reference counterpart): drag force
+// -alpha rho u with u taken implicitly, u = (p + Fb/2) / (rho (1 + alpha/2)); the Guo terms then see the per-cell
 // force F = Fb - alpha rho u. With alpha = 0 every value is the same bits as
 // the plain forcing path.
 template <bool LES, bool FORCE, bool NT, bool BRINK = false>

@@ -99,7 +99,7 @@ def test_budget_and_report():
     assert b.combined == pytest.approx(2.0 * math.sqrt(b.u_num ** 2 + b.u_stat ** 2 + b.u_ma ** 2))
     md = cr.markdown(r, ["gate one: yes"])
     for head in ("# Credibility report — mean C_D", "## Result", "## Uncertainty budget",
-                 "## Resolution ladder", "## Validation domain", "## Solver verification (this build)",
+                 "## Resolution ladder", "## Validation domain", "## Solver verification (this solver)",
                  "## Reproducibility", "## What this report does not claim"):
         assert head in md
     assert b.headline in md and "gate one: yes" in md and "ab" * 32 in md
