@@ -297,7 +297,8 @@ template <class S> struct Lattice final : Base {
     }
 
     // Kernels.metal:105-228
-    // alphav / brink: Brinkman porosity (no reference counterpart): the cell feels the drag force -alpha rho u, with u taken
+    // alphav / brink: Brinkman porosity (no reference counterpart): the cell
+    // feels the drag force -alpha rho u, with u taken
     // implicitly, u = (p + Fb/2) / (rho (1 + alpha/2)), so the force enters
     // the Guo terms as F = Fb - alpha rho u. With alpha = 0 every value is
     // the same bits as without it.
