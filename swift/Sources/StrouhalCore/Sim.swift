@@ -262,7 +262,8 @@ public final class Batch {
     public func autotune() throws -> Int { var bs: Int32 = 0; try check(strh_batch_autotune(h, &bs)); return Int(bs) }
     public func setGraphSteps(_ n: Int) throws { try check(strh_batch_set_graph_steps(h, Int32(n))) }
     /// 0: stream every step through memory; 1: hold lattices one cell deep in LDS where the
-    /// active ones fit on chip in one launch; 2: everywhere, time-sliced.
+    /// active ones fit on chip in one launch; 2: everywhere, time-sliced; 3 (default): as 1,
+    /// and where groups would take turns, the faster of the two measured per segment.
     public func setResident(_ mode: Int) throws { try check(strh_batch_set_resident(h, Int32(mode))) }
     public func setResidentTimeout(seconds: Double) throws {
         try check(strh_batch_set_resident_timeout(h, seconds))
