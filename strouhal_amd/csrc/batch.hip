@@ -67,44 +67,69 @@ step_batch(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__
 
 // After batch steps k .. k + span - 1 (span <= kProbeSpan): for every lattice
 // and every force step t among them, sum the per-cell momentum exchange of
-// that step's ring buffer over the probe box into the time series. Block
-// (l, c) sums component c of lattice l with exactly the chunking of the
-// single-lattice probe (reduce_stage1 over Pb chunks, then reduce_stage2), so
-// the FP64 values are the same bits.
-// The probe's per-chunk partials, read back from LDS in stage 2's order.
-struct SharedPartials {
+// that step's ring buffer over the probe box into the time series, with
+// exactly the chunking of the single-lattice probe (reduce_stage1 over Pb
+// chunks, then reduce_stage2), so the FP64 values are the same bits. Two
+// launches: probe_chunks sums chunk pb of component c of lattice l for the
+// span's j-th force step into parts[((l 3 + c) maxSamples + j) maxPb + pb];
+// probe_totals adds each sample's Pb chunk sums in stage 2's order.
+__device__ __forceinline__ int64_t first_sample(const BatchLat& B, int64_t t0) {
+    return t0 <= B.fT0 ? B.fT0 : B.fT0 + ((t0 - B.fT0 + B.fPeriod - 1) / B.fPeriod) * B.fPeriod;
+}
+__device__ __forceinline__ int64_t span_end(const BatchLat& B, int64_t t0, int span) {
+    int64_t tLast = t0 + span;
+    const int64_t tCap = B.fT0 + B.fCount * B.fPeriod;        // one past the last sample's step
+    tLast = tLast < B.tEnd ? tLast : B.tEnd;
+    return tLast < tCap ? tLast : tCap;
+}
+
+// Block (l, c, j0, pb) of 3 L jb maxPb takes samples j = j0, j0 + jb, ...
+__global__ void __launch_bounds__(256)
+probe_chunks(const BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k, int span,
+             double* __restrict__ parts, int maxPb, int maxSamples, int jb) {
+    __shared__ double lds[4];
+    const int pb = (int)(blockIdx.x % (uint32_t)maxPb);
+    const int j0 = (int)((blockIdx.x / (uint32_t)maxPb) % (uint32_t)jb);
+    const int lc = (int)(blockIdx.x / ((uint32_t)maxPb * (uint32_t)jb));
+    const int l = lc / 3, c = lc % 3;
+    if (l >= L) return;
+    const BatchLat& B = lats[l];
+    if (B.fCount <= 0 || (uint32_t)pb >= B.Pb) return;
+    const int64_t t0 = B.step0 + *base + k, tLast = span_end(B, t0, span), tFirst = first_sample(B, t0);
+    const uint32_t lo = (uint32_t)pb * B.chunk;
+    const uint32_t hi = (B.M - lo) > B.chunk ? lo + B.chunk : B.M;
+    for (int j = j0; j < maxSamples; j += jb) {
+        const int64_t t = tFirst + (int64_t)j * B.fPeriod;
+        if (t >= tLast) return;
+        const BoxForceElem e{force_slot(B, t), (uint32_t)B.p.nx, (uint32_t)B.p.ny, B.bx0, B.by0, B.fbw, B.fbh, c};
+        const double sum = block_tree_sum(chunk_sum(e, 0u, lo, hi), lds);
+        if (threadIdx.x == 0) parts[((size_t)lc * maxSamples + j) * maxPb + pb] = sum;
+    }
+}
+
+// The chunk sums of one sample, read in stage 2's order.
+struct GlobalPartials {
     const double* p;
     __device__ double operator()(uint32_t, uint32_t k) const { return p[k]; }
 };
 
+// Block (l, c, j) of 3 L maxSamples.
 __global__ void __launch_bounds__(256)
-probe_batch(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k, int span) {
+probe_totals(BatchLat* __restrict__ lats, int L, const int64_t* __restrict__ base, int k, int span,
+             const double* __restrict__ parts, int maxPb, int maxSamples) {
     __shared__ double lds[4];
-    __shared__ double part[kBatchMaxChunks];
-    const int l = blockIdx.x / 3, c = blockIdx.x % 3;
+    const int j = (int)(blockIdx.x % (uint32_t)maxSamples);
+    const int lc = (int)(blockIdx.x / (uint32_t)maxSamples);
+    const int l = lc / 3, c = lc % 3;
     if (l >= L) return;
     const BatchLat& B = lats[l];
     if (B.fCount <= 0) return;
-    const int64_t t0 = B.step0 + *base + k;
-    int64_t tLast = t0 + span;
-    const int64_t tCap = B.fT0 + B.fCount * B.fPeriod;        // one past the last sample's step
-    tLast = tLast < B.tEnd ? tLast : B.tEnd;
-    tLast = tLast < tCap ? tLast : tCap;
-    // first sample step at or after t0
-    int64_t t = t0 <= B.fT0 ? B.fT0 : B.fT0 + ((t0 - B.fT0 + B.fPeriod - 1) / B.fPeriod) * B.fPeriod;
-    for (; t < tLast; t += B.fPeriod) {
-        const BoxForceElem e{force_slot(B, t), (uint32_t)B.p.nx, (uint32_t)B.p.ny, B.bx0, B.by0, B.fbw, B.fbh, c};
-        for (uint32_t pb = 0; pb < B.Pb; pb++) {
-            const uint32_t lo = pb * B.chunk;
-            const uint32_t hi = (B.M - lo) > B.chunk ? lo + B.chunk : B.M;
-            const double sum = block_tree_sum(chunk_sum(e, 0u, lo, hi), lds);
-            if (threadIdx.x == 0) part[pb] = sum;
-            __syncthreads();
-        }
-        const double total = block_tree_sum(chunk_sum(SharedPartials{part}, 0u, 0u, B.Pb), lds);
-        if (threadIdx.x == 0) B.series[3 * ((t - B.fT0) / B.fPeriod) + c] = total;
-        __syncthreads();
-    }
+    const int64_t t0 = B.step0 + *base + k, tLast = span_end(B, t0, span);
+    const int64_t t = first_sample(B, t0) + (int64_t)j * B.fPeriod;
+    if (t >= tLast) return;
+    const GlobalPartials g{parts + ((size_t)lc * maxSamples + j) * maxPb};
+    const double total = block_tree_sum(chunk_sum(g, 0u, 0u, B.Pb), lds);
+    if (threadIdx.x == 0) B.series[3 * ((t - B.fT0) / B.fPeriod) + c] = total;
 }
 
 __global__ void advance_base(int64_t* base, int64_t by) {
@@ -157,8 +182,18 @@ hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool f
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s) {
-    hipLaunchKernelGGL(probe_batch, dim3(3 * L), dim3(256), 0, s, lats, L, base, k, span);
+hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, double* parts,
+                              int maxPb, int maxSamples, hipStream_t s) {
+    if (L <= 0 || maxPb <= 0 || maxSamples <= 0 || !parts) return hipErrorInvalidValue;
+    // at most about 2^20 blocks of chunk sums; each takes every jb-th sample
+    int jb = maxSamples;
+    while (jb > 1 && (int64_t)3 * L * jb * maxPb > ((int64_t)1 << 20)) jb = (jb + 1) / 2;
+    if ((int64_t)3 * L * jb * maxPb > ((int64_t)1 << 30) || (int64_t)3 * L * maxSamples > ((int64_t)1 << 30))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(probe_chunks, dim3((uint32_t)(3 * L * jb * maxPb)), dim3(256), 0, s, lats, L, base, k, span,
+                       parts, maxPb, maxSamples, jb);
+    hipLaunchKernelGGL(probe_totals, dim3((uint32_t)(3 * L * maxSamples)), dim3(256), 0, s, lats, L, base, k, span,
+                       parts, maxPb, maxSamples);
     return hipGetLastError();
 }
 

@@ -162,7 +162,10 @@ hipError_t launch_batch_step(int precision, int bs, int parity, bool les, bool f
 // lattice's ring of force buffers for the longer span would be too large.
 constexpr int kProbeSpan = 256;
 constexpr int kProbeSpanSmall = 64;
-hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, hipStream_t s);
+// parts: 3 L maxSamples maxPb doubles (maxPb: the largest Pb of the lattices,
+// maxSamples: the most force steps any of them has in one span).
+hipError_t launch_batch_probe(BatchLat* lats, int L, const int64_t* base, int k, int span, double* parts,
+                              int maxPb, int maxSamples, hipStream_t s);
 hipError_t launch_advance_base(int64_t* base, int64_t by, hipStream_t s);
 // Steps k0 .. k0 + K - 1 (K <= kProbeSpan) of every tile of one resident group;
 // lds is the largest tile's (w + 2)(h + 2) 19 sizeof(T) bytes.

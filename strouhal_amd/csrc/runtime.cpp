@@ -953,6 +953,11 @@ struct strh_batch {
     uint64_t residentSpin = kResidentSpinTicks;   // wall-clock ticks a wave waits before aborting
     ResTile* tiles = nullptr;            // every resident group's tiles of one run (device)
     size_t tilesCap = 0;
+    // the probe's chunk sums: 3 lattices x probeSamples x probePb doubles (device)
+    double* probeParts = nullptr;
+    size_t probePartsCap = 0;
+    int probePb = 0;                     // most chunks of any probe box
+    int probeSamples = 0;                // most force samples of any lattice in one span
 };
 
 namespace {
@@ -1169,7 +1174,10 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
                 HIP_TRY(launch_resident(B->precision, g.les, g.forced, sg.dev, B->tiles + g.first, g.count, g.bs, g.lds,
                                         B->base, kOff + j, K, B->abortFlag, B->residentSpin, s),
                         "resident launch");
-            if (sg.probes) HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, kOff + j, K, s), "batch probe launch");
+            if (sg.probes)
+                HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, kOff + j, K, B->probeParts, B->probePb,
+                                           B->probeSamples, s),
+                        "batch probe launch");
         }
         return STRH_OK;
     }
@@ -1180,7 +1188,9 @@ int batch_launch_steps(strh_batch* B, const Segment& sg, int bs, int count, int 
         HIP_TRY(launch_batch_step(B->precision, bs, par, sg.les, sg.forced, sg.planar, sg.dev, sg.L, sg.blocks, B->base, k, s),
                 "batch step launch");
         if (sg.probes && (k + 1 - k0 == span || j == count - 1)) {
-            HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k0, k + 1 - k0, s), "batch probe launch");
+            HIP_TRY(launch_batch_probe(sg.dev, sg.L, B->base, k0, k + 1 - k0, B->probeParts, B->probePb,
+                                       B->probeSamples, s),
+                    "batch probe launch");
             k0 = k + 1;
         }
     }
@@ -1334,6 +1344,27 @@ int batch_execute(strh_batch* B, const int64_t* steps, bool useGraph, bool probe
         HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->ctx->device), "CU count");
         B->residentSlots = std::max(0, cus * perCU);
         if (B->residentSlots == 0) return fail(STRH_ERR_UNSUPPORTED, "the resident kernel does not fit on a CU");
+    }
+    // scratch for the probes' chunk sums, sized for every lattice's box and period
+    if (probes) {
+        const int span = batch_span(B);
+        int pb = 0, ns = 0;
+        for (int i = 0; i < n; i++) {
+            const BatchLat& d = B->host[(size_t)i];
+            if (d.fCount <= 0) continue;
+            pb = std::max(pb, (int)d.Pb);
+            ns = std::max(ns, (int)((span + d.fPeriod - 1) / d.fPeriod));
+        }
+        const size_t need = (size_t)3 * (size_t)n * (size_t)std::max(pb, 1) * (size_t)std::max(ns, 1);
+        if (need > B->probePartsCap) {
+            if (B->probeParts) hipFree(B->probeParts);
+            B->probeParts = nullptr;
+            B->probePartsCap = 0;
+            HIP_TRY(hipMalloc(&B->probeParts, need * sizeof(double)), "hipMalloc(probe chunk sums)");
+            B->probePartsCap = need;
+        }
+        B->probePb = std::max(pb, 1);
+        B->probeSamples = std::max(ns, 1);
     }
     // compact descriptor arrays of every segment, uploaded once
     std::vector<Segment> segs;
@@ -1561,6 +1592,7 @@ int strh_batch_destroy(strh_batch* B) {
     for (void* p : B->fring) if (p) hipFree(p);
     for (uint64_t* p : B->xchg) if (p) hipFree(p);
     if (B->tiles) hipFree(B->tiles);
+    if (B->probeParts) hipFree(B->probeParts);
     for (hipEvent_t ev : B->trialEv) if (ev) hipEventDestroy(ev);
     if (B->abortFlag) hipFree(B->abortFlag);
     if (B->dev) hipFree(B->dev);
